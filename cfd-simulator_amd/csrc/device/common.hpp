@@ -1,0 +1,131 @@
+// common.hpp - device helpers shared by the SIMPLE kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+
+namespace cfdgpu {
+
+// std::max / std::min / std::clamp with the standard's exact NaN and signed
+// zero behaviour (the reference uses these, not fmax/fmin).
+__device__ __forceinline__ double mx(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double mn(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+  return (v < lo) ? lo : ((hi < v) ? hi : v);
+}
+__device__ __forceinline__ float mxf(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ bool imposes(const FaceBC& bc, double F) {
+  // SimpleSolver::imposesVelocity (SimpleSolver.cpp:247-262)
+  return bc.kind == kWall || bc.kind == kInlet || (bc.kind == kFarField && F < bc.dead);
+}
+
+// Blocks are handed to the 8 XCDs round-robin; remap so each XCD works on a
+// contiguous run of tiles and the j-1 / j+1 halo rows it re-reads stay in its
+// own L2.
+__device__ __forceinline__ int xcdTile(int b, int nb) {
+  constexpr int kXcd = 8;
+  if (nb < 2 * kXcd) return b;
+  const int per = nb / kXcd;
+  const int rem = nb - per * kXcd;
+  const int xcd = b % kXcd;
+  const int k = b / kXcd;
+  // XCD x owns tiles [start(x), start(x) + count(x)); the first `rem` XCDs own one extra.
+  const int start = xcd * per + (xcd < rem ? xcd : rem);
+  const int count = per + (xcd < rem ? 1 : 0);
+  return (k < count) ? start + k : -1;
+}
+
+// Deterministic block reduction of NV sums over a 256-thread block (4 waves of
+// 64): wave shuffles, then the 4 wave results added in wave order by thread 0.
+template <int NV>
+__device__ __forceinline__ void blockSum(double (&v)[NV], double* sm) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] += __shfl_down(v[q], o, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sm[q * 4 + wave] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = ((sm[q * 4 + 0] + sm[q * 4 + 1]) + sm[q * 4 + 2]) + sm[q * 4 + 3];
+  }
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void blockMax(double (&v)[NV], double* sm) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = mx(v[q], __shfl_down(v[q], o, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sm[q * 4 + wave] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = mx(mx(mx(sm[q * 4 + 0], sm[q * 4 + 1]), sm[q * 4 + 2]), sm[q * 4 + 3]);
+  }
+  __syncthreads();
+}
+
+// Publish this block's partials, count arrivals; true in every thread of the
+// block that arrives last (which then owns the final reduction). The counter
+// is reset by the last block, so the next kernel on the same stream reuses it.
+__device__ __forceinline__ bool arriveLast(unsigned* counter, unsigned total) {
+  __shared__ unsigned sLast;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    sLast = (prev == total - 1u) ? 1u : 0u;
+    if (sLast) {
+      __threadfence();
+      *counter = 0u;
+    }
+  }
+  __syncthreads();
+  const bool last = sLast != 0u;
+  if (last) __threadfence();
+  return last;
+}
+
+// Sum partials[0..n) of one slot with wave 0 in a fixed order; result in lane 0.
+__device__ __forceinline__ double waveSumPartials(const double* p, int n) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n; b += 64) s += p[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  return s;
+}
+
+__device__ __forceinline__ double waveMaxPartials(const double* p, int n) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n; b += 64) s = mx(s, p[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = mx(s, __shfl_down(s, o, 64));
+  return s;
+}
+
+__device__ __forceinline__ double waveMinPartials(const double* p, int n, double init) {
+  double s = init;
+  for (int b = threadIdx.x; b < n; b += 64) s = mn(s, p[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = mn(s, __shfl_down(s, o, 64));
+  return s;
+}
+
+__device__ __forceinline__ double* partialSlot(const Inst& I, int slot) {
+  return I.partials + static_cast<size_t>(slot) * kMaxRedBlocks;
+}
+
+}  // namespace cfdgpu

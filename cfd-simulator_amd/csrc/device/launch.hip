@@ -1,0 +1,99 @@
+// launch.hip - the launch schedule of one SIMPLE outer iteration
+// (SimpleSolver.cpp:662-794), for a batch of instances sharing one descriptor
+// array (grid.y = instance).
+#include <hip/hip_runtime.h>
+
+#include "../host/engine.hpp"
+#include "layout.hpp"
+
+namespace cfdgpu {
+
+__global__ void k_begin_iteration(const Inst*, int);
+__global__ void k_grad(const Inst*);
+__global__ void k_mom(const Inst*);
+__global__ void k_gs_lex(const Inst*, int, int);
+__global__ void k_gs_color(const Inst*, int, int, int);
+__global__ void k_rhiechow(const Inst*);
+__global__ void k_pressure(const Inst*, int);
+__global__ void k_cg_init(const Inst*);
+__global__ void k_cg_a(const Inst*, int);
+__global__ void k_cg_b(const Inst*, int);
+__global__ void k_corr_mean(const Inst*);
+__global__ void k_correct_cells(const Inst*);
+__global__ void k_correct_faces(const Inst*);
+__global__ void k_divergence(const Inst*, double*);
+__global__ void k_monitor(const Inst*, int);
+__global__ void k_sst_blend(const Inst*);
+__global__ void k_sst_asm(const Inst*);
+__global__ void k_sst_wall(const Inst*);
+__global__ void k_sst_mut(const Inst*);
+
+int gsThreadsFor(int nJ) {
+  int t = ((nJ + 63) / 64) * 64;
+  if (t < 64) t = 64;
+  if (t > 1024) t = 1024;
+  return t;
+}
+
+size_t gsLdsFor(int nJ, int nI, int threads) {
+  (void)nJ;
+  return (static_cast<size_t>(2) * 2 * static_cast<size_t>(threads) + 2 * static_cast<size_t>(nI)) * sizeof(double);
+}
+
+void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
+  const dim3 block(kBlock);
+  const dim3 one((cfg.batch + 63) / 64);
+  const bool lex = h.gsOrder == 0;
+  hipLaunchKernelGGL(k_begin_iteration, one, dim3(64), 0, s, d, cfg.batch);
+  hipLaunchKernelGGL(k_grad, grid, block, 0, s, d);
+  hipLaunchKernelGGL(k_mom, grid, block, 0, s, d);
+  if (lex) {
+    hipLaunchKernelGGL(k_gs_lex, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
+  } else {
+    for (int sw = 0; sw < h.momSweeps; ++sw) {
+      hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 0, 0);
+      hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 1, 0);
+    }
+  }
+  hipLaunchKernelGGL(k_rhiechow, grid, block, 0, s, d);
+  // SimpleSolver.cpp:716-748
+  const int passes = h.nonOrthPasses < 1 ? 1 : h.nonOrthPasses;
+  for (int pass = 0; pass < passes; ++pass) {
+    hipLaunchKernelGGL(k_pressure, grid, block, 0, s, d, pass);
+    hipLaunchKernelGGL(k_cg_init, grid, block, 0, s, d);
+    for (int it = 0; it < h.cgMax; ++it) {
+      hipLaunchKernelGGL(k_cg_a, grid, block, 0, s, d, it);
+      hipLaunchKernelGGL(k_cg_b, grid, block, 0, s, d, it);
+    }
+    if (!h.hasPRef) hipLaunchKernelGGL(k_corr_mean, grid, block, 0, s, d);
+    hipLaunchKernelGGL(k_correct_cells, grid, block, 0, s, d);
+    hipLaunchKernelGGL(k_correct_faces, grid, block, 0, s, d);
+  }
+  hipLaunchKernelGGL(k_divergence, grid, block, 0, s, d, static_cast<double*>(nullptr));
+  if (h.model == 2) {
+    hipLaunchKernelGGL(k_sst_blend, grid, block, 0, s, d);
+    hipLaunchKernelGGL(k_sst_asm, grid, block, 0, s, d);
+    if (lex) {
+      hipLaunchKernelGGL(k_gs_lex, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
+    } else {
+      for (int sw = 0; sw < 3; ++sw) {
+        const int floor = (sw == 2) ? 1 : 0;
+        hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 1, 0, 0);
+        hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 1, 1, floor);
+        hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 2, 0, 0);
+        hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 2, 1, floor);
+      }
+    }
+    hipLaunchKernelGGL(k_sst_wall, dim3(1, cfg.batch), block, 0, s, d);
+    hipLaunchKernelGGL(k_sst_mut, grid, block, 0, s, d);
+  }
+  hipLaunchKernelGGL(k_monitor, one, dim3(64), 0, s, d, cfg.batch);
+}
+
+void launchDivergence(const Inst* d, const LaunchCfg& cfg, double* out, hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
+  hipLaunchKernelGGL(k_divergence, grid, dim3(kBlock), 0, s, d, out);
+}
+
+}  // namespace cfdgpu

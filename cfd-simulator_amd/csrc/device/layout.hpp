@@ -1,0 +1,162 @@
+// layout.hpp - device data layout of one SIMPLE instance on a structured C-grid.
+//
+// Cells are i-major: c = j*nI + i (nI cells streamwise, nJ wall-normal), the
+// reference's own numbering (include/cfd/mesh/Mesh.hpp:167-168). Faces come in
+// two families, both row-major so a wavefront of 64 lanes walking i reads
+// contiguous memory:
+//   j-faces (constant j, between rows j-1 and j): index j*nI + i, j in [0, nJ]
+//   i-faces (constant i, between columns i-1 and i): index j*(nI+1) + i, i in [0, nI]
+// The reference stores the i-family column-major (src/mesh/Mesh.cpp:153-156);
+// the host repacks it on upload and back on download.
+//
+// Boundary data lives on the four edges only: bottom (j = 0, nI faces, which
+// holds the wall and both sides of the wake cut), top (j = nJ), left (i = 0)
+// and right (i = nI). A wake-cut face in the bottom row couples cell (i, 0) to
+// cell (partner[i], 0) (src/mesh/CGrid.cpp:370-381).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cfdgpu {
+
+// SimpleSolver::FaceGeometry (include/cfd/solver/SimpleSolver.hpp:199-205),
+// precomputed on the host in FP64 exactly as SimpleSolver.cpp:82-131 does.
+struct FaceGeo {
+  double Sx, Sy;  // S = n |S|, out of the owner
+  double Tx, Ty;  // T = S - g d, the non-orthogonal remainder
+  double g;       // |S|^2 / (S.d)
+  double w;       // owner interpolation weight
+};
+
+// Per boundary face: the FaceCondition (BoundaryConditions.hpp:101-109) plus
+// the frozen far-field dead band (SimpleSolver.cpp:150-166) and the k-omega
+// SST geometric factor |S|^2/|S.d| (KOmegaSST.cpp:423-427).
+struct FaceBC {
+  double u, v, p;
+  double dead;
+  double gSst;
+  int kind;  // flow::BoundaryKind
+  int pad;
+};
+
+enum BKind { kInternal = 0, kInlet = 1, kOutlet = 2, kFarField = 3, kWall = 4 };
+
+// Reduction slots of one instance (FP64 scalars in device memory).
+enum Slot {
+  sVelScale = 0,   // max |u|
+  sMomResX,        // sum |b - A u|
+  sMomResY,
+  sMomDiagL1,      // sum |a_P|
+  sNetFluxL1,      // sum |netFlux|
+  sSourceSum,      // sum b (mean removal when unanchored)
+  sCgNorm0,        // CG: initial L1 residual
+  sCgNorm,         // CG: current L1 residual
+  sCgRz,           // CG: r.z
+  sCgAlpha,
+  sCgBeta,
+  sCgPAp,
+  sCorrMean,       // mean of the pressure correction (unanchored)
+  sMaxDiv,         // max |div|
+  sTurbResK,       // SST residual numerators and denominators
+  sTurbDiagK,
+  sTurbResW,
+  sTurbDiagW,
+  sMinK, sMaxK, sMinW, sMaxW, sMinMut, sMaxMut,
+  sMaxEddyRatio,
+  sMaxYPlus,
+  sRefineNorm,     // mixed path: FP64 residual norm of the pressure correction
+  kSlots
+};
+
+// Integer state of one instance.
+enum IFlag {
+  fCgDone = 0,     // CG finished (converged or broke down)
+  fCgIters,        // iterations of the current CG solve
+  fCgTotal,        // iterations over all passes of this outer iteration
+  fStopped,        // solve finished: later launches for this instance are no-ops
+  fIteration,      // completed outer iterations
+  fStatus,         // 0 running/converged, 1 iteration-limit, 2 diverged
+  kIFlags
+};
+
+// Monitor written by the device each outer iteration (SolverMonitor,
+// include/cfd/solver/SimpleSolver.hpp:135-154).
+struct DevMonitor {
+  double continuity, momentumX, momentumY;
+  double massImbalance, maxDivergence;
+  int pressureIterations;
+  int pad;
+  double turbFirst, turbSecond;
+  double minK, maxK, minW, maxW, minMut, maxMut;
+  double maxEddyRatio;
+};
+
+constexpr int kMaxRedBlocks = 4096;  // partial-sum capacity per slot
+constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
+
+struct Inst {
+  int nI, nJ, N;
+  int hasPRef;     // a boundary sets the pressure level (SimpleSolver.cpp:71-77)
+  int scheme;      // 0 upwind, 1 second-order upwind (deferred correction)
+  int model;       // 0 none, 1 laminar, 2 SST
+  double alphaU, alphaP, blend, alphaT;
+  int momSweeps, cgMax;
+  double cgTol;
+  int nonOrthPasses;  // max(1, nonOrthogonalCorrectors)
+  int gsOrder;        // 0 lexicographic (exact), 1 two-colour
+  int precision;      // 0 FP64, 1 mixed
+  int psolve;         // 0 Jacobi-PCG, 1 line-preconditioned CG
+  double refMassFlow;
+  double stopTol;  // device-side stop rule (worst() < stopTol); <= 0 disables
+  int stopMaxIter;
+  int nBlocks;     // blocks of a cell-parallel launch for this instance
+
+  // geometry (read-only)
+  const double* V;
+  const FaceGeo* jg;
+  const FaceGeo* ig;
+  const int* partner;  // nI entries: partner cell in row 0, or -1
+  const FaceBC* bcB;   // nI
+  const FaceBC* bcT;   // nI
+  const FaceBC* bcL;   // nJ
+  const FaceBC* bcR;   // nJ
+  const double* wallDist;
+  const double* jgSst;  // |g| for SST on j-faces (interior and wake)
+  const double* igSst;
+
+  // flow state (FP64)
+  double *u, *v, *p, *rho, *mu, *muMol;
+  double *Fj, *Fi;
+  double2 *gU, *gV, *gP;
+
+  // momentum system (cell-centric rows; aP is the relaxed diagonal = momentumDiagonal_)
+  double *aP, *aS, *aN, *aW, *aE, *bx, *by;
+  // pressure system: diagonal, coefficient of the above / right face, wake coefficient
+  double *pP, *pN, *pE, *pWk, *pb;
+  // CG vectors
+  double *x, *r, *z, *pd0, *pd1, *Ap;
+
+  // k-omega SST state and systems
+  double *k, *w, *mut, *f1, *f2, *cd, *strain, *prod;
+  double2 *gK, *gW;
+  double *kP, *kS, *kN, *kW, *kE, *kB;
+  double *wP, *wS, *wN, *wW, *wE, *wB;
+  double kIn, wIn;
+  double sigK1, sigW1, beta1, sigK2, sigW2, beta2, betaStar, kappa, a1, prodLimit;
+  double gamma1, gamma2, betaStarQ;  // gamma1(), gamma2(), betaStar^(1/4) (host-computed)
+
+  // mixed-precision mirrors (FP32 bulk)
+  float *faP, *faS, *faN, *faW, *faE, *fbx, *fby, *fdx, *fdy;
+  float *fpP, *fpN, *fpE, *fpWk, *fx, *fr, *fz, *fpd0, *fpd1, *fAp, *fb;
+
+  // reductions
+  double* red;        // kSlots scalars
+  double* partials;   // kSlots * kMaxRedBlocks
+  unsigned* counter;  // one arrival counter per slot group
+  int* flags;         // kIFlags
+  DevMonitor* monitor;
+  DevMonitor* history;  // optional ring of monitors, one per outer iteration
+  int historyCap;
+};
+
+}  // namespace cfdgpu

@@ -1,0 +1,1034 @@
+// simple_fp64.hip - FP64 kernels of one SIMPLE outer iteration on a structured
+// C-grid (reference: src/solver/SimpleSolver.cpp:662-794).
+//
+// Every kernel is cell-centric: one lane per cell gathers the contributions
+// of its four faces (below, above, left, right) in the order the reference's
+// face loop scatters them (faces are visited by index: the j-family first,
+// then the i-family, so a cell sees below, above, left, right). With the same
+// per-face arithmetic and FP contraction disabled, each cell's sums are the
+// reference's sums bit for bit; only the global reductions (L1 norms, dot
+// products) are re-associated, as tree sums.
+//
+// Grid: blockIdx.y = instance of a batch, blockIdx.x = tile of 256 cells.
+// Every kernel first checks the instance's stop flag, so a batch can keep
+// launching while finished instances idle.
+#include "common.hpp"
+
+namespace cfdgpu {
+
+#define CFD_PROLOGUE                                          \
+  const Inst& I = insts[blockIdx.y];                          \
+  if (static_cast<int>(blockIdx.x) >= I.nBlocks) return;     \
+  if (I.flags[fStopped]) return;                              \
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.nBlocks); \
+  const int c = tile * kBlock + static_cast<int>(threadIdx.x); \
+  const bool live = c < I.N;                                  \
+  const int nI = I.nI;                                        \
+  const int nJ = I.nJ;                                        \
+  const int i = live ? c % nI : 0;                            \
+  const int j = live ? c / nI : 0;                            \
+  (void)i; (void)j; (void)nJ;
+
+// ---------------------------------------------------------------------------
+// Face interpolation + Green-Gauss gradients of u, v, p
+// (SimpleSolver.cpp:268-317, Gradient.cpp:20-50) and the velocity scale
+// max |u| (SimpleSolver.cpp:677-683).
+// ---------------------------------------------------------------------------
+
+struct Face3 {
+  double u, v, p;
+};
+
+__device__ __forceinline__ Face3 faceValues(const Inst& I, int c, int o, int n, const FaceGeo& f) {
+  const double w = f.w;
+  (void)c;
+  return Face3{w * I.u[o] + (1.0 - w) * I.u[n], w * I.v[o] + (1.0 - w) * I.v[n],
+               w * I.p[o] + (1.0 - w) * I.p[n]};
+}
+
+__device__ __forceinline__ Face3 boundaryValues(const Inst& I, int c, const FaceBC& bc, double F) {
+  if (imposes(bc, F)) return Face3{bc.u, bc.v, I.p[c]};
+  return Face3{I.u[c], I.v[c], bc.p};
+}
+
+__global__ void __launch_bounds__(kBlock) k_grad(const Inst* insts) {
+  CFD_PROLOGUE
+  __shared__ double sm[16];
+  double vmax[1] = {0.0};
+  if (live) {
+    double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0, px = 0.0, py = 0.0;
+    auto add = [&](const FaceGeo& f, const Face3& q, bool plus) {
+      if (plus) {
+        ux += f.Sx * q.u; uy += f.Sy * q.u;
+        vx += f.Sx * q.v; vy += f.Sy * q.v;
+        px += f.Sx * q.p; py += f.Sy * q.p;
+      } else {
+        ux -= f.Sx * q.u; uy -= f.Sy * q.u;
+        vx -= f.Sx * q.v; vy -= f.Sy * q.v;
+        px -= f.Sx * q.p; py -= f.Sy * q.p;
+      }
+    };
+    // below: j-face (i, j)
+    {
+      const FaceGeo& f = I.jg[j * nI + i];
+      if (j > 0) {
+        add(f, faceValues(I, c, c - nI, c, f), false);
+      } else {
+        const int pr = I.partner[i];
+        if (pr >= 0) add(f, faceValues(I, c, c, pr, f), true);
+        else add(f, boundaryValues(I, c, I.bcB[i], I.Fj[i]), true);
+      }
+    }
+    // above: j-face (i, j+1), owned by this cell
+    {
+      const int fi = (j + 1) * nI + i;
+      const FaceGeo& f = I.jg[fi];
+      if (j < nJ - 1) add(f, faceValues(I, c, c, c + nI, f), true);
+      else add(f, boundaryValues(I, c, I.bcT[i], I.Fj[fi]), true);
+    }
+    // left: i-face (i, j)
+    {
+      const int fi = j * (nI + 1) + i;
+      const FaceGeo& f = I.ig[fi];
+      if (i > 0) add(f, faceValues(I, c, c - 1, c, f), false);
+      else add(f, boundaryValues(I, c, I.bcL[j], I.Fi[fi]), true);
+    }
+    // right: i-face (i+1, j), owned by this cell
+    {
+      const int fi = j * (nI + 1) + i + 1;
+      const FaceGeo& f = I.ig[fi];
+      if (i < nI - 1) add(f, faceValues(I, c, c, c + 1, f), true);
+      else add(f, boundaryValues(I, c, I.bcR[j], I.Fi[fi]), true);
+    }
+    const double vol = I.V[c];
+    if (vol > 0.0) {
+      const double inv = 1.0 / vol;
+      ux *= inv; uy *= inv; vx *= inv; vy *= inv; px *= inv; py *= inv;
+    }
+    I.gU[c] = make_double2(ux, uy);
+    I.gV[c] = make_double2(vx, vy);
+    I.gP[c] = make_double2(px, py);
+    vmax[0] = mx(0.0, hypot(I.u[c], I.v[c]));
+  }
+  blockMax<1>(vmax, sm);
+  if (threadIdx.x == 0) partialSlot(I, sVelScale)[blockIdx.x] = vmax[0];
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double s = waveMaxPartials(partialSlot(I, sVelScale), I.nBlocks);
+      if (threadIdx.x == 0) I.red[sVelScale] = (s > 0.0) ? s : 1.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum assembly for both components + implicit relaxation + the L1
+// residual measured before the sweeps (SimpleSolver.cpp:319-443, 685-696).
+// ---------------------------------------------------------------------------
+
+struct MomRow {
+  double d, bx, by;
+};
+
+// An interior face seen from one of its two cells.
+__device__ __forceinline__ double momInterior(const Inst& I, const FaceGeo& f, double F, int o, int n,
+                                              bool cellIsOwner, MomRow& row) {
+  const double w = f.w;
+  const double visc = w * I.mu[o] + (1.0 - w) * I.mu[n];
+  const double diff = visc * f.g;
+  const double outward = mx(F, 0.0);
+  const double inward = mx(-F, 0.0);
+  double off;
+  if (cellIsOwner) {
+    row.d += outward + diff;
+    off = -inward - diff;
+  } else {
+    row.d += inward + diff;
+    off = -outward - diff;
+  }
+  const double2 guo = I.gU[o], gun = I.gU[n], gvo = I.gV[o], gvn = I.gV[n];
+  const double fux = guo.x * w + gun.x * (1.0 - w), fuy = guo.y * w + gun.y * (1.0 - w);
+  const double fvx = gvo.x * w + gvn.x * (1.0 - w), fvy = gvo.y * w + gvn.y * (1.0 - w);
+  const double cu = I.blend * visc * (fux * f.Tx + fuy * f.Ty);
+  const double cv = I.blend * visc * (fvx * f.Tx + fvy * f.Ty);
+  if (cellIsOwner) { row.bx += cu; row.by += cv; }
+  else { row.bx -= cu; row.by -= cv; }
+  if (I.scheme == 1) {
+    const double uo = I.u[o], un = I.u[n], vo = I.v[o], vn = I.v[n];
+    const double upU = (F >= 0.0) ? uo : un;
+    const double upV = (F >= 0.0) ? vo : vn;
+    const double cenU = w * uo + (1.0 - w) * un;
+    const double cenV = w * vo + (1.0 - w) * vn;
+    const double defU = F * (cenU - upU);
+    const double defV = F * (cenV - upV);
+    if (cellIsOwner) { row.bx -= defU; row.by -= defV; }
+    else { row.bx += defU; row.by += defV; }
+  }
+  return off;
+}
+
+// A wake-cut face, owned by cell c; `a` is the cell across the cut.
+__device__ __forceinline__ double momWake(const Inst& I, const FaceGeo& f, double F, int c, int a, MomRow& row) {
+  const double w = f.w;
+  const double visc = w * I.mu[c] + (1.0 - w) * I.mu[a];
+  const double diff = visc * f.g;
+  const double outward = mx(F, 0.0);
+  const double inward = mx(-F, 0.0);
+  row.d += outward + diff;
+  const double off = -inward - diff;
+  const double2 guo = I.gU[c], gun = I.gU[a], gvo = I.gV[c], gvn = I.gV[a];
+  const double fux = guo.x * w + gun.x * (1.0 - w), fuy = guo.y * w + gun.y * (1.0 - w);
+  const double fvx = gvo.x * w + gvn.x * (1.0 - w), fvy = gvo.y * w + gvn.y * (1.0 - w);
+  row.bx += I.blend * visc * (fux * f.Tx + fuy * f.Ty);
+  row.by += I.blend * visc * (fvx * f.Tx + fvy * f.Ty);
+  return off;
+}
+
+__device__ __forceinline__ void momBoundary(const Inst& I, const FaceGeo& f, const FaceBC& bc, double F, int c,
+                                            MomRow& row) {
+  const double visc = I.mu[c];
+  const double diff = visc * f.g;
+  if (imposes(bc, F)) {
+    const double2 gu = I.gU[c], gv = I.gV[c];
+    row.bx -= F * bc.u;
+    row.by -= F * bc.v;
+    row.d += diff;
+    row.bx += diff * bc.u;
+    row.by += diff * bc.v;
+    row.bx += I.blend * visc * (gu.x * f.Tx + gu.y * f.Ty);
+    row.by += I.blend * visc * (gv.x * f.Tx + gv.y * f.Ty);
+  } else {
+    row.d += F;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
+  CFD_PROLOGUE
+  __shared__ double sm[16];
+  double part[3] = {0.0, 0.0, 0.0};
+  if (live) {
+    MomRow row{0.0, 0.0, 0.0};
+    double aS = 0.0, aN = 0.0, aW = 0.0, aE = 0.0;
+    int mirror = -1;
+    // below
+    {
+      const FaceGeo& f = I.jg[j * nI + i];
+      const double F = I.Fj[j * nI + i];
+      if (j > 0) {
+        aS = momInterior(I, f, F, c - nI, c, false, row);
+      } else {
+        mirror = I.partner[i];
+        if (mirror >= 0) aS = momWake(I, f, F, c, mirror, row);
+        else momBoundary(I, f, I.bcB[i], F, c, row);
+      }
+    }
+    // above
+    {
+      const int fi = (j + 1) * nI + i;
+      const FaceGeo& f = I.jg[fi];
+      const double F = I.Fj[fi];
+      if (j < nJ - 1) aN = momInterior(I, f, F, c, c + nI, true, row);
+      else momBoundary(I, f, I.bcT[i], F, c, row);
+    }
+    // left
+    {
+      const int fi = j * (nI + 1) + i;
+      const FaceGeo& f = I.ig[fi];
+      const double F = I.Fi[fi];
+      if (i > 0) aW = momInterior(I, f, F, c - 1, c, false, row);
+      else momBoundary(I, f, I.bcL[j], F, c, row);
+    }
+    // right
+    {
+      const int fi = j * (nI + 1) + i + 1;
+      const FaceGeo& f = I.ig[fi];
+      const double F = I.Fi[fi];
+      if (i < nI - 1) aE = momInterior(I, f, F, c, c + 1, true, row);
+      else momBoundary(I, f, I.bcR[j], F, c, row);
+    }
+    const double2 gp = I.gP[c];
+    const double vol = I.V[c];
+    row.bx -= gp.x * vol;
+    row.by -= gp.y * vol;
+    // implicit under-relaxation (SimpleSolver.cpp:425-443)
+    const double uc = I.u[c], vc = I.v[c];
+    double d = row.d;
+    if (!(fabs(d) > 0.0)) {
+      d = 1.0;
+    } else {
+      const double relaxed = d / I.alphaU;
+      row.bx += (relaxed - d) * uc;
+      row.by += (relaxed - d) * vc;
+      d = relaxed;
+    }
+    I.aP[c] = d;
+    I.aS[c] = aS;
+    I.aN[c] = aN;
+    I.aW[c] = aW;
+    I.aE[c] = aE;
+    I.bx[c] = row.bx;
+    I.by[c] = row.by;
+    // residual with the starting field (LinearSystem.cpp:18-54, face order)
+    double axu = d * uc, axv = d * vc;
+    if (j > 0) { axu += aS * I.u[c - nI]; axv += aS * I.v[c - nI]; }
+    else if (mirror >= 0) { axu += aS * I.u[mirror]; axv += aS * I.v[mirror]; }
+    if (j < nJ - 1) { axu += aN * I.u[c + nI]; axv += aN * I.v[c + nI]; }
+    if (i > 0) { axu += aW * I.u[c - 1]; axv += aW * I.v[c - 1]; }
+    if (i < nI - 1) { axu += aE * I.u[c + 1]; axv += aE * I.v[c + 1]; }
+    part[0] = fabs(row.bx - axu);
+    part[1] = fabs(row.by - axv);
+    part[2] = fabs(d);
+  }
+  blockSum<3>(part, sm);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sMomResX)[blockIdx.x] = part[0];
+    partialSlot(I, sMomResY)[blockIdx.x] = part[1];
+    partialSlot(I, sMomDiagL1)[blockIdx.x] = part[2];
+  }
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double a = waveSumPartials(partialSlot(I, sMomResX), I.nBlocks);
+      const double b = waveSumPartials(partialSlot(I, sMomResY), I.nBlocks);
+      const double d = waveSumPartials(partialSlot(I, sMomDiagL1), I.nBlocks);
+      if (threadIdx.x == 0) {
+        I.red[sMomResX] = a;
+        I.red[sMomResY] = b;
+        I.red[sMomDiagL1] = d;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lexicographic Gauss-Seidel, exactly (LinearSystem.cpp:77-116).
+//
+// The reference sweeps cells in index order c = j*nI + i, in place. Cell (i,j)
+// then sees new values below and left and old values above and right. All
+// cells on one anti-diagonal i + j = k are independent given diagonal k-1, so
+// a sweep is nI + nJ - 1 dependent steps with lane j owning row j. The wake
+// cut couples (i,0) to (p,0) in the same row, which is walked in i order by
+// lane 0 - exactly the reference's order - so the schedule reproduces the
+// reference bit for bit for any partner map. Off-diagonal terms are summed in
+// the reference's cellFaces order: below, right, above, left (Mesh.cpp:206-213).
+//
+// One workgroup per system; NC components share one matrix (u and v).
+// ---------------------------------------------------------------------------
+
+struct GsSys {
+  const double *aP, *aS, *aN, *aW, *aE;
+  const double* b[2];
+  double* x[2];
+  double floorValue;  // applied after the sweeps when hasFloor
+  int hasFloor;
+};
+
+template <int NC>
+__device__ void gsLexBody(const Inst& I, const GsSys& S, int sweeps) {
+  extern __shared__ double lds[];
+  const int nI = I.nI, nJ = I.nJ;
+  const int R = static_cast<int>(blockDim.x);
+  double* buf = lds;                 // [2][NC][R]: value just written by each row
+  double* row0 = lds + 2 * NC * R;   // [NC][nI]: row j = 0, updated in place
+  const int jr = static_cast<int>(threadIdx.x);
+  const bool active = jr < nJ;
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    for (int t = jr; t < nI; t += R) {
+#pragma unroll
+      for (int q = 0; q < NC; ++q) row0[q * nI + t] = S.x[q][t];
+    }
+    __syncthreads();
+    double xw[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) xw[q] = 0.0;
+    const int steps = nI + nJ - 1;
+    for (int k = 0; k < steps; ++k) {
+      const int i = k - jr;
+      if (active && i >= 0 && i < nI) {
+        const int c = jr * nI + i;
+        const double d = S.aP[c];
+        double xn[NC];
+        if (fabs(d) > 0.0) {
+          double off[NC];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) off[q] = 0.0;
+          // below
+          if (jr > 0) {
+            const double a = S.aS[c];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) off[q] += a * buf[((k - 1) & 1) * NC * R + q * R + (jr - 1)];
+          } else {
+            const int pr = I.partner[i];
+            if (pr >= 0) {
+              const double a = S.aS[c];
+#pragma unroll
+              for (int q = 0; q < NC; ++q) off[q] += a * row0[q * nI + pr];
+            }
+          }
+          // right
+          if (i < nI - 1) {
+            const double a = S.aE[c];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) off[q] += a * ((jr == 0) ? row0[q * nI + i + 1] : S.x[q][c + 1]);
+          }
+          // above
+          if (jr < nJ - 1) {
+            const double a = S.aN[c];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][c + nI];
+          }
+          // left
+          if (i > 0) {
+            const double a = S.aW[c];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) off[q] += a * xw[q];
+          }
+#pragma unroll
+          for (int q = 0; q < NC; ++q) xn[q] = (S.b[q][c] - off[q]) / d;
+        } else {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) xn[q] = (jr == 0) ? row0[q * nI + i] : S.x[q][c];
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          S.x[q][c] = xn[q];
+          buf[(k & 1) * NC * R + q * R + jr] = xn[q];
+          xw[q] = xn[q];
+          if (jr == 0) row0[q * nI + i] = xn[q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (S.hasFloor) {
+    for (int c = jr; c < I.N; c += R) {
+#pragma unroll
+      for (int q = 0; q < NC; ++q) S.x[q][c] = mx(S.x[q][c], S.floorValue);
+    }
+  }
+}
+
+// system 0: momentum (u, v); system 1: k; system 2: omega. blockIdx.x picks
+// the system among [first, first + count).
+__global__ void __launch_bounds__(1024) k_gs_lex(const Inst* insts, int first, int sweeps) {
+  const Inst& I = insts[blockIdx.y];
+  if (I.flags[fStopped]) return;
+  const int sys = first + static_cast<int>(blockIdx.x);
+  GsSys S{};
+  if (sys == 0) {
+    S = GsSys{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
+    gsLexBody<2>(I, S, sweeps);
+  } else if (sys == 1) {
+    S = GsSys{I.kP, I.kS, I.kN, I.kW, I.kE, {I.kB, nullptr}, {I.k, nullptr}, 0.0, 1};
+    gsLexBody<1>(I, S, sweeps);
+  } else {
+    S = GsSys{I.wP, I.wS, I.wN, I.wW, I.wE, {I.wB, nullptr}, {I.w, nullptr}, 1.0e-10, 1};
+    gsLexBody<1>(I, S, sweeps);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-colour Gauss-Seidel: colour (i + j) & 1, one launch per colour. Same
+// fixed point as the lexicographic sweep, full parallelism. The C-grid has an
+// even nI, so a wake cell and its partner always differ in colour.
+// ---------------------------------------------------------------------------
+
+template <int NC>
+__device__ __forceinline__ void gsPoint(const Inst& I, const GsSys& S, int c, int i, int j) {
+  const int nI = I.nI, nJ = I.nJ;
+  const double d = S.aP[c];
+  if (!(fabs(d) > 0.0)) return;
+  double off[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) off[q] = 0.0;
+  if (j > 0) {
+    const double a = S.aS[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][c - nI];
+  } else {
+    const int pr = I.partner[i];
+    if (pr >= 0) {
+      const double a = S.aS[c];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][pr];
+    }
+  }
+  if (i < nI - 1) {
+    const double a = S.aE[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][c + 1];
+  }
+  if (j < nJ - 1) {
+    const double a = S.aN[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][c + nI];
+  }
+  if (i > 0) {
+    const double a = S.aW[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][c - 1];
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) S.x[q][c] = (S.b[q][c] - off[q]) / d;
+}
+
+__global__ void __launch_bounds__(kBlock) k_gs_color(const Inst* insts, int sys, int color, int applyFloor) {
+  CFD_PROLOGUE
+  if (!live) return;
+  if (sys == 0) {
+    GsSys S{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
+    if (((i + j) & 1) == color) gsPoint<2>(I, S, c, i, j);
+  } else if (sys == 1) {
+    GsSys S{I.kP, I.kS, I.kN, I.kW, I.kE, {I.kB, nullptr}, {I.k, nullptr}, 0.0, 1};
+    if (((i + j) & 1) == color) gsPoint<1>(I, S, c, i, j);
+    if (applyFloor) I.k[c] = mx(I.k[c], 0.0);
+  } else {
+    GsSys S{I.wP, I.wS, I.wN, I.wW, I.wE, {I.wB, nullptr}, {I.w, nullptr}, 1.0e-10, 1};
+    if (((i + j) & 1) == color) gsPoint<1>(I, S, c, i, j);
+    if (applyFloor) I.w[c] = mx(I.w[c], 1.0e-10);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rhie-Chow face fluxes (SimpleSolver.cpp:445-496). Cell (i,j) writes its
+// below and left faces; the last row / column also write the top / right.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double rcInterior(const Inst& I, const FaceGeo& f, int o, int n) {
+  const double w = f.w;
+  const double ux = I.u[o] * w + I.u[n] * (1.0 - w);
+  const double uy = I.v[o] * w + I.v[n] * (1.0 - w);
+  const double dens = w * I.rho[o] + (1.0 - w) * I.rho[n];
+  const double mob = w * (I.V[o] / I.aP[o]) + (1.0 - w) * (I.V[n] / I.aP[n]);
+  const double2 go = I.gP[o], gn = I.gP[n];
+  const double sgx = go.x * w + gn.x * (1.0 - w);
+  const double sgy = go.y * w + gn.y * (1.0 - w);
+  const double compact = (I.p[n] - I.p[o]) * f.g;
+  const double smooth = sgx * f.Sx + sgy * f.Sy;
+  return dens * (ux * f.Sx + uy * f.Sy) - dens * mob * (compact - smooth);
+}
+
+__device__ __forceinline__ double rcBoundary(const Inst& I, const FaceGeo& f, const FaceBC& bc, double Fold, int o) {
+  const double dens = I.rho[o];
+  if (imposes(bc, Fold)) return dens * (bc.u * f.Sx + bc.v * f.Sy);
+  const double mob = I.V[o] / I.aP[o];
+  const double compact = (bc.p - I.p[o]) * f.g;
+  const double2 g = I.gP[o];
+  const double smooth = g.x * f.Sx + g.y * f.Sy;
+  return dens * (I.u[o] * f.Sx + I.v[o] * f.Sy) - dens * mob * (compact - smooth);
+}
+
+__global__ void __launch_bounds__(kBlock) k_rhiechow(const Inst* insts) {
+  CFD_PROLOGUE
+  if (!live) return;
+  // below
+  {
+    const int fi = j * nI + i;
+    const FaceGeo& f = I.jg[fi];
+    if (j > 0) {
+      I.Fj[fi] = rcInterior(I, f, c - nI, c);
+    } else {
+      const int pr = I.partner[i];
+      if (pr >= 0) I.Fj[fi] = rcInterior(I, f, c, pr);
+      else I.Fj[fi] = rcBoundary(I, f, I.bcB[i], I.Fj[fi], c);
+    }
+  }
+  if (j == nJ - 1) {
+    const int fi = (j + 1) * nI + i;
+    I.Fj[fi] = rcBoundary(I, I.jg[fi], I.bcT[i], I.Fj[fi], c);
+  }
+  // left
+  {
+    const int fi = j * (nI + 1) + i;
+    const FaceGeo& f = I.ig[fi];
+    if (i > 0) I.Fi[fi] = rcInterior(I, f, c - 1, c);
+    else I.Fi[fi] = rcBoundary(I, f, I.bcL[j], I.Fi[fi], c);
+  }
+  if (i == nI - 1) {
+    const int fi = j * (nI + 1) + i + 1;
+    I.Fi[fi] = rcBoundary(I, I.ig[fi], I.bcR[j], I.Fi[fi], c);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pressure-correction assembly (SimpleSolver.cpp:498-580) and the continuity
+// residual of pass 0 (:720-727).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double presCoef(const Inst& I, const FaceGeo& f, int o, int n) {
+  const double w = f.w;
+  const double dens = w * I.rho[o] + (1.0 - w) * I.rho[n];
+  const double mob = w * (I.V[o] / I.aP[o]) + (1.0 - w) * (I.V[n] / I.aP[n]);
+  return dens * mob * f.g;
+}
+
+__device__ __forceinline__ void presBoundary(const Inst& I, const FaceGeo& f, const FaceBC& bc, double F, int o,
+                                             double& diag) {
+  if (imposes(bc, F)) return;
+  const double dens = I.rho[o];
+  const double mob = I.V[o] / I.aP[o];
+  diag += dens * mob * f.g;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass) {
+  CFD_PROLOGUE
+  __shared__ double sm[16];
+  double part[2] = {0.0, 0.0};
+  if (live) {
+    double net = 0.0, diag = 0.0, cN = 0.0, cE = 0.0, cWk = 0.0;
+    // below
+    {
+      const int fi = j * nI + i;
+      const FaceGeo& f = I.jg[fi];
+      const double F = I.Fj[fi];
+      if (j > 0) {
+        net -= F;
+        diag += presCoef(I, f, c - nI, c);
+      } else {
+        net += F;
+        const int pr = I.partner[i];
+        if (pr >= 0) {
+          const double cf = presCoef(I, f, c, pr);
+          diag += cf;
+          cWk = -cf;
+        } else {
+          presBoundary(I, f, I.bcB[i], F, c, diag);
+        }
+      }
+    }
+    // above
+    {
+      const int fi = (j + 1) * nI + i;
+      const FaceGeo& f = I.jg[fi];
+      const double F = I.Fj[fi];
+      net += F;
+      if (j < nJ - 1) {
+        const double cf = presCoef(I, f, c, c + nI);
+        diag += cf;
+        cN = -cf;
+      } else {
+        presBoundary(I, f, I.bcT[i], F, c, diag);
+      }
+    }
+    // left
+    {
+      const int fi = j * (nI + 1) + i;
+      const FaceGeo& f = I.ig[fi];
+      const double F = I.Fi[fi];
+      if (i > 0) {
+        net -= F;
+        diag += presCoef(I, f, c - 1, c);
+      } else {
+        net += F;
+        presBoundary(I, f, I.bcL[j], F, c, diag);
+      }
+    }
+    // right
+    {
+      const int fi = j * (nI + 1) + i + 1;
+      const FaceGeo& f = I.ig[fi];
+      const double F = I.Fi[fi];
+      net += F;
+      if (i < nI - 1) {
+        const double cf = presCoef(I, f, c, c + 1);
+        diag += cf;
+        cE = -cf;
+      } else {
+        presBoundary(I, f, I.bcR[j], F, c, diag);
+      }
+    }
+    I.pP[c] = diag;
+    I.pN[c] = cN;
+    I.pE[c] = cE;
+    I.pWk[c] = cWk;
+    I.pb[c] = -net;
+    part[0] = fabs(net);
+    part[1] = -net;
+  }
+  blockSum<2>(part, sm);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sNetFluxL1)[blockIdx.x] = part[0];
+    partialSlot(I, sSourceSum)[blockIdx.x] = part[1];
+  }
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double a = waveSumPartials(partialSlot(I, sNetFluxL1), I.nBlocks);
+      const double b = waveSumPartials(partialSlot(I, sSourceSum), I.nBlocks);
+      if (threadIdx.x == 0) {
+        if (pass == 0) I.red[sNetFluxL1] = a;
+        I.red[sSourceSum] = b;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned CG with the reference's L1 stopping test
+// (LinearSystem.cpp:118-210), two launches per iteration:
+//   A: p_k = z + beta p_{k-1} (halo recomputed locally), Ap, p.Ap
+//   B: x += alpha p, r -= alpha Ap, |r|_1, z = r / diag, r.z
+// The scalars live in device memory and the last block of each launch
+// finalises them, so the host never waits inside the solve; once the test
+// passes (or the iteration breaks down) every later launch is a no-op.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c, int i, int j) {
+  const int nI = I.nI, nJ = I.nJ;
+  double y = I.pP[c] * x[c];
+  if (j > 0) y += I.pN[c - nI] * x[c - nI];
+  else {
+    const int pr = I.partner[i];
+    if (pr >= 0) y += I.pWk[c] * x[pr];
+  }
+  if (j < nJ - 1) y += I.pN[c] * x[c + nI];
+  if (i > 0) y += I.pE[c - 1] * x[c - 1];
+  if (i < nI - 1) y += I.pE[c] * x[c + 1];
+  return y;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
+  CFD_PROLOGUE
+  __shared__ double sm[16];
+  double part[2] = {0.0, 0.0};
+  if (live) {
+    double b = I.pb[c];
+    if (!I.hasPRef) {
+      const double mean = I.red[sSourceSum] / static_cast<double>(I.N);
+      b -= mean;
+      I.pb[c] = b;
+    }
+    const double r = b - 0.0;
+    const double d = I.pP[c];
+    const double z = (fabs(d) > 0.0) ? r / d : r;
+    I.r[c] = r;
+    I.z[c] = z;
+    I.pd0[c] = z;
+    I.x[c] = 0.0;
+    part[0] = fabs(r);
+    part[1] = r * z;
+  }
+  blockSum<2>(part, sm);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sCgNorm0)[blockIdx.x] = part[0];
+    partialSlot(I, sCgRz)[blockIdx.x] = part[1];
+  }
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double n0 = waveSumPartials(partialSlot(I, sCgNorm0), I.nBlocks);
+      const double rz = waveSumPartials(partialSlot(I, sCgRz), I.nBlocks);
+      if (threadIdx.x == 0) {
+        I.red[sCgNorm0] = n0;
+        I.red[sCgNorm] = n0;
+        I.red[sCgRz] = rz;
+        I.red[sCgBeta] = 0.0;
+        I.flags[fCgIters] = 0;
+        I.flags[fCgDone] = (n0 <= 0.0) ? 1 : 0;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
+  CFD_PROLOGUE
+  if (I.flags[fCgDone]) return;
+  __shared__ double sm[16];
+  double part[1] = {0.0};
+  if (live) {
+    double* pNew = (it & 1) ? I.pd1 : I.pd0;
+    const double* pOld = (it & 1) ? I.pd0 : I.pd1;
+    double y;
+    double pc;
+    if (it == 0) {
+      pc = pNew[c];
+      y = spmvPres(I, pNew, c, i, j);
+    } else {
+      const double beta = I.red[sCgBeta];
+      auto pv = [&](int q) { return I.z[q] + beta * pOld[q]; };
+      pc = pv(c);
+      y = I.pP[c] * pc;
+      if (j > 0) y += I.pN[c - nI] * pv(c - nI);
+      else {
+        const int pr = I.partner[i];
+        if (pr >= 0) y += I.pWk[c] * pv(pr);
+      }
+      if (j < nJ - 1) y += I.pN[c] * pv(c + nI);
+      if (i > 0) y += I.pE[c - 1] * pv(c - 1);
+      if (i < nI - 1) y += I.pE[c] * pv(c + 1);
+      pNew[c] = pc;
+    }
+    I.Ap[c] = y;
+    part[0] = pc * y;
+  }
+  blockSum<1>(part, sm);
+  if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double pap = waveSumPartials(partialSlot(I, sCgPAp), I.nBlocks);
+      if (threadIdx.x == 0) {
+        I.red[sCgPAp] = pap;
+        if (!(fabs(pap) > 0.0)) I.flags[fCgDone] = 1;
+        else I.red[sCgAlpha] = I.red[sCgRz] / pap;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
+  CFD_PROLOGUE
+  if (I.flags[fCgDone]) return;
+  __shared__ double sm[16];
+  double part[2] = {0.0, 0.0};
+  if (live) {
+    const double* p = (it & 1) ? I.pd1 : I.pd0;
+    const double alpha = I.red[sCgAlpha];
+    const double x = I.x[c] + alpha * p[c];
+    const double r = I.r[c] - alpha * I.Ap[c];
+    I.x[c] = x;
+    I.r[c] = r;
+    const double d = I.pP[c];
+    const double z = (fabs(d) > 0.0) ? r / d : r;
+    I.z[c] = z;
+    part[0] = fabs(r);
+    part[1] = r * z;
+  }
+  blockSum<2>(part, sm);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
+    partialSlot(I, sCgRz)[blockIdx.x] = part[1];
+  }
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double nrm = waveSumPartials(partialSlot(I, sCgNorm), I.nBlocks);
+      const double rzn = waveSumPartials(partialSlot(I, sCgRz), I.nBlocks);
+      if (threadIdx.x == 0) {
+        I.red[sCgNorm] = nrm;
+        I.flags[fCgIters] = it + 1;
+        I.flags[fCgTotal] += 1;
+        if (nrm <= I.cgTol * I.red[sCgNorm0]) {
+          I.flags[fCgDone] = 1;
+        } else {
+          const double rz = I.red[sCgRz];
+          if (!(fabs(rz) > 0.0)) {
+            I.flags[fCgDone] = 1;
+          } else {
+            I.red[sCgBeta] = rzn / rz;
+            I.red[sCgRz] = rzn;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Mean of the correction when no boundary anchors the pressure level
+// (SimpleSolver.cpp:735-745).
+__global__ void __launch_bounds__(kBlock) k_corr_mean(const Inst* insts) {
+  CFD_PROLOGUE
+  __shared__ double sm[16];
+  double part[1] = {live ? I.x[c] : 0.0};
+  blockSum<1>(part, sm);
+  if (threadIdx.x == 0) partialSlot(I, sCorrMean)[blockIdx.x] = part[0];
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double s = waveSumPartials(partialSlot(I, sCorrMean), I.nBlocks);
+      if (threadIdx.x == 0) I.red[sCorrMean] = s / static_cast<double>(I.N);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Correction (SimpleSolver.cpp:582-660): p' face values -> Green-Gauss ->
+// p += alpha_p p', u -= grad p' V / a_P. Face fluxes in a second launch, so
+// the imposed-velocity test of the first still sees the uncorrected fluxes.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double corrAt(const Inst& I, int q) {
+  return I.hasPRef ? I.x[q] : I.x[q] - I.red[sCorrMean];
+}
+
+__global__ void __launch_bounds__(kBlock) k_correct_cells(const Inst* insts) {
+  CFD_PROLOGUE
+  if (!live) return;
+  double gx = 0.0, gy = 0.0;
+  const double pc = corrAt(I, c);
+  auto add = [&](const FaceGeo& f, double val, bool plus) {
+    if (plus) { gx += f.Sx * val; gy += f.Sy * val; }
+    else { gx -= f.Sx * val; gy -= f.Sy * val; }
+  };
+  {
+    const int fi = j * nI + i;
+    const FaceGeo& f = I.jg[fi];
+    if (j > 0) add(f, f.w * corrAt(I, c - nI) + (1.0 - f.w) * pc, false);
+    else {
+      const int pr = I.partner[i];
+      if (pr >= 0) add(f, f.w * pc + (1.0 - f.w) * corrAt(I, pr), true);
+      else add(f, imposes(I.bcB[i], I.Fj[fi]) ? pc : 0.0, true);
+    }
+  }
+  {
+    const int fi = (j + 1) * nI + i;
+    const FaceGeo& f = I.jg[fi];
+    if (j < nJ - 1) add(f, f.w * pc + (1.0 - f.w) * corrAt(I, c + nI), true);
+    else add(f, imposes(I.bcT[i], I.Fj[fi]) ? pc : 0.0, true);
+  }
+  {
+    const int fi = j * (nI + 1) + i;
+    const FaceGeo& f = I.ig[fi];
+    if (i > 0) add(f, f.w * corrAt(I, c - 1) + (1.0 - f.w) * pc, false);
+    else add(f, imposes(I.bcL[j], I.Fi[fi]) ? pc : 0.0, true);
+  }
+  {
+    const int fi = j * (nI + 1) + i + 1;
+    const FaceGeo& f = I.ig[fi];
+    if (i < nI - 1) add(f, f.w * pc + (1.0 - f.w) * corrAt(I, c + 1), true);
+    else add(f, imposes(I.bcR[j], I.Fi[fi]) ? pc : 0.0, true);
+  }
+  const double vol = I.V[c];
+  if (vol > 0.0) {
+    const double inv = 1.0 / vol;
+    gx *= inv;
+    gy *= inv;
+  }
+  I.p[c] += I.alphaP * pc;
+  const double mob = vol / I.aP[c];
+  I.u[c] -= gx * mob;
+  I.v[c] -= gy * mob;
+}
+
+__device__ __forceinline__ double corrFlux(const Inst& I, const FaceGeo& f, int o, int n) {
+  const double w = f.w;
+  const double dens = w * I.rho[o] + (1.0 - w) * I.rho[n];
+  const double mob = w * (I.V[o] / I.aP[o]) + (1.0 - w) * (I.V[n] / I.aP[n]);
+  return dens * mob * f.g * (corrAt(I, n) - corrAt(I, o));
+}
+
+__device__ __forceinline__ void corrBoundaryFlux(const Inst& I, const FaceGeo& f, const FaceBC& bc, double* F,
+                                                 int o) {
+  if (imposes(bc, *F)) return;
+  const double dens = I.rho[o];
+  const double mob = I.V[o] / I.aP[o];
+  *F -= dens * mob * f.g * (0.0 - corrAt(I, o));
+}
+
+__global__ void __launch_bounds__(kBlock) k_correct_faces(const Inst* insts) {
+  CFD_PROLOGUE
+  if (!live) return;
+  {
+    const int fi = j * nI + i;
+    const FaceGeo& f = I.jg[fi];
+    if (j > 0) I.Fj[fi] -= corrFlux(I, f, c - nI, c);
+    else {
+      const int pr = I.partner[i];
+      if (pr >= 0) I.Fj[fi] -= corrFlux(I, f, c, pr);
+      else corrBoundaryFlux(I, f, I.bcB[i], &I.Fj[fi], c);
+    }
+  }
+  if (j == nJ - 1) {
+    const int fi = (j + 1) * nI + i;
+    corrBoundaryFlux(I, I.jg[fi], I.bcT[i], &I.Fj[fi], c);
+  }
+  {
+    const int fi = j * (nI + 1) + i;
+    const FaceGeo& f = I.ig[fi];
+    if (i > 0) I.Fi[fi] -= corrFlux(I, f, c - 1, c);
+    else corrBoundaryFlux(I, f, I.bcL[j], &I.Fi[fi], c);
+  }
+  if (i == nI - 1) {
+    const int fi = j * (nI + 1) + i + 1;
+    corrBoundaryFlux(I, I.ig[fi], I.bcR[j], &I.Fi[fi], c);
+  }
+}
+
+// SimpleSolver::divergence (SimpleSolver.cpp:796-817) -> max |div|.
+__device__ __forceinline__ double divergenceAt(const Inst& I, int c, int i, int j) {
+  const int nI = I.nI, nJ = I.nJ;
+  constexpr double kTiny = 1e-300;
+  double net = 0.0;
+  if (j > 0) net -= I.Fj[j * nI + i] / mx(I.rho[c - nI], kTiny);
+  else net += I.Fj[i] / mx(I.rho[c], kTiny);
+  net += I.Fj[(j + 1) * nI + i] / mx(I.rho[c], kTiny);
+  (void)nJ;
+  if (i > 0) net -= I.Fi[j * (nI + 1) + i] / mx(I.rho[c - 1], kTiny);
+  else net += I.Fi[j * (nI + 1) + i] / mx(I.rho[c], kTiny);
+  net += I.Fi[j * (nI + 1) + i + 1] / mx(I.rho[c], kTiny);
+  const double a = I.V[c];
+  return (a > 0.0) ? net / a : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_divergence(const Inst* insts, double* out) {
+  CFD_PROLOGUE
+  __shared__ double sm[16];
+  double part[1] = {0.0};
+  if (live) {
+    const double d = divergenceAt(I, c, i, j);
+    if (out) out[c] = d;
+    part[0] = mx(0.0, fabs(d));
+  }
+  blockMax<1>(part, sm);
+  if (threadIdx.x == 0) partialSlot(I, sMaxDiv)[blockIdx.x] = part[0];
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double s = waveMaxPartials(partialSlot(I, sMaxDiv), I.nBlocks);
+      if (threadIdx.x == 0) I.red[sMaxDiv] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Monitor assembly + the caller's stop rule (SolverWorker.cpp:161-192).
+// One thread per instance.
+// ---------------------------------------------------------------------------
+
+__global__ void k_monitor(const Inst* insts, int count) {
+  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= count) return;
+  const Inst& I = insts[b];
+  if (I.flags[fStopped]) return;
+  constexpr double kTiny = 1e-300;
+  DevMonitor m{};
+  m.continuity = I.red[sNetFluxL1] / I.refMassFlow;
+  m.massImbalance = I.red[sNetFluxL1];
+  const double scale = I.red[sMomDiagL1] * I.red[sVelScale];
+  m.momentumX = I.red[sMomResX] / mx(scale, kTiny);
+  m.momentumY = I.red[sMomResY] / mx(scale, kTiny);
+  m.maxDivergence = I.red[sMaxDiv];
+  m.pressureIterations = I.flags[fCgTotal];
+  if (I.model == 2) {
+    m.turbFirst = I.red[sTurbResK] / mx(I.red[sTurbDiagK] * mx(I.kIn, 1e-30), 1e-30);
+    m.turbSecond = I.red[sTurbResW] / mx(I.red[sTurbDiagW] * mx(I.wIn, 1e-30), 1e-30);
+    m.minK = I.red[sMinK];
+    m.maxK = I.red[sMaxK];
+    m.minW = I.red[sMinW];
+    m.maxW = I.red[sMaxW];
+    m.minMut = I.red[sMinMut];
+    m.maxMut = I.red[sMaxMut];
+    m.maxEddyRatio = I.red[sMaxEddyRatio];
+  }
+  *I.monitor = m;
+  const int iter = I.flags[fIteration] + 1;
+  I.flags[fIteration] = iter;
+  if (I.history && I.historyCap > 0) I.history[(iter - 1) % I.historyCap] = m;
+  if (I.stopTol > 0.0 || I.stopMaxIter > 0) {
+    const bool finite = isfinite(m.continuity) && isfinite(m.momentumX) && isfinite(m.momentumY);
+    const double worst = mx(mx(fabs(m.continuity), fabs(m.momentumX)), fabs(m.momentumY));
+    if (!finite) {
+      I.flags[fStatus] = 2;
+      I.flags[fStopped] = 1;
+    } else if (worst < I.stopTol) {
+      I.flags[fStatus] = 0;
+      I.flags[fStopped] = 1;
+    } else if (I.stopMaxIter > 0 && iter >= I.stopMaxIter) {
+      I.flags[fStatus] = 1;
+      I.flags[fStopped] = 1;
+    }
+  }
+}
+
+// Reset the per-iteration CG total before the pressure passes.
+__global__ void k_begin_iteration(const Inst* insts, int count) {
+  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= count) return;
+  const Inst& I = insts[b];
+  if (I.flags[fStopped]) return;
+  I.flags[fCgTotal] = 0;
+}
+
+}  // namespace cfdgpu

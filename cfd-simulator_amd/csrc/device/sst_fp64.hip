@@ -1,0 +1,409 @@
+// sst_fp64.hip - k-omega SST update on the corrected field
+// (reference: src/solver/KOmegaSST.cpp:531-548), FP64, cell-centric.
+//
+//   k_sst_blend : strain (start-of-iteration gradients), Green-Gauss of k and
+//                 omega, cross diffusion, F1, F2, limited production
+//                 (KOmegaSST.cpp:202-287, 357-368)
+//   k_sst_asm   : both transport systems + sources + relaxation + residuals
+//                 (:370-466, 475-510)
+//   k_gs_lex / k_gs_color (systems 1, 2) : 3 sweeps + floors (:468-471)
+//   k_sst_wall  : wall-cell omega and y+ (:305-345)
+//   k_sst_mut   : limited eddy viscosity, mu_eff = mu + mu_t, ranges
+//                 (:289-303, 513-529; SimpleSolver.cpp:232-245, 780-790)
+#include "common.hpp"
+
+namespace cfdgpu {
+
+#define SST_PROLOGUE                                          \
+  const Inst& I = insts[blockIdx.y];                          \
+  if (static_cast<int>(blockIdx.x) >= I.nBlocks) return;     \
+  if (I.flags[fStopped]) return;                              \
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.nBlocks); \
+  const int c = tile * kBlock + static_cast<int>(threadIdx.x); \
+  const bool live = c < I.N;                                  \
+  const int nI = I.nI;                                        \
+  const int nJ = I.nJ;                                        \
+  const int i = live ? c % nI : 0;                            \
+  const int j = live ? c / nI : 0;
+
+constexpr double kMinOmega = 1.0e-10;
+constexpr double kTinySst = 1.0e-30;
+
+// KOmegaSST.cpp:26-68: face value of k or omega on a boundary face.
+__device__ __forceinline__ double sstBoundaryValue(const FaceBC& bc, double F, double cell, double wallValue,
+                                                   double inflow) {
+  switch (bc.kind) {
+    case kWall: return wallValue;
+    case kInlet: return inflow;
+    case kFarField: return (F < 0.0) ? inflow : cell;
+    default: return cell;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sst_blend(const Inst* insts) {
+  SST_PROLOGUE
+  if (!live) return;
+  // strain from the gradients of the start of the iteration (KOmegaSST.cpp:202-216)
+  const double2 gu = I.gU[c], gv = I.gV[c];
+  const double s11 = gu.x, s22 = gv.y, s12 = 0.5 * (gu.y + gv.x);
+  const double S = sqrt(2.0 * (s11 * s11 + s22 * s22 + 2.0 * s12 * s12));
+  I.strain[c] = S;
+
+  // Green-Gauss of k (wall value 0) and omega (wall faces take the owner value)
+  const double kc = I.k[c], wc = I.w[c];
+  double kx = 0.0, ky = 0.0, wx = 0.0, wy = 0.0;
+  auto add = [&](const FaceGeo& f, double fk, double fw, bool plus) {
+    if (plus) { kx += f.Sx * fk; ky += f.Sy * fk; wx += f.Sx * fw; wy += f.Sy * fw; }
+    else { kx -= f.Sx * fk; ky -= f.Sy * fk; wx -= f.Sx * fw; wy -= f.Sy * fw; }
+  };
+  auto interior = [&](const FaceGeo& f, int o, int n, bool plus) {
+    const double w = f.w;
+    add(f, w * I.k[o] + (1.0 - w) * I.k[n], w * I.w[o] + (1.0 - w) * I.w[n], plus);
+  };
+  auto boundary = [&](const FaceGeo& f, const FaceBC& bc, double F) {
+    const double fk = sstBoundaryValue(bc, F, kc, 0.0, I.kIn);
+    const double fw = (bc.kind == kWall) ? wc : sstBoundaryValue(bc, F, wc, 0.0, I.wIn);
+    add(f, fk, fw, true);
+  };
+  {
+    const int fi = j * nI + i;
+    const FaceGeo& f = I.jg[fi];
+    if (j > 0) interior(f, c - nI, c, false);
+    else {
+      const int pr = I.partner[i];
+      if (pr >= 0) {
+        // the wake face has no neighbour, so ownerWeight() is 1 (Gradient.cpp:9-11)
+        add(f, 1.0 * kc + (1.0 - 1.0) * I.k[pr], 1.0 * wc + (1.0 - 1.0) * I.w[pr], true);
+      } else {
+        boundary(f, I.bcB[i], I.Fj[fi]);
+      }
+    }
+  }
+  {
+    const int fi = (j + 1) * nI + i;
+    const FaceGeo& f = I.jg[fi];
+    if (j < nJ - 1) interior(f, c, c + nI, true);
+    else boundary(f, I.bcT[i], I.Fj[fi]);
+  }
+  {
+    const int fi = j * (nI + 1) + i;
+    const FaceGeo& f = I.ig[fi];
+    if (i > 0) interior(f, c - 1, c, false);
+    else boundary(f, I.bcL[j], I.Fi[fi]);
+  }
+  {
+    const int fi = j * (nI + 1) + i + 1;
+    const FaceGeo& f = I.ig[fi];
+    if (i < nI - 1) interior(f, c, c + 1, true);
+    else boundary(f, I.bcR[j], I.Fi[fi]);
+  }
+  const double vol = I.V[c];
+  if (vol > 0.0) {
+    const double inv = 1.0 / vol;
+    kx *= inv; ky *= inv; wx *= inv; wy *= inv;
+  }
+  I.gK[c] = make_double2(kx, ky);
+  I.gW[c] = make_double2(wx, wy);
+
+  // blending functions (KOmegaSST.cpp:247-286)
+  const double d = mx(I.wallDist[c], kTinySst);
+  const double rho = I.rho[c];
+  const double nu = I.muMol[c] / mx(rho, kTinySst);
+  const double omega = mx(wc, kMinOmega);
+  const double energy = mx(kc, 0.0);
+  const double cd = 2.0 * rho * I.sigW2 * (kx * wx + ky * wy) / omega;
+  const double cdPlus = mx(cd, 1.0e-10);
+  const double term1 = sqrt(mx(energy, 0.0)) / (I.betaStar * omega * d);
+  const double term2 = 500.0 * nu / (d * d * omega);
+  const double term3 = 4.0 * rho * I.sigW2 * energy / (cdPlus * d * d);
+  const double arg1 = mn(mx(term1, term2), term3);
+  const double f1 = tanh(clampd(arg1 * arg1 * arg1 * arg1, 0.0, 50.0));
+  const double arg2 = mx(2.0 * term1, term2);
+  const double f2 = tanh(clampd(arg2 * arg2, 0.0, 50.0));
+  I.cd[c] = cd;
+  I.f1[c] = f1;
+  I.f2[c] = f2;
+  // limited production (KOmegaSST.cpp:362-368)
+  const double raw = I.mut[c] * S * S;
+  const double ceiling = I.prodLimit * I.betaStar * rho * mx(kc, 0.0) * mx(wc, kMinOmega);
+  I.prod[c] = mn(raw, ceiling);
+}
+
+// One transport equation's row for cell c.
+struct SstRow {
+  double d, b, aS, aN, aW, aE;
+};
+
+__device__ __forceinline__ double sigmaK(const Inst& I, int q) { return I.f1[q] * I.sigK1 + (1.0 - I.f1[q]) * I.sigK2; }
+__device__ __forceinline__ double sigmaW(const Inst& I, int q) { return I.f1[q] * I.sigW1 + (1.0 - I.f1[q]) * I.sigW2; }
+
+// KOmegaSST.cpp:384-444 for both equations at once. isK selects sigma.
+template <bool isK>
+__device__ __forceinline__ double sstDiffusivity(const Inst& I, int q) {
+  return I.muMol[q] + (isK ? sigmaK(I, q) : sigmaW(I, q)) * I.mut[q];
+}
+
+template <bool isK>
+__device__ __forceinline__ double sstInteriorOff(const Inst& I, double w, double gs, double F, int o, int n,
+                                                 bool cellIsOwner, double& d) {
+  const double diffusivity = w * sstDiffusivity<isK>(I, o) + (1.0 - w) * sstDiffusivity<isK>(I, n);
+  const double diff = diffusivity * gs;
+  const double outward = mx(F, 0.0), inward = mx(-F, 0.0);
+  if (cellIsOwner) {
+    d += outward + diff;
+    return -inward - diff;
+  }
+  d += inward + diff;
+  return -outward - diff;
+}
+
+template <bool isK>
+__device__ __forceinline__ void sstBoundaryRow(const Inst& I, const FaceBC& bc, double F, int o, double wallValue,
+                                               double inflow, double& d, double& b) {
+  const double diff = sstDiffusivity<isK>(I, o) * bc.gSst;
+  const bool imposed = bc.kind == kWall || bc.kind == kInlet || (bc.kind == kFarField && F < 0.0);
+  if (imposed) {
+    const double value = (bc.kind == kWall) ? wallValue : inflow;
+    d += diff + mx(F, 0.0);
+    b += (diff - mn(F, 0.0)) * value;
+  } else {
+    d += mx(F, 0.0);
+  }
+}
+
+template <bool isK>
+__device__ __forceinline__ SstRow sstRow(const Inst& I, int c, int i, int j, double wallValue, double inflow) {
+  const int nI = I.nI, nJ = I.nJ;
+  SstRow r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  {
+    const int fi = j * nI + i;
+    const double F = I.Fj[fi];
+    if (j > 0) r.aS = sstInteriorOff<isK>(I, I.jg[fi].w, I.jgSst[fi], F, c - nI, c, false, r.d);
+    else {
+      const int pr = I.partner[i];
+      if (pr >= 0) r.aS = sstInteriorOff<isK>(I, 1.0, I.jgSst[fi], F, c, pr, true, r.d);
+      else sstBoundaryRow<isK>(I, I.bcB[i], F, c, wallValue, inflow, r.d, r.b);
+    }
+  }
+  {
+    const int fi = (j + 1) * nI + i;
+    const double F = I.Fj[fi];
+    if (j < nJ - 1) r.aN = sstInteriorOff<isK>(I, I.jg[fi].w, I.jgSst[fi], F, c, c + nI, true, r.d);
+    else sstBoundaryRow<isK>(I, I.bcT[i], F, c, wallValue, inflow, r.d, r.b);
+  }
+  {
+    const int fi = j * (nI + 1) + i;
+    const double F = I.Fi[fi];
+    if (i > 0) r.aW = sstInteriorOff<isK>(I, I.ig[fi].w, I.igSst[fi], F, c - 1, c, false, r.d);
+    else sstBoundaryRow<isK>(I, I.bcL[j], F, c, wallValue, inflow, r.d, r.b);
+  }
+  {
+    const int fi = j * (nI + 1) + i + 1;
+    const double F = I.Fi[fi];
+    if (i < nI - 1) r.aE = sstInteriorOff<isK>(I, I.ig[fi].w, I.igSst[fi], F, c, c + 1, true, r.d);
+    else sstBoundaryRow<isK>(I, I.bcR[j], F, c, wallValue, inflow, r.d, r.b);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double sstResidualTerm(const Inst& I, const SstRow& r, const double* phi, int c, int i,
+                                                  int j) {
+  const int nI = I.nI, nJ = I.nJ;
+  double ax = r.d * phi[c];
+  if (j > 0) ax += r.aS * phi[c - nI];
+  else {
+    const int pr = I.partner[i];
+    if (pr >= 0) ax += r.aS * phi[pr];
+  }
+  if (j < nJ - 1) ax += r.aN * phi[c + nI];
+  if (i > 0) ax += r.aW * phi[c - 1];
+  if (i < nI - 1) ax += r.aE * phi[c + 1];
+  return fabs(r.b - ax);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
+  SST_PROLOGUE
+  __shared__ double sm[16];
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  if (live) {
+    const double vol = I.V[c];
+    const double rho = I.rho[c];
+    const double f1 = I.f1[c];
+    const double S = I.strain[c];
+    const double relax = clampd(I.alphaT, 0.05, 1.0);
+    // k equation
+    {
+      SstRow r = sstRow<true>(I, c, i, j, 0.0, I.kIn);
+      const double srcE = I.prod[c];
+      const double srcI = I.betaStar * rho * mx(I.w[c], kMinOmega);
+      r.b += srcE * vol;
+      r.d += srcI * vol;
+      const double phi = I.k[c];
+      if (fabs(r.d) > kTinySst) {
+        const double relaxed = r.d / relax;
+        r.b += (relaxed - r.d) * phi;
+        r.d = relaxed;
+      }
+      I.kP[c] = r.d; I.kS[c] = r.aS; I.kN[c] = r.aN; I.kW[c] = r.aW; I.kE[c] = r.aE; I.kB[c] = r.b;
+      part[0] = sstResidualTerm(I, r, I.k, c, i, j);
+      part[1] = fabs(r.d);
+    }
+    // omega equation (wall value is omega_inflow: KOmegaSST.cpp:507-509)
+    {
+      SstRow r = sstRow<false>(I, c, i, j, I.wIn, I.wIn);
+      const double gamma = f1 * I.gamma1 + (1.0 - f1) * I.gamma2;
+      const double beta = f1 * I.beta1 + (1.0 - f1) * I.beta2;
+      double srcE = gamma * rho * S * S;
+      srcE += (1.0 - f1) * I.cd[c];
+      const double srcI = beta * rho * mx(I.w[c], kMinOmega);
+      r.b += srcE * vol;
+      r.d += srcI * vol;
+      const double phi = I.w[c];
+      if (fabs(r.d) > kTinySst) {
+        const double relaxed = r.d / relax;
+        r.b += (relaxed - r.d) * phi;
+        r.d = relaxed;
+      }
+      I.wP[c] = r.d; I.wS[c] = r.aS; I.wN[c] = r.aN; I.wW[c] = r.aW; I.wE[c] = r.aE; I.wB[c] = r.b;
+      part[2] = sstResidualTerm(I, r, I.w, c, i, j);
+      part[3] = fabs(r.d);
+    }
+  }
+  blockSum<4>(part, sm);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sTurbResK)[blockIdx.x] = part[0];
+    partialSlot(I, sTurbDiagK)[blockIdx.x] = part[1];
+    partialSlot(I, sTurbResW)[blockIdx.x] = part[2];
+    partialSlot(I, sTurbDiagW)[blockIdx.x] = part[3];
+  }
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const double a = waveSumPartials(partialSlot(I, sTurbResK), I.nBlocks);
+      const double b = waveSumPartials(partialSlot(I, sTurbDiagK), I.nBlocks);
+      const double e = waveSumPartials(partialSlot(I, sTurbResW), I.nBlocks);
+      const double f = waveSumPartials(partialSlot(I, sTurbDiagW), I.nBlocks);
+      if (threadIdx.x == 0) {
+        I.red[sTurbResK] = a;
+        I.red[sTurbDiagK] = b;
+        I.red[sTurbResW] = e;
+        I.red[sTurbDiagW] = f;
+      }
+    }
+  }
+}
+
+// KOmegaSST.cpp:86-92
+__device__ __forceinline__ double wallOmega(const Inst& I, double nu, double distance, double energy) {
+  const double d = mx(distance, kTinySst);
+  const double viscous = 6.0 * nu / (I.beta1 * d * d);
+  const double logarithmic = sqrt(mx(energy, 0.0)) / (I.betaStarQ * I.kappa * d);
+  return sqrt(viscous * viscous + logarithmic * logarithmic);
+}
+
+// One block per instance walks the four boundary edges (KOmegaSST.cpp:305-345).
+__global__ void __launch_bounds__(kBlock) k_sst_wall(const Inst* insts) {
+  const Inst& I = insts[blockIdx.y];
+  if (I.flags[fStopped]) return;
+  __shared__ double sm[16];
+  const int nI = I.nI, nJ = I.nJ;
+  const int total = 2 * nI + 2 * nJ;
+  double ymax[1] = {0.0};
+  for (int e = static_cast<int>(threadIdx.x); e < total; e += kBlock) {
+    const FaceBC* bc;
+    const FaceGeo* f;
+    int o;
+    if (e < nI) { bc = &I.bcB[e]; f = &I.jg[e]; o = e; }
+    else if (e < 2 * nI) { const int q = e - nI; bc = &I.bcT[q]; f = &I.jg[nJ * nI + q]; o = (nJ - 1) * nI + q; }
+    else if (e < 2 * nI + nJ) { const int q = e - 2 * nI; bc = &I.bcL[q]; f = &I.ig[q * (nI + 1)]; o = q * nI; }
+    else { const int q = e - 2 * nI - nJ; bc = &I.bcR[q]; f = &I.ig[q * (nI + 1) + nI]; o = q * nI + nI - 1; }
+    if (bc->kind != kWall) continue;
+    const double d = mx(I.wallDist[o], kTinySst);
+    const double rho = mx(I.rho[o], kTinySst);
+    const double nu = I.muMol[o] / rho;
+    I.w[o] = wallOmega(I, nu, d, I.k[o]);
+    // y+ with the tangential velocity of the wall cell (KOmegaSST.cpp:337-343)
+    const double area = hypot(f->Sx, f->Sy);
+    const double nx = -(f->Sx / area), ny = -(f->Sy / area);
+    const double uo = I.u[o], vo = I.v[o];
+    const double un = uo * nx + vo * ny;
+    const double px = uo - nx * un, py = vo - ny * un;
+    const double tang = hypot(px, py);
+    const double shear = I.muMol[o] * tang / d;
+    const double fv = sqrt(shear / rho);
+    ymax[0] = mx(ymax[0], fv * d / mx(nu, kTinySst));
+  }
+  blockMax<1>(ymax, sm);
+  if (threadIdx.x == 0) I.red[sMaxYPlus] = ymax[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_sst_mut(const Inst* insts) {
+  SST_PROLOGUE
+  (void)i; (void)j; (void)nJ;
+  __shared__ double sm[32];
+  double mxv[4] = {-INFINITY, -INFINITY, -INFINITY, 0.0};  // max k, max w, max mut, max ratio
+  double mnv[3] = {INFINITY, INFINITY, INFINITY};
+  if (live) {
+    const double energy = mx(I.k[c], 0.0);
+    const double omega = mx(I.w[c], kMinOmega);
+    const double den = mx(I.a1 * omega, I.strain[c] * I.f2[c]);
+    const double mut = I.rho[c] * I.a1 * energy / mx(den, kTinySst);
+    I.mut[c] = mut;
+    I.mu[c] = I.muMol[c] + mut;
+    mxv[0] = I.k[c]; mxv[1] = I.w[c]; mxv[2] = mut;
+    mnv[0] = I.k[c]; mnv[1] = I.w[c]; mnv[2] = mut;
+    const double m = I.muMol[c];
+    if (m > 0.0) mxv[3] = mx(0.0, mut / m);
+  }
+  blockMax<4>(mxv, sm);
+  double neg[3] = {-mnv[0], -mnv[1], -mnv[2]};
+  blockMax<3>(neg, sm + 16);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sMaxK)[blockIdx.x] = mxv[0];
+    partialSlot(I, sMaxW)[blockIdx.x] = mxv[1];
+    partialSlot(I, sMaxMut)[blockIdx.x] = mxv[2];
+    partialSlot(I, sMaxEddyRatio)[blockIdx.x] = mxv[3];
+    partialSlot(I, sMinK)[blockIdx.x] = -neg[0];
+    partialSlot(I, sMinW)[blockIdx.x] = -neg[1];
+    partialSlot(I, sMinMut)[blockIdx.x] = -neg[2];
+  }
+  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
+    if (threadIdx.x < 64) {
+      const int n = I.nBlocks;
+      double vmaxK = -INFINITY, vmaxW = -INFINITY, vmaxM = -INFINITY, vminK = INFINITY, vminW = INFINITY,
+             vminM = INFINITY, vr = 0.0;
+      for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) {
+        vmaxK = mx(vmaxK, partialSlot(I, sMaxK)[b]);
+        vmaxW = mx(vmaxW, partialSlot(I, sMaxW)[b]);
+        vmaxM = mx(vmaxM, partialSlot(I, sMaxMut)[b]);
+        vr = mx(vr, partialSlot(I, sMaxEddyRatio)[b]);
+        vminK = mn(vminK, partialSlot(I, sMinK)[b]);
+        vminW = mn(vminW, partialSlot(I, sMinW)[b]);
+        vminM = mn(vminM, partialSlot(I, sMinMut)[b]);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        vmaxK = mx(vmaxK, __shfl_down(vmaxK, o, 64));
+        vmaxW = mx(vmaxW, __shfl_down(vmaxW, o, 64));
+        vmaxM = mx(vmaxM, __shfl_down(vmaxM, o, 64));
+        vr = mx(vr, __shfl_down(vr, o, 64));
+        vminK = mn(vminK, __shfl_down(vminK, o, 64));
+        vminW = mn(vminW, __shfl_down(vminW, o, 64));
+        vminM = mn(vminM, __shfl_down(vminM, o, 64));
+      }
+      if (threadIdx.x == 0) {
+        I.red[sMaxK] = vmaxK; I.red[sMaxW] = vmaxW; I.red[sMaxMut] = vmaxM;
+        I.red[sMinK] = vminK; I.red[sMinW] = vminW; I.red[sMinMut] = vminM;
+        I.red[sMaxEddyRatio] = vr;
+      }
+    }
+  }
+}
+
+// Laminar closure (TurbulenceModel.cpp:67-77): mu_eff = mu + 0.
+__global__ void __launch_bounds__(kBlock) k_laminar_mu(const Inst* insts) {
+  SST_PROLOGUE
+  (void)i; (void)j; (void)nJ;
+  if (live) I.mu[c] = I.muMol[c] + 0.0;
+}
+
+}  // namespace cfdgpu

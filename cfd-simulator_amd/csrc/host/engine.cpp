@@ -1,0 +1,723 @@
+// engine.cpp - host setup and device residency of one SIMPLE instance.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+namespace cfdgpu {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr double kTiny = 1e-300;     // SimpleSolver.cpp:16
+constexpr double kTinySst = 1.0e-30; // KOmegaSST.cpp:22
+constexpr double kMinOmega = 1.0e-10;
+
+struct Carver {
+  size_t offset{0};
+  char* base{nullptr};
+  template <class T>
+  T* take(size_t count) {
+    offset = (offset + 255) & ~static_cast<size_t>(255);
+    T* p = base ? reinterpret_cast<T*>(base + offset) : nullptr;
+    offset += count * sizeof(T);
+    return p;
+  }
+};
+
+}  // namespace
+
+DeviceArena::~DeviceArena() {
+  if (base) (void)hipFree(base);
+}
+
+Engine::Engine(const Grid& grid, std::vector<int> kind, std::vector<Vec2> bcVelocity, std::vector<double> bcPressure,
+               const cfdgpu_settings& settings)
+    : grid_(grid), kind_(std::move(kind)), bcVel_(std::move(bcVelocity)), bcP_(std::move(bcPressure)),
+      settings_(settings) {
+  const int nf = grid_.faces();
+  if (static_cast<int>(kind_.size()) != nf || static_cast<int>(bcVel_.size()) != nf ||
+      static_cast<int>(bcP_.size()) != nf)
+    throw std::invalid_argument("face conditions do not match the mesh");
+  if (grid_.cells() == 0) throw std::invalid_argument("cannot solve on an empty mesh");
+  if ((grid_.cells() + kBlock - 1) / kBlock > kMaxRedBlocks)
+    throw std::invalid_argument("mesh exceeds the per-instance cell capacity (1,048,576 cells)");
+  if (grid_.nJ > 1024) throw std::invalid_argument("more than 1024 cells wall-normal is not supported");
+  // SimpleSolver.cpp:71-77
+  for (int f = 0; f < nf; ++f) {
+    if (kind_[static_cast<size_t>(f)] == CFDGPU_K_OUTLET || kind_[static_cast<size_t>(f)] == CFDGPU_K_FARFIELD) {
+      hasPRef_ = true;
+      break;
+    }
+  }
+  cfdgpu_default_sst(&sst_);
+  dead_.assign(static_cast<size_t>(nf), 0.0);
+  precomputeGeometry();
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  allocate();
+  uploadStatic();
+}
+
+Engine::~Engine() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (dInst_) (void)hipFree(dInst_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// SimpleSolver.cpp:82-131 (+ the SST geometric factor of KOmegaSST.cpp:399-427),
+// in the structured layout.
+void Engine::precomputeGeometry() {
+  const Grid& g = grid_;
+  const int nI = g.nI, nJ = g.nJ;
+  auto centroid = [&](int i, int j) { return g.centroid[static_cast<size_t>(j * nI + i)]; };
+  auto make = [&](Vec2 n, double len, Vec2 centre, Vec2 co, int kindOf, Vec2 cn, double& gSst) {
+    // kindOf: 0 interior (cn = neighbour centroid), 1 wake (cn = across), 2 boundary
+    FaceGeo f{};
+    const Vec2 S = n * len;
+    f.Sx = S.x;
+    f.Sy = S.y;
+    Vec2 d;
+    if (kindOf == 0) {
+      d = cn - co;
+      const double toO = distance(centre, co), toN = distance(centre, cn);
+      const double total = toO + toN;
+      f.w = (total > 0.0) ? toN / total : 0.5;
+    } else if (kindOf == 1) {
+      d = cn - co;
+      const double toO = distance(centre, co), toA = distance(centre, cn);
+      const double span = toO + toA;
+      f.w = (span > 0.0) ? toA / span : 0.5;
+    } else {
+      d = centre - co;
+      f.w = 1.0;
+    }
+    const double ad = dot(S, d), aa = dot(S, S);
+    if (std::abs(ad) > kTiny) {
+      f.g = aa / ad;
+      const Vec2 T = S - d * f.g;
+      f.Tx = T.x;
+      f.Ty = T.y;
+    } else {
+      f.g = 0.0;
+      f.Tx = f.Ty = 0.0;
+    }
+    gSst = (std::abs(ad) > kTinySst) ? std::abs(aa / ad) : 0.0;
+    return f;
+  };
+  jg_.resize(static_cast<size_t>(g.jFaces()));
+  jgSst_.assign(static_cast<size_t>(g.jFaces()), 0.0);
+  ig_.resize(static_cast<size_t>(g.iFaces()));
+  igSst_.assign(static_cast<size_t>(g.iFaces()), 0.0);
+  bcB_.assign(static_cast<size_t>(nI), FaceBC{});
+  bcT_.assign(static_cast<size_t>(nI), FaceBC{});
+  bcL_.assign(static_cast<size_t>(nJ), FaceBC{});
+  bcR_.assign(static_cast<size_t>(nJ), FaceBC{});
+  auto fillBC = [&](FaceBC& bc, int refFace, double gs) {
+    bc.kind = kind_[static_cast<size_t>(refFace)];
+    bc.u = bcVel_[static_cast<size_t>(refFace)].x;
+    bc.v = bcVel_[static_cast<size_t>(refFace)].y;
+    bc.p = bcP_[static_cast<size_t>(refFace)];
+    bc.dead = 0.0;
+    bc.gSst = gs;
+  };
+  for (int j = 0; j <= nJ; ++j) {
+    for (int i = 0; i < nI; ++i) {
+      const size_t f = static_cast<size_t>(j * nI + i);
+      double gs = 0.0;
+      if (j > 0 && j < nJ) {
+        jg_[f] = make(g.jNormal[f], g.jLen[f], g.jCentre[f], centroid(i, j - 1), 0, centroid(i, j), gs);
+        jgSst_[f] = gs;
+      } else if (j == 0) {
+        const int pr = g.bottomPartner[static_cast<size_t>(i)];
+        if (pr >= 0) {
+          jg_[f] = make(g.jNormal[f], g.jLen[f], g.jCentre[f], centroid(i, 0), 1, centroid(pr, 0), gs);
+          jgSst_[f] = gs;
+          fillBC(bcB_[static_cast<size_t>(i)], g.refJFace(i, 0), 0.0);
+        } else {
+          jg_[f] = make(g.jNormal[f], g.jLen[f], g.jCentre[f], centroid(i, 0), 2, Vec2{}, gs);
+          fillBC(bcB_[static_cast<size_t>(i)], g.refJFace(i, 0), gs);
+        }
+      } else {
+        jg_[f] = make(g.jNormal[f], g.jLen[f], g.jCentre[f], centroid(i, nJ - 1), 2, Vec2{}, gs);
+        fillBC(bcT_[static_cast<size_t>(i)], g.refJFace(i, nJ), gs);
+      }
+    }
+  }
+  for (int j = 0; j < nJ; ++j) {
+    for (int i = 0; i <= nI; ++i) {
+      const size_t f = static_cast<size_t>(j * (nI + 1) + i);
+      double gs = 0.0;
+      if (i > 0 && i < nI) {
+        ig_[f] = make(g.iNormal[f], g.iLen[f], g.iCentre[f], centroid(i - 1, j), 0, centroid(i, j), gs);
+        igSst_[f] = gs;
+      } else if (i == 0) {
+        ig_[f] = make(g.iNormal[f], g.iLen[f], g.iCentre[f], centroid(0, j), 2, Vec2{}, gs);
+        fillBC(bcL_[static_cast<size_t>(j)], g.refIFace(0, j), gs);
+      } else {
+        ig_[f] = make(g.iNormal[f], g.iLen[f], g.iCentre[f], centroid(nI - 1, j), 2, Vec2{}, gs);
+        fillBC(bcR_[static_cast<size_t>(j)], g.refIFace(nI, j), gs);
+      }
+    }
+  }
+}
+
+void Engine::allocate() {
+  const size_t N = static_cast<size_t>(grid_.cells());
+  const size_t nJf = static_cast<size_t>(grid_.jFaces()), nIf = static_cast<size_t>(grid_.iFaces());
+  const size_t nI = static_cast<size_t>(grid_.nI), nJ = static_cast<size_t>(grid_.nJ);
+  auto layout = [&](Carver& cv, Inst& I) {
+    I.V = cv.take<double>(N);
+    I.jg = cv.take<FaceGeo>(nJf);
+    I.ig = cv.take<FaceGeo>(nIf);
+    I.partner = cv.take<int>(nI);
+    I.bcB = cv.take<FaceBC>(nI);
+    I.bcT = cv.take<FaceBC>(nI);
+    I.bcL = cv.take<FaceBC>(nJ);
+    I.bcR = cv.take<FaceBC>(nJ);
+    I.wallDist = cv.take<double>(N);
+    I.jgSst = cv.take<double>(nJf);
+    I.igSst = cv.take<double>(nIf);
+    double** cellArrays[] = {&I.u, &I.v, &I.p, &I.rho, &I.mu, &I.muMol, &I.aP, &I.aS, &I.aN, &I.aW, &I.aE,
+                             &I.bx, &I.by, &I.pP, &I.pN, &I.pE, &I.pWk, &I.pb, &I.x, &I.r, &I.z, &I.pd0,
+                             &I.pd1, &I.Ap, &I.k, &I.w, &I.mut, &I.f1, &I.f2, &I.cd, &I.strain, &I.prod,
+                             &I.kP, &I.kS, &I.kN, &I.kW, &I.kE, &I.kB, &I.wP, &I.wS, &I.wN, &I.wW, &I.wE, &I.wB};
+    for (double** a : cellArrays) *a = cv.take<double>(N);
+    double2** vecArrays[] = {&I.gU, &I.gV, &I.gP, &I.gK, &I.gW};
+    for (double2** a : vecArrays) *a = cv.take<double2>(N);
+    I.Fj = cv.take<double>(nJf);
+    I.Fi = cv.take<double>(nIf);
+    float** fArrays[] = {&I.faP, &I.faS, &I.faN, &I.faW, &I.faE, &I.fbx, &I.fby, &I.fdx, &I.fdy, &I.fpP, &I.fpN,
+                         &I.fpE, &I.fpWk, &I.fx, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.fb};
+    for (float** a : fArrays) *a = cv.take<float>(N);
+    I.red = cv.take<double>(kSlots);
+    I.partials = cv.take<double>(static_cast<size_t>(kSlots) * kMaxRedBlocks);
+    I.counter = cv.take<unsigned>(16);
+    I.flags = cv.take<int>(kIFlags);
+    I.monitor = cv.take<DevMonitor>(1);
+    I.historyCap = 0;
+    I.history = nullptr;
+  };
+  Carver sizing;
+  Inst dummy{};
+  layout(sizing, dummy);
+  arena_.bytes = sizing.offset + 256;
+  check(hipMalloc(&arena_.base, arena_.bytes), "hipMalloc (solver state)");
+  check(hipMemsetAsync(arena_.base, 0, arena_.bytes, stream_), "hipMemset");
+  Carver real;
+  real.base = static_cast<char*>(arena_.base);
+  layout(real, h_);
+  h_.nI = grid_.nI;
+  h_.nJ = grid_.nJ;
+  h_.N = grid_.cells();
+  h_.nBlocks = (h_.N + kBlock - 1) / kBlock;
+  check(hipMalloc(&dInst_, sizeof(Inst)), "hipMalloc (descriptor)");
+  cfg_.maxBlocks = h_.nBlocks;
+  cfg_.batch = 1;
+  cfg_.gsThreads = gsThreadsFor(grid_.nJ);
+  cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
+}
+
+void Engine::uploadStatic() {
+  const size_t N = static_cast<size_t>(grid_.cells());
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "upload");
+  };
+  up(const_cast<double*>(h_.V), grid_.area.data(), N * sizeof(double));
+  up(const_cast<FaceGeo*>(h_.jg), jg_.data(), jg_.size() * sizeof(FaceGeo));
+  up(const_cast<FaceGeo*>(h_.ig), ig_.data(), ig_.size() * sizeof(FaceGeo));
+  up(const_cast<int*>(h_.partner), grid_.bottomPartner.data(), grid_.bottomPartner.size() * sizeof(int));
+  up(const_cast<FaceBC*>(h_.bcB), bcB_.data(), bcB_.size() * sizeof(FaceBC));
+  up(const_cast<FaceBC*>(h_.bcT), bcT_.data(), bcT_.size() * sizeof(FaceBC));
+  up(const_cast<FaceBC*>(h_.bcL), bcL_.data(), bcL_.size() * sizeof(FaceBC));
+  up(const_cast<FaceBC*>(h_.bcR), bcR_.data(), bcR_.size() * sizeof(FaceBC));
+  up(const_cast<double*>(h_.jgSst), jgSst_.data(), jgSst_.size() * sizeof(double));
+  up(const_cast<double*>(h_.igSst), igSst_.data(), igSst_.size() * sizeof(double));
+  check(hipStreamSynchronize(stream_), "upload sync");
+}
+
+void Engine::fillInstScalars() {
+  h_.hasPRef = hasPRef_ ? 1 : 0;
+  h_.scheme = settings_.scheme;
+  h_.model = model_;
+  h_.alphaU = settings_.velocity_relaxation;
+  h_.alphaP = settings_.pressure_relaxation;
+  h_.blend = settings_.non_orthogonal_blend;
+  h_.alphaT = settings_.turbulence_relaxation;
+  h_.momSweeps = settings_.momentum_sweeps;
+  h_.cgMax = settings_.pressure_iterations;
+  h_.cgTol = settings_.pressure_tolerance;
+  h_.nonOrthPasses = std::max(1, settings_.non_orthogonal_correctors);
+  h_.gsOrder = settings_.gs_ordering;
+  h_.precision = settings_.precision;
+  h_.psolve = settings_.pressure_solver;
+  h_.refMassFlow = refMassFlow_;
+  h_.kIn = kIn_;
+  h_.wIn = wIn_;
+  h_.sigK1 = sst_.sigma_k1; h_.sigW1 = sst_.sigma_w1; h_.beta1 = sst_.beta1;
+  h_.sigK2 = sst_.sigma_k2; h_.sigW2 = sst_.sigma_w2; h_.beta2 = sst_.beta2;
+  h_.betaStar = sst_.beta_star; h_.kappa = sst_.kappa; h_.a1 = sst_.a1; h_.prodLimit = sst_.production_limit;
+  // SSTConstants::gamma1/gamma2 (KOmegaSST.cpp:96-102) and beta*^(1/4) (:90)
+  h_.gamma1 = sst_.beta1 / sst_.beta_star - sst_.sigma_w1 * sst_.kappa * sst_.kappa / std::sqrt(sst_.beta_star);
+  h_.gamma2 = sst_.beta2 / sst_.beta_star - sst_.sigma_w2 * sst_.kappa * sst_.kappa / std::sqrt(sst_.beta_star);
+  h_.betaStarQ = std::pow(sst_.beta_star, 0.25);
+}
+
+void Engine::uploadInst() {
+  fillInstScalars();
+  check(hipMemcpyAsync(dInst_, &h_, sizeof(Inst), hipMemcpyHostToDevice, stream_), "upload descriptor");
+  check(hipStreamSynchronize(stream_), "descriptor sync");
+}
+
+void Engine::setSettings(const cfdgpu_settings& s) {
+  settings_ = s;
+  if (initialised_) uploadInst();
+}
+
+// SimpleSolver.cpp:212-230
+int Engine::setModel(int model, const cfdgpu_sst_constants* constants) {
+  if (model < CFDGPU_MODEL_NONE || model > CFDGPU_MODEL_SST) return CFDGPU_INVALID_ARGUMENT;
+  model_ = model;
+  if (constants) sst_ = *constants;
+  else cfdgpu_default_sst(&sst_);
+  if (!initialised_) return CFDGPU_OK;
+  // attached after initialise: bring the model up on the current field
+  const size_t N = static_cast<size_t>(grid_.cells());
+  std::vector<double> u(N), v(N), p(N);
+  getField(u.data(), v.data(), p.data(), nullptr);
+  u_ = u;
+  v_ = v;
+  p_ = p;
+  if (model_ == CFDGPU_MODEL_NONE) {
+    mu_ = muMol_;
+  } else if (model_ == CFDGPU_MODEL_SST) {
+    if (!sstInitialise()) return CFDGPU_INVALID_ARGUMENT;
+    applyEddyViscosity();
+  }
+  check(hipMemcpyAsync(h_.mu, mu_.data(), N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload mu");
+  if (model_ == CFDGPU_MODEL_SST) {
+    check(hipMemcpyAsync(h_.k, k_.data(), N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload k");
+    check(hipMemcpyAsync(h_.w, w_.data(), N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload w");
+    check(hipMemcpyAsync(h_.mut, mut_.data(), N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload mut");
+    check(hipMemcpyAsync(const_cast<double*>(h_.wallDist), wallDist_.data(), N * sizeof(double),
+                         hipMemcpyHostToDevice, stream_), "upload d");
+  }
+  uploadInst();
+  return CFDGPU_OK;
+}
+
+// KOmegaSST.cpp:104-200 with wallDistances (TurbulenceModel.cpp:28-65)
+bool Engine::sstInitialise() {
+  if (!std::isfinite(settings_.intensity) || settings_.intensity < 0.0) return false;
+  if (!std::isfinite(settings_.viscosity_ratio) || settings_.viscosity_ratio <= 0.0) return false;
+  if (!std::isfinite(settings_.length_scale) || settings_.length_scale < 0.0) return false;
+  const Grid& g = grid_;
+  const size_t N = static_cast<size_t>(g.cells());
+  wallDist_.assign(N, 0.0);
+  std::vector<Vec2> walls;
+  auto gather = [&](int refFace, Vec2 centre) {
+    if (kind_[static_cast<size_t>(refFace)] == CFDGPU_K_WALL) walls.push_back(centre);
+  };
+  // reference face order: j-family (row-major), then i-family (column-major)
+  for (int j = 0; j <= g.nJ; ++j)
+    for (int i = 0; i < g.nI; ++i) gather(g.refJFace(i, j), g.jCentre[static_cast<size_t>(j * g.nI + i)]);
+  for (int i = 0; i <= g.nI; ++i)
+    for (int j = 0; j < g.nJ; ++j) gather(g.refIFace(i, j), g.iCentre[static_cast<size_t>(j * (g.nI + 1) + i)]);
+  if (walls.empty()) {
+    const double far = 1.0e6 * std::sqrt(std::abs(g.totalArea) + 1.0);
+    std::fill(wallDist_.begin(), wallDist_.end(), far);
+  } else {
+    for (size_t c = 0; c < N; ++c) {
+      const Vec2 cc = g.centroid[c];
+      double best = std::numeric_limits<double>::max();
+      for (const Vec2& w : walls) {
+        const double dx = cc.x - w.x, dy = cc.y - w.y;
+        best = std::min(best, dx * dx + dy * dy);
+      }
+      wallDist_[c] = std::sqrt(best);
+    }
+  }
+  double refSpeed = 0.0, refRho = 0.0, refMu = 0.0;
+  for (size_t c = 0; c < N; ++c) {
+    const double sp = std::hypot(u_[c], v_[c]);
+    if (sp > refSpeed) refSpeed = sp;
+    refRho = std::max(refRho, rho_[c]);
+    refMu = std::max(refMu, mu_[c]);
+  }
+  if (!(refSpeed > 0.0) || !(refRho > 0.0) || !(refMu > 0.0)) return false;
+  const double intensity = std::max(settings_.intensity, 1.0e-8);
+  kIn_ = 1.5 * (intensity * refSpeed) * (intensity * refSpeed);
+  double eddy = 0.0;
+  if (settings_.length_scale > 0.0) {
+    wIn_ = std::max(std::sqrt(kIn_) / (std::pow(sst_.beta_star, 0.25) * settings_.length_scale), kMinOmega);
+    eddy = refRho * kIn_ / wIn_;
+  } else {
+    eddy = settings_.viscosity_ratio * refMu;
+    wIn_ = std::max(refRho * kIn_ / eddy, kMinOmega);
+  }
+  k_.assign(N, kIn_);
+  w_.assign(N, wIn_);
+  mut_.assign(N, eddy);
+  const double bq = std::pow(sst_.beta_star, 0.25);
+  for (size_t c = 0; c < N; ++c) {
+    const double d = std::max(wallDist_[c], kTinySst);
+    const double nu = mu_[c] / std::max(rho_[c], kTinySst);
+    // wallOmega (KOmegaSST.cpp:86-92)
+    const double dd = std::max(d, kTinySst);
+    const double viscous = 6.0 * nu / (sst_.beta1 * dd * dd);
+    const double logarithmic = std::sqrt(std::max(k_[c], 0.0)) / (bq * sst_.kappa * dd);
+    const double wall = std::sqrt(viscous * viscous + logarithmic * logarithmic);
+    w_[c] = std::max(wIn_, wall);
+    mut_[c] = rho_[c] * k_[c] / w_[c];
+  }
+  return true;
+}
+
+// SimpleSolver.cpp:232-245
+void Engine::applyEddyViscosity() {
+  if (model_ == CFDGPU_MODEL_SST) {
+    for (size_t c = 0; c < mu_.size(); ++c) mu_[c] = muMol_[c] + mut_[c];
+  } else if (model_ == CFDGPU_MODEL_LAMINAR) {
+    for (size_t c = 0; c < mu_.size(); ++c) mu_[c] = muMol_[c] + 0.0;
+  }
+}
+
+// SimpleSolver.cpp:133-210: everything here is per-solve setup in FP64 on the host.
+int Engine::initialise(const double* u, const double* v, const double* p, const double* rho, const double* mu) {
+  const Grid& g = grid_;
+  const int nI = g.nI, nJ = g.nJ;
+  const size_t N = static_cast<size_t>(g.cells());
+  u_.assign(u, u + N);
+  v_.assign(v, v + N);
+  p_.assign(p, p + N);
+  rho_.assign(rho, rho + N);
+  mu_.assign(mu, mu + N);
+  muMol_ = mu_;
+  // far-field dead band from the starting field (:150-166)
+  double refSpeed = 0.0;
+  for (size_t c = 0; c < N; ++c) refSpeed = std::max(refSpeed, std::hypot(u_[c], v_[c]));
+  auto deadFor = [&](FaceBC& bc, int owner, const FaceGeo& f) {
+    bc.dead = 0.0;
+    if (bc.kind != CFDGPU_K_FARFIELD) return;
+    bc.dead = 1.0e-3 * rho_[static_cast<size_t>(owner)] * refSpeed * std::hypot(f.Sx, f.Sy);
+  };
+  for (int i = 0; i < nI; ++i) {
+    deadFor(bcB_[static_cast<size_t>(i)], i, jg_[static_cast<size_t>(i)]);
+    deadFor(bcT_[static_cast<size_t>(i)], (nJ - 1) * nI + i, jg_[static_cast<size_t>(nJ * nI + i)]);
+  }
+  for (int j = 0; j < nJ; ++j) {
+    deadFor(bcL_[static_cast<size_t>(j)], j * nI, ig_[static_cast<size_t>(j * (nI + 1))]);
+    deadFor(bcR_[static_cast<size_t>(j)], j * nI + nI - 1, ig_[static_cast<size_t>(j * (nI + 1) + nI)]);
+  }
+  // face values with zero flux, then the seeded flux F = rho_f (u_f . S) (:168-183)
+  Fj_.assign(static_cast<size_t>(g.jFaces()), 0.0);
+  Fi_.assign(static_cast<size_t>(g.iFaces()), 0.0);
+  auto imposesHost = [](const FaceBC& bc, double F) {
+    return bc.kind == CFDGPU_K_WALL || bc.kind == CFDGPU_K_INLET || (bc.kind == CFDGPU_K_FARFIELD && F < bc.dead);
+  };
+  auto seedInterior = [&](const FaceGeo& f, size_t o, size_t n, bool trueNeighbour) {
+    const double w = f.w;
+    const double fu = w * u_[o] + (1.0 - w) * u_[n];
+    const double fv = w * v_[o] + (1.0 - w) * v_[n];
+    double dens = rho_[o];
+    if (trueNeighbour) dens = w * rho_[o] + (1.0 - w) * rho_[n];
+    return dens * (fu * f.Sx + fv * f.Sy);
+  };
+  auto seedBoundary = [&](const FaceGeo& f, const FaceBC& bc, size_t o) {
+    const bool imp = imposesHost(bc, 0.0);
+    const double fu = imp ? bc.u : u_[o];
+    const double fv = imp ? bc.v : v_[o];
+    return rho_[o] * (fu * f.Sx + fv * f.Sy);
+  };
+  for (int j = 0; j <= nJ; ++j) {
+    for (int i = 0; i < nI; ++i) {
+      const size_t f = static_cast<size_t>(j * nI + i);
+      if (j > 0 && j < nJ) Fj_[f] = seedInterior(jg_[f], static_cast<size_t>((j - 1) * nI + i), static_cast<size_t>(j * nI + i), true);
+      else if (j == 0) {
+        const int pr = g.bottomPartner[static_cast<size_t>(i)];
+        if (pr >= 0) Fj_[f] = seedInterior(jg_[f], static_cast<size_t>(i), static_cast<size_t>(pr), false);
+        else Fj_[f] = seedBoundary(jg_[f], bcB_[static_cast<size_t>(i)], static_cast<size_t>(i));
+      } else {
+        Fj_[f] = seedBoundary(jg_[f], bcT_[static_cast<size_t>(i)], static_cast<size_t>((nJ - 1) * nI + i));
+      }
+    }
+  }
+  for (int j = 0; j < nJ; ++j) {
+    for (int i = 0; i <= nI; ++i) {
+      const size_t f = static_cast<size_t>(j * (nI + 1) + i);
+      if (i > 0 && i < nI) Fi_[f] = seedInterior(ig_[f], static_cast<size_t>(j * nI + i - 1), static_cast<size_t>(j * nI + i), true);
+      else if (i == 0) Fi_[f] = seedBoundary(ig_[f], bcL_[static_cast<size_t>(j)], static_cast<size_t>(j * nI));
+      else Fi_[f] = seedBoundary(ig_[f], bcR_[static_cast<size_t>(j)], static_cast<size_t>(j * nI + nI - 1));
+    }
+  }
+  // inflow through the boundary, for normalising continuity (:188-198), reference face order
+  refMassFlow_ = 0.0;
+  auto addInflow = [&](int refFace, double F) {
+    const int type = g.faceType(refFace);
+    if (type == kInterior || kind_[static_cast<size_t>(refFace)] == CFDGPU_K_INTERNAL) return;
+    refMassFlow_ += std::max(-F, 0.0);
+  };
+  for (int j = 0; j <= nJ; ++j)
+    for (int i = 0; i < nI; ++i) addInflow(g.refJFace(i, j), Fj_[static_cast<size_t>(j * nI + i)]);
+  for (int i = 0; i <= nI; ++i)
+    for (int j = 0; j < nJ; ++j) addInflow(g.refIFace(i, j), Fi_[static_cast<size_t>(j * (nI + 1) + i)]);
+  if (!(refMassFlow_ > 0.0)) refMassFlow_ = 1.0;
+  if (model_ == CFDGPU_MODEL_SST) {
+    if (!sstInitialise()) return CFDGPU_INVALID_ARGUMENT;
+  }
+  applyEddyViscosity();
+  // upload state
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "upload state");
+  };
+  up(h_.u, u_.data(), N * sizeof(double));
+  up(h_.v, v_.data(), N * sizeof(double));
+  up(h_.p, p_.data(), N * sizeof(double));
+  up(h_.rho, rho_.data(), N * sizeof(double));
+  up(h_.mu, mu_.data(), N * sizeof(double));
+  up(h_.muMol, muMol_.data(), N * sizeof(double));
+  up(h_.Fj, Fj_.data(), Fj_.size() * sizeof(double));
+  up(h_.Fi, Fi_.data(), Fi_.size() * sizeof(double));
+  up(const_cast<FaceBC*>(h_.bcB), bcB_.data(), bcB_.size() * sizeof(FaceBC));
+  up(const_cast<FaceBC*>(h_.bcT), bcT_.data(), bcT_.size() * sizeof(FaceBC));
+  up(const_cast<FaceBC*>(h_.bcL), bcL_.data(), bcL_.size() * sizeof(FaceBC));
+  up(const_cast<FaceBC*>(h_.bcR), bcR_.data(), bcR_.size() * sizeof(FaceBC));
+  if (model_ == CFDGPU_MODEL_SST) {
+    up(h_.k, k_.data(), N * sizeof(double));
+    up(h_.w, w_.data(), N * sizeof(double));
+    up(h_.mut, mut_.data(), N * sizeof(double));
+    up(const_cast<double*>(h_.wallDist), wallDist_.data(), N * sizeof(double));
+  }
+  // momentumDiagonal_ starts at 1 (SimpleSolver.cpp:64)
+  std::vector<double> ones(N, 1.0);
+  up(h_.aP, ones.data(), N * sizeof(double));
+  std::vector<int> flags(kIFlags, 0);
+  up(h_.flags, flags.data(), flags.size() * sizeof(int));
+  check(hipStreamSynchronize(stream_), "initialise sync");
+  initialised_ = true;
+  h_.stopTol = 0.0;
+  h_.stopMaxIter = 0;
+  uploadInst();
+  return CFDGPU_OK;
+}
+
+void Engine::readMonitor(cfdgpu_monitor* out) const {
+  DevMonitor m{};
+  check(hipMemcpyAsync(&m, h_.monitor, sizeof(DevMonitor), hipMemcpyDeviceToHost, stream_), "read monitor");
+  check(hipStreamSynchronize(stream_), "monitor sync");
+  out->continuity = m.continuity;
+  out->momentum_x = m.momentumX;
+  out->momentum_y = m.momentumY;
+  out->mass_imbalance = m.massImbalance;
+  out->max_divergence = m.maxDivergence;
+  out->pressure_iterations = m.pressureIterations;
+  out->turb_first = m.turbFirst;
+  out->turb_second = m.turbSecond;
+  out->min_k = m.minK;
+  out->max_k = m.maxK;
+  out->min_w = m.minW;
+  out->max_w = m.maxW;
+  out->min_mut = m.minMut;
+  out->max_mut = m.maxMut;
+  out->max_eddy_ratio = m.maxEddyRatio;
+}
+
+int Engine::iterate(int count, cfdgpu_monitor* out) {
+  if (!initialised_) return CFDGPU_INITIALIZATION_FAILURE;
+  // a plain iterate() never stops by itself (SimpleSolver::iterate never fails)
+  int zero[2] = {0, 0};
+  check(hipMemcpyAsync(h_.flags + fStopped, zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
+  for (int n = 0; n < count; ++n) launchIteration(dInst_, cfg_, h_, stream_);
+  check(hipGetLastError(), "launch");
+  if (out) readMonitor(out);
+  else check(hipStreamSynchronize(stream_), "iterate sync");
+  return CFDGPU_OK;
+}
+
+// SolverWorker::runLoop stop rules (SolverWorker.cpp:161-192), evaluated on
+// the device after every outer iteration; the host only polls.
+int Engine::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
+  if (!initialised_) return CFDGPU_INITIALIZATION_FAILURE;
+  const auto t0 = std::chrono::steady_clock::now();
+  int startIter = 0;
+  check(hipMemcpyAsync(&startIter, h_.flags + fIteration, sizeof(int), hipMemcpyDeviceToHost, stream_), "flags");
+  check(hipStreamSynchronize(stream_), "sync");
+  h_.stopTol = rule.tolerance;
+  h_.stopMaxIter = (rule.max_iterations > 0) ? startIter + rule.max_iterations : 0;
+  uploadInst();
+  int zero = 0;
+  check(hipMemcpyAsync(h_.flags + fStopped, &zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
+  const int chunk = std::max(1, rule.check_every);
+  int launched = 0;
+  int flags[kIFlags] = {};
+  for (;;) {
+    const int n = (rule.max_iterations > 0) ? std::min(chunk, rule.max_iterations - launched) : chunk;
+    for (int q = 0; q < n; ++q) launchIteration(dInst_, cfg_, h_, stream_);
+    launched += n;
+    check(hipGetLastError(), "launch");
+    check(hipMemcpyAsync(flags, h_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream_), "flags");
+    check(hipStreamSynchronize(stream_), "solve sync");
+    if (flags[fStopped]) break;
+    if (rule.max_iterations > 0 && launched >= rule.max_iterations) break;
+  }
+  if (out) {
+    out->iterations = flags[fIteration] - startIter;
+    out->status = flags[fStopped] ? flags[fStatus] : 1;
+    readMonitor(&out->last);
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  h_.stopTol = 0.0;
+  h_.stopMaxIter = 0;
+  uploadInst();
+  return CFDGPU_OK;
+}
+
+int Engine::getField(double* u, double* v, double* p, double* mu) const {
+  const size_t N = static_cast<size_t>(grid_.cells());
+  auto down = [&](double* dst, const double* src) {
+    if (dst) check(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToHost, stream_), "download");
+  };
+  down(u, h_.u);
+  down(v, h_.v);
+  down(p, h_.p);
+  down(mu, h_.mu);
+  check(hipStreamSynchronize(stream_), "download sync");
+  return CFDGPU_OK;
+}
+
+int Engine::setField(const double* u, const double* v, const double* p) {
+  const size_t N = static_cast<size_t>(grid_.cells());
+  auto up = [&](double* dst, const double* src) {
+    if (src) check(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload");
+  };
+  up(h_.u, u);
+  up(h_.v, v);
+  up(h_.p, p);
+  check(hipStreamSynchronize(stream_), "upload sync");
+  return CFDGPU_OK;
+}
+
+// device faces -> reference numbering
+int Engine::getMassFlux(double* flux) const {
+  const Grid& g = grid_;
+  std::vector<double> fj(static_cast<size_t>(g.jFaces())), fi(static_cast<size_t>(g.iFaces()));
+  check(hipMemcpyAsync(fj.data(), h_.Fj, fj.size() * sizeof(double), hipMemcpyDeviceToHost, stream_), "flux");
+  check(hipMemcpyAsync(fi.data(), h_.Fi, fi.size() * sizeof(double), hipMemcpyDeviceToHost, stream_), "flux");
+  check(hipStreamSynchronize(stream_), "flux sync");
+  std::copy(fj.begin(), fj.end(), flux);
+  for (int i = 0; i <= g.nI; ++i)
+    for (int j = 0; j < g.nJ; ++j) flux[g.refIFace(i, j)] = fi[static_cast<size_t>(j * (g.nI + 1) + i)];
+  return CFDGPU_OK;
+}
+
+int Engine::setMassFlux(const double* flux) {
+  const Grid& g = grid_;
+  std::vector<double> fi(static_cast<size_t>(g.iFaces()));
+  for (int i = 0; i <= g.nI; ++i)
+    for (int j = 0; j < g.nJ; ++j) fi[static_cast<size_t>(j * (g.nI + 1) + i)] = flux[g.refIFace(i, j)];
+  check(hipMemcpyAsync(h_.Fj, flux, static_cast<size_t>(g.jFaces()) * sizeof(double), hipMemcpyHostToDevice, stream_), "flux");
+  check(hipMemcpyAsync(h_.Fi, fi.data(), fi.size() * sizeof(double), hipMemcpyHostToDevice, stream_), "flux");
+  check(hipStreamSynchronize(stream_), "flux sync");
+  return CFDGPU_OK;
+}
+
+int Engine::getTurbulence(double* k, double* omega, double* mut, double* wd) const {
+  if (model_ != CFDGPU_MODEL_SST) return CFDGPU_NOT_FOUND;
+  const size_t N = static_cast<size_t>(grid_.cells());
+  auto down = [&](double* dst, const double* src) {
+    if (dst) check(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyDeviceToHost, stream_), "download");
+  };
+  down(k, h_.k);
+  down(omega, h_.w);
+  down(mut, h_.mut);
+  down(wd, h_.wallDist);
+  check(hipStreamSynchronize(stream_), "download sync");
+  return CFDGPU_OK;
+}
+
+int Engine::setTurbulence(const double* k, const double* omega, const double* mut) {
+  if (model_ != CFDGPU_MODEL_SST) return CFDGPU_NOT_FOUND;
+  const size_t N = static_cast<size_t>(grid_.cells());
+  auto up = [&](double* dst, const double* src) {
+    if (src) check(hipMemcpyAsync(dst, src, N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload");
+  };
+  up(h_.k, k);
+  up(h_.w, omega);
+  up(h_.mut, mut);
+  if (mut) {
+    std::vector<double> mu(N);
+    for (size_t c = 0; c < N; ++c) mu[c] = muMol_[c] + mut[c];
+    check(hipMemcpyAsync(h_.mu, mu.data(), N * sizeof(double), hipMemcpyHostToDevice, stream_), "upload");
+  }
+  check(hipStreamSynchronize(stream_), "upload sync");
+  return CFDGPU_OK;
+}
+
+int Engine::turbulenceInfo(double* kIn, double* wIn, double* yplus) const {
+  if (kIn) *kIn = kIn_;
+  if (wIn) *wIn = wIn_;
+  if (yplus) {
+    check(hipMemcpyAsync(yplus, h_.red + sMaxYPlus, sizeof(double), hipMemcpyDeviceToHost, stream_), "y+");
+    check(hipStreamSynchronize(stream_), "sync");
+  }
+  return CFDGPU_OK;
+}
+
+int Engine::divergence(double* out) const {
+  double* d = nullptr;
+  const size_t N = static_cast<size_t>(grid_.cells());
+  check(hipMallocAsync(reinterpret_cast<void**>(&d), N * sizeof(double), stream_), "hipMalloc");
+  int zero = 0;
+  check(hipMemcpyAsync(h_.flags + fStopped, &zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
+  launchDivergence(dInst_, cfg_, d, stream_);
+  check(hipMemcpyAsync(out, d, N * sizeof(double), hipMemcpyDeviceToHost, stream_), "download");
+  check(hipFreeAsync(d, stream_), "hipFree");
+  check(hipStreamSynchronize(stream_), "sync");
+  return CFDGPU_OK;
+}
+
+int Engine::debugArray(const std::string& name, double* out, int capacity) const {
+  const size_t N = static_cast<size_t>(grid_.cells());
+  const double* src = nullptr;
+  size_t count = N;
+  const double2* vec = nullptr;
+  if (name == "gradU") vec = h_.gU;
+  else if (name == "gradV") vec = h_.gV;
+  else if (name == "gradP") vec = h_.gP;
+  else if (name == "momDiag") src = h_.aP;
+  else if (name == "momS") src = h_.aS;
+  else if (name == "momN") src = h_.aN;
+  else if (name == "momW") src = h_.aW;
+  else if (name == "momE") src = h_.aE;
+  else if (name == "momBx") src = h_.bx;
+  else if (name == "momBy") src = h_.by;
+  else if (name == "pDiag") src = h_.pP;
+  else if (name == "pN") src = h_.pN;
+  else if (name == "pE") src = h_.pE;
+  else if (name == "pWake") src = h_.pWk;
+  else if (name == "pSource") src = h_.pb;
+  else if (name == "correction") src = h_.x;
+  else if (name == "strain") src = h_.strain;
+  else if (name == "f1") src = h_.f1;
+  else if (name == "f2") src = h_.f2;
+  else if (name == "crossDiffusion") src = h_.cd;
+  else return -CFDGPU_NOT_FOUND;
+  if (vec) {
+    count = 2 * N;
+    if (static_cast<int>(count) > capacity) return -CFDGPU_OUT_OF_RANGE;
+    check(hipMemcpyAsync(out, vec, N * sizeof(double2), hipMemcpyDeviceToHost, stream_), "debug");
+  } else {
+    if (static_cast<int>(count) > capacity) return -CFDGPU_OUT_OF_RANGE;
+    check(hipMemcpyAsync(out, src, N * sizeof(double), hipMemcpyDeviceToHost, stream_), "debug");
+  }
+  check(hipStreamSynchronize(stream_), "debug sync");
+  return static_cast<int>(count);
+}
+
+}  // namespace cfdgpu

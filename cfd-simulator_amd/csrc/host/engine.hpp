@@ -1,0 +1,115 @@
+// engine.hpp - host side of the MI355X SIMPLE solver: setup (FP64, on the
+// host, once per solve), device residency, and the launch schedule of one
+// outer iteration. The C ABI (capi.cpp) and the C++ facade (SimpleSolver.hpp)
+// are thin layers over this.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/cfdgpu.h"
+#include "../device/layout.hpp"
+#include "grid.hpp"
+
+struct cfdgpu_mesh {
+  cfdgpu::Grid grid;
+};
+
+namespace cfdgpu {
+
+// Kernel launchers (defined in launch.hip).
+struct LaunchCfg {
+  int maxBlocks;   // grid.x for cell-parallel kernels
+  int batch;       // grid.y
+  int gsThreads;   // block size of the lexicographic sweep
+  size_t gsLds;    // dynamic LDS of the lexicographic sweep
+};
+void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hipStream_t s);
+void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
+int gsThreadsFor(int nJ);
+size_t gsLdsFor(int nJ, int nI, int threads);
+
+// Device arena for one instance: every array of Inst carved out of one allocation.
+struct DeviceArena {
+  void* base{nullptr};
+  size_t bytes{0};
+  ~DeviceArena();
+};
+
+class Engine {
+ public:
+  Engine(const Grid& grid, std::vector<int> kind, std::vector<Vec2> bcVelocity, std::vector<double> bcPressure,
+         const cfdgpu_settings& settings);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  int setModel(int model, const cfdgpu_sst_constants* constants);
+  int initialise(const double* u, const double* v, const double* p, const double* rho, const double* mu);
+  int iterate(int count, cfdgpu_monitor* out);
+  int solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out);
+  void setSettings(const cfdgpu_settings& s);
+  const cfdgpu_settings& settings() const { return settings_; }
+
+  int getField(double* u, double* v, double* p, double* mu) const;
+  int setField(const double* u, const double* v, const double* p);
+  int getMassFlux(double* flux) const;
+  int setMassFlux(const double* flux);
+  int getTurbulence(double* k, double* omega, double* mut, double* wd) const;
+  int setTurbulence(const double* k, const double* omega, const double* mut);
+  int turbulenceInfo(double* kIn, double* wIn, double* yplus) const;
+  int divergence(double* out) const;
+  int debugArray(const std::string& name, double* out, int capacity) const;
+
+  const Grid& grid() const { return grid_; }
+  bool ready() const { return initialised_; }
+  hipStream_t stream() const { return stream_; }
+  // for batching: the instance descriptor with device pointers
+  const Inst& inst() const { return h_; }
+  void uploadInst();
+
+ private:
+  void precomputeGeometry();
+  void allocate();
+  void uploadStatic();
+  bool sstInitialise();
+  void applyEddyViscosity();
+  void fillInstScalars();
+  void readMonitor(cfdgpu_monitor* out) const;
+
+  const Grid& grid_;
+  std::vector<int> kind_;
+  std::vector<Vec2> bcVel_;
+  std::vector<double> bcP_;
+  cfdgpu_settings settings_;
+  int model_{CFDGPU_MODEL_NONE};
+  cfdgpu_sst_constants sst_{};
+  bool initialised_{false};
+  bool hasPRef_{false};
+  double refMassFlow_{1.0};
+
+  // host-side FP64 setup (SimpleSolver.cpp:82-210)
+  std::vector<FaceGeo> jg_, ig_;
+  std::vector<double> jgSst_, igSst_;
+  std::vector<FaceBC> bcB_, bcT_, bcL_, bcR_;
+  std::vector<double> dead_;  // per reference face
+  // host mirrors used during initialise
+  std::vector<double> u_, v_, p_, rho_, mu_, muMol_, Fj_, Fi_;
+  std::vector<double> k_, w_, mut_, wallDist_;
+  double kIn_{0.0}, wIn_{0.0};
+
+  DeviceArena arena_;
+  Inst h_{};
+  Inst* dInst_{nullptr};
+  hipStream_t stream_{nullptr};
+  LaunchCfg cfg_{};
+};
+
+}  // namespace cfdgpu
+
+struct cfdgpu_solver {
+  std::unique_ptr<cfdgpu::Engine> engine;
+};

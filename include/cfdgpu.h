@@ -1,0 +1,226 @@
+/* cfdgpu.h - C ABI of the MI355X (gfx950) SIMPLE solver.
+ *
+ * Drop-in boundary for the reference's cfd::solver::SimpleSolver class API and
+ * its TurbulenceModel plugin (sidhanth-sridhar/CFD-SIMULATOR,
+ * include/cfd/solver/SimpleSolver.hpp:159-197, TurbulenceModel.hpp:155-206),
+ * plus the inputs it is built from (mesh, face conditions) and the outputs it
+ * feeds (surface data, forces, polar). Every entry point returns an int status
+ * whose values map 1:1 to the reference's ErrorCode
+ * (include/cfd/core/Error.hpp:37-46); 0 is success. No exception crosses the
+ * ABI. Arrays are caller-owned host buffers in the reference's numbering:
+ *   cell  c = j*(nodesI-1) + i
+ *   faces [0, nJfaces)            j-family, index j*(nodesI-1) + i
+ *         [nJfaces, nJfaces+nIf)  i-family, index nJfaces + i*(nodesJ-1) + j
+ * Device copies are owned by the solver handle. One handle per host thread.
+ */
+#ifndef CFDGPU_H
+#define CFDGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: include/cfd/core/Error.hpp:37-46 ---- */
+enum {
+  CFDGPU_OK = 0,
+  CFDGPU_INVALID_ARGUMENT = 1,
+  CFDGPU_OUT_OF_RANGE = 2,
+  CFDGPU_NOT_FOUND = 3,
+  CFDGPU_IO_FAILURE = 4,
+  CFDGPU_NOT_IMPLEMENTED = 5,
+  CFDGPU_INITIALIZATION_FAILURE = 6,
+  CFDGPU_NUMERICAL_FAILURE = 7,
+  CFDGPU_INTERNAL = 8
+};
+
+/* mesh::BoundaryType (include/cfd/mesh/Mesh.hpp:42-48) */
+enum { CFDGPU_INTERIOR = 0, CFDGPU_WALL = 1, CFDGPU_FARFIELD = 2, CFDGPU_OUTLET = 3, CFDGPU_WAKECUT = 4 };
+/* flow::BoundaryKind (include/cfd/flow/BoundaryConditions.hpp:34-60) */
+enum { CFDGPU_K_INTERNAL = 0, CFDGPU_K_INLET = 1, CFDGPU_K_OUTLET = 2, CFDGPU_K_FARFIELD = 3,
+       CFDGPU_K_WALL = 4 };
+/* turbulence closures */
+enum { CFDGPU_MODEL_NONE = 0, CFDGPU_MODEL_LAMINAR = 1, CFDGPU_MODEL_SST = 2 };
+/* arithmetic of the hot path */
+enum {
+  CFDGPU_PRECISION_FP64 = 0, /* plain FP64 path: the drop-in and the judge */
+  CFDGPU_PRECISION_MIXED = 1 /* FP32 sweeps/CG bulk, FP64 state, residuals and reductions */
+};
+/* Gauss-Seidel ordering of the momentum and turbulence sweeps */
+enum {
+  CFDGPU_GS_LEXICOGRAPHIC = 0, /* the reference's cell order, exactly (wavefront schedule) */
+  CFDGPU_GS_REDBLACK = 1       /* two-colour order: fully parallel, same fixed point */
+};
+/* pressure-correction solver */
+enum {
+  CFDGPU_PSOLVE_JACOBI_CG = 0, /* reference: Jacobi-PCG, L1 stop (LinearSystem.cpp:118-210) */
+  CFDGPU_PSOLVE_LINE_CG = 1    /* CG preconditioned by wall-normal line (tridiagonal) solves */
+};
+
+typedef struct cfdgpu_mesh cfdgpu_mesh;
+typedef struct cfdgpu_solver cfdgpu_solver;
+
+/* mesh::CGridOptions (include/cfd/mesh/CGrid.hpp:53-84) */
+typedef struct {
+  double upstream_chords, downstream_chords, vertical_chords;
+  int surface_points, wake_points, normal_points;
+  double first_layer_height;
+  double normal_blend_length;
+  int normal_smoothing_passes;
+} cfdgpu_cgrid_options;
+
+/* mesh::BoxOptions (include/cfd/mesh/BoxGrid.hpp:29-53) */
+typedef struct {
+  double origin_x, origin_y, length, height;
+  int cells_x, cells_y;
+  int grading; /* 0 uniform, 1 toward lower, 2 toward both */
+  double first_cell_height;
+  int left, right, lower, upper; /* CFDGPU_* boundary types */
+} cfdgpu_box_options;
+
+/* solver::SimpleSettings (include/cfd/solver/SimpleSolver.hpp:86-132) + TurbulenceInflow
+ * (TurbulenceModel.hpp:78-101) + the MI355X path selection. */
+typedef struct {
+  double velocity_relaxation, pressure_relaxation;
+  int momentum_sweeps, pressure_iterations;
+  double pressure_tolerance;
+  int scheme; /* 0 upwind, 1 second-order upwind */
+  double turbulence_relaxation;
+  double intensity, viscosity_ratio, length_scale;
+  int non_orthogonal_correctors;
+  double non_orthogonal_blend;
+  /* MI355X path selection (no reference counterpart) */
+  int precision;      /* CFDGPU_PRECISION_* */
+  int gs_ordering;    /* CFDGPU_GS_* */
+  int pressure_solver;/* CFDGPU_PSOLVE_* */
+  int refinement_passes; /* FP64-residual refinement passes of the pressure solve (mixed) */
+} cfdgpu_settings;
+
+/* solver::SSTConstants (include/cfd/solver/KOmegaSST.hpp:147-171) */
+typedef struct {
+  double sigma_k1, sigma_w1, beta1, sigma_k2, sigma_w2, beta2, beta_star, kappa, a1,
+      production_limit;
+} cfdgpu_sst_constants;
+
+/* solver::SolverMonitor (include/cfd/solver/SimpleSolver.hpp:135-154) */
+typedef struct {
+  double continuity, momentum_x, momentum_y;
+  double mass_imbalance, max_divergence;
+  int pressure_iterations;
+  double turb_first, turb_second;
+  double min_k, max_k, min_w, max_w, min_mut, max_mut;
+  double max_eddy_ratio;
+} cfdgpu_monitor;
+
+/* Stop rule of the caller loop (src/app/SolverWorker.cpp:161-192). */
+typedef struct {
+  double tolerance;   /* converged when residuals.worst() < tolerance */
+  int max_iterations; /* iteration limit */
+  int check_every;    /* monitor read-back interval (1 = every iteration, like the reference) */
+} cfdgpu_stop_rule;
+
+typedef struct {
+  int status;         /* 0 converged, 1 iteration-limit, 2 diverged (post/Polar.cpp:79-84) */
+  int iterations;
+  cfdgpu_monitor last;
+  double seconds;
+} cfdgpu_solve_report;
+
+/* post::AerodynamicForces (include/cfd/post/Forces.hpp) + separation summary */
+typedef struct {
+  double cl, cd, cm, cd_pressure, cd_friction, lift, drag, pitching_moment;
+  double fx_pressure, fy_pressure, fx_friction, fy_friction;
+  double upper_separation, lower_separation; /* chord fraction, -1 if attached */
+  double stagnation_xc;
+  int n_upper, n_lower;
+  int stations;
+  int upper_separation_face, lower_separation_face; /* wall face index (after), -1 if none */
+} cfdgpu_forces;
+
+/* ---- library ---- */
+const char* cfdgpu_version(void);
+const char* cfdgpu_last_error(void);
+int cfdgpu_device_count(int* count);
+
+/* ---- mesh (host side; replaces mesh::generateCGrid CGrid.hpp:92-96, generateBox
+ *      BoxGrid.hpp:53, buildStructured Mesh.hpp:209) ---- */
+int cfdgpu_cgrid_defaults(int resolution, cfdgpu_cgrid_options* out); /* CGrid.cpp:73-96 */
+int cfdgpu_mesh_cgrid(int m, int p, int t, double chord, const cfdgpu_cgrid_options* options,
+                      cfdgpu_mesh** out);
+int cfdgpu_mesh_box(const cfdgpu_box_options* options, cfdgpu_mesh** out);
+int cfdgpu_mesh_structured(int nodes_i, int nodes_j, const double* nodes_xy,
+                           const int* jmin_type, const int* jmin_partner, int jmax_type,
+                           int imin_type, int imax_type, cfdgpu_mesh** out);
+void cfdgpu_mesh_destroy(cfdgpu_mesh* mesh);
+int cfdgpu_mesh_dims(const cfdgpu_mesh* mesh, int* nodes_i, int* nodes_j, int* cells, int* faces);
+int cfdgpu_mesh_nodes(const cfdgpu_mesh* mesh, double* xy);
+int cfdgpu_mesh_cells(const cfdgpu_mesh* mesh, double* centroid_xy, double* area);
+int cfdgpu_mesh_faces(const cfdgpu_mesh* mesh, double* centre_xy, double* normal_xy,
+                      double* area, int* owner, int* neighbour, int* boundary, int* partner);
+/* min area, max area, max aspect, max non-orthogonality (deg), inverted, min wall spacing,
+ * total area (Mesh.hpp:80-94) */
+int cfdgpu_mesh_quality(const cfdgpu_mesh* mesh, double* out7);
+
+/* ---- flow inputs (flow::buildFaceConditions BoundaryConditions.hpp:111-113 with the default
+ *      BoundaryConditions; FlowField::uniform FlowField.hpp:55-57) ---- */
+int cfdgpu_face_conditions(const cfdgpu_mesh* mesh, double speed, double alpha_deg,
+                           double reference_pressure, int* kind, double* velocity_xy,
+                           double* pressure);
+int cfdgpu_uniform_field(int cells, double speed, double alpha_deg, double density,
+                         double reynolds, double chord, double reference_pressure, double* u,
+                         double* v, double* p, double* rho, double* mu);
+
+/* ---- the solver (SimpleSolver.hpp:159-197) ---- */
+void cfdgpu_default_settings(cfdgpu_settings* out);
+void cfdgpu_default_sst(cfdgpu_sst_constants* out);
+/* SimpleSolver::create: the mesh must outlive the solver (SimpleSolver.hpp:156-158) */
+int cfdgpu_create(cfdgpu_solver** out, const cfdgpu_mesh* mesh, const int* kind,
+                  const double* velocity_xy, const double* pressure,
+                  const cfdgpu_settings* settings);
+void cfdgpu_destroy(cfdgpu_solver* solver);
+/* SimpleSolver::setTurbulenceModel (SimpleSolver.cpp:212-230); constants NULL = defaults */
+int cfdgpu_set_turbulence(cfdgpu_solver* solver, int model, const cfdgpu_sst_constants* constants);
+/* SimpleSolver::initialise (SimpleSolver.cpp:133-210) */
+int cfdgpu_initialise(cfdgpu_solver* solver, const double* u, const double* v, const double* p,
+                      const double* rho, const double* mu);
+/* SimpleSolver::iterate (SimpleSolver.cpp:662-794), count times; monitor of the last */
+int cfdgpu_iterate(cfdgpu_solver* solver, int count, cfdgpu_monitor* out);
+/* the SolverWorker loop (SolverWorker.cpp:134-277) run device-resident */
+int cfdgpu_solve(cfdgpu_solver* solver, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out);
+int cfdgpu_get_settings(const cfdgpu_solver* solver, cfdgpu_settings* out);
+int cfdgpu_set_settings(cfdgpu_solver* solver, const cfdgpu_settings* settings);
+/* SimpleSolver::field() */
+int cfdgpu_get_field(const cfdgpu_solver* solver, double* u, double* v, double* p, double* mu);
+int cfdgpu_set_field(cfdgpu_solver* solver, const double* u, const double* v, const double* p);
+/* SimpleSolver::massFlux() */
+int cfdgpu_get_massflux(const cfdgpu_solver* solver, double* flux);
+int cfdgpu_set_massflux(cfdgpu_solver* solver, const double* flux);
+/* TurbulenceModel::turbulentEnergyField/dissipationField/eddyViscosity (+ wall distance) */
+int cfdgpu_get_turbulence(const cfdgpu_solver* solver, double* k, double* omega, double* mut,
+                          double* wall_distance);
+int cfdgpu_set_turbulence_fields(cfdgpu_solver* solver, const double* k, const double* omega,
+                                 const double* mut);
+int cfdgpu_turbulence_info(const cfdgpu_solver* solver, double* k_inflow, double* omega_inflow,
+                           double* max_wall_yplus);
+/* SimpleSolver::divergence() (SimpleSolver.cpp:796-817) */
+int cfdgpu_divergence(const cfdgpu_solver* solver, double* out);
+
+/* ---- post-processing, FP64 (SurfaceData.hpp:159, Forces.hpp:127-133) ----
+ * rows (12 doubles per station, upper surface then lower): side(1 upper/0 lower), face, x, y,
+ * x/c, arc length, p, Cp, tau_w, Cf, nearStagnation, segment length. */
+int cfdgpu_surface(const cfdgpu_mesh* mesh, const double* u, const double* v, const double* p,
+                   const double* mu, double speed, double alpha_deg, double density,
+                   double reference_pressure, double chord, double* rows, int capacity,
+                   int* station_count, cfdgpu_forces* forces);
+/* post::sweepAngles (Polar.cpp:36-77): returns count, or -status on error */
+int cfdgpu_sweep_angles(double start, double end, double step, double* out, int capacity);
+
+/* ---- staged access for parity tests (one SIMPLE stage on the current device state) ----
+ * names: "gradU" "gradV" "gradP" (cells, xy interleaved), "momDiag" "momS" "momN" "momW"
+ * "momE" "momBx" "momBy", "pDiag" "pN" "pE" "pWake" "pSource", "correction", "strain",
+ * "f1" "f2" "crossDiffusion" */
+int cfdgpu_debug_array(const cfdgpu_solver* solver, const char* name, double* out, int capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
