@@ -17,7 +17,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcfdgpu.so")
+LIB_PATH = os.environ.get("CFDGPU_LIBRARY", os.path.join(_HERE, "libcfdgpu.so"))
 
 INTERIOR, WALL, FARFIELD, OUTLET, WAKECUT = 0, 1, 2, 3, 4
 K_INTERNAL, K_INLET, K_OUTLET, K_FARFIELD, K_WALL = 0, 1, 2, 3, 4

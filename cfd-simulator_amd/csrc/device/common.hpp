@@ -79,24 +79,23 @@ __device__ __forceinline__ void blockMax(double (&v)[NV], double* sm) {
   __syncthreads();
 }
 
-// Publish this block's partials, count arrivals; true in every thread of the
-// block that arrives last (which then owns the final reduction). The counter
-// is reset by the last block, so the next kernel on the same stream reuses it.
-__device__ __forceinline__ bool arriveLast(unsigned* counter, unsigned total) {
-  __shared__ unsigned sLast;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    sLast = (prev == total - 1u) ? 1u : 0u;
-    if (sLast) {
-      __threadfence();
-      *counter = 0u;
-    }
+// Cross-block scalars: each kernel writes one partial per block and the
+// kernel that needs the total sums the partials itself, in a fixed order, at
+// its start (the kernel boundary makes them visible). No fences, no atomics:
+// deterministic, and nothing forces the per-XCD L2s to write back mid-kernel.
+// Every thread of the block gets the same value.
+__device__ __forceinline__ double blockSumPartials(const double* p, int n, double* sm) {
+  if (threadIdx.x < 64) {
+    double s = 0.0;
+    for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) s += p[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (threadIdx.x == 0) sm[0] = s;
   }
   __syncthreads();
-  const bool last = sLast != 0u;
-  if (last) __threadfence();
-  return last;
+  const double r = sm[0];
+  __syncthreads();
+  return r;
 }
 
 // Sum partials[0..n) of one slot with wave 0 in a fixed order; result in lane 0.

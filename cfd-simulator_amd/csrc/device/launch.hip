@@ -12,6 +12,7 @@ __global__ void k_begin_iteration(const Inst*, int);
 __global__ void k_grad(const Inst*);
 __global__ void k_mom(const Inst*);
 __global__ void k_gs_lex(const Inst*, int, int);
+__global__ void k_gs_lex_wide(const Inst*, int, int);
 __global__ void k_gs_color(const Inst*, int, int, int);
 __global__ void k_rhiechow(const Inst*);
 __global__ void k_pressure(const Inst*, int);
@@ -22,7 +23,7 @@ __global__ void k_corr_mean(const Inst*);
 __global__ void k_correct_cells(const Inst*);
 __global__ void k_correct_faces(const Inst*);
 __global__ void k_divergence(const Inst*, double*);
-__global__ void k_monitor(const Inst*, int);
+__global__ void k_monitor(const Inst*);
 __global__ void k_sst_blend(const Inst*);
 __global__ void k_sst_asm(const Inst*);
 __global__ void k_sst_wall(const Inst*);
@@ -37,6 +38,7 @@ int gsThreadsFor(int nJ) {
 
 size_t gsLdsFor(int nJ, int nI, int threads) {
   (void)nJ;
+  // [2][2][threads] step buffer + [2][nI] row 0
   return (static_cast<size_t>(2) * 2 * static_cast<size_t>(threads) + 2 * static_cast<size_t>(nI)) * sizeof(double);
 }
 
@@ -49,7 +51,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   hipLaunchKernelGGL(k_grad, grid, block, 0, s, d);
   hipLaunchKernelGGL(k_mom, grid, block, 0, s, d);
   if (lex) {
-    hipLaunchKernelGGL(k_gs_lex, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
+    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
+    else hipLaunchKernelGGL(k_gs_lex_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
   } else {
     for (int sw = 0; sw < h.momSweeps; ++sw) {
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 0, 0);
@@ -75,7 +78,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     hipLaunchKernelGGL(k_sst_blend, grid, block, 0, s, d);
     hipLaunchKernelGGL(k_sst_asm, grid, block, 0, s, d);
     if (lex) {
-      hipLaunchKernelGGL(k_gs_lex, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
+      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
+      else hipLaunchKernelGGL(k_gs_lex_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
     } else {
       for (int sw = 0; sw < 3; ++sw) {
         const int floor = (sw == 2) ? 1 : 0;
@@ -88,7 +92,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     hipLaunchKernelGGL(k_sst_wall, dim3(1, cfg.batch), block, 0, s, d);
     hipLaunchKernelGGL(k_sst_mut, grid, block, 0, s, d);
   }
-  hipLaunchKernelGGL(k_monitor, one, dim3(64), 0, s, d, cfg.batch);
+  hipLaunchKernelGGL(k_monitor, dim3(1, cfg.batch), dim3(64), 0, s, d);
 }
 
 void launchDivergence(const Inst* d, const LaunchCfg& cfg, double* out, hipStream_t s) {

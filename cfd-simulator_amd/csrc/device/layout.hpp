@@ -65,6 +65,7 @@ enum Slot {
   sMaxEddyRatio,
   sMaxYPlus,
   sRefineNorm,     // mixed path: FP64 residual norm of the pressure correction
+  sCgRz0, sCgRz1,  // CG: r.z of even / odd iterations (persisted by block 0)
   kSlots
 };
 
@@ -92,6 +93,7 @@ struct DevMonitor {
 };
 
 constexpr int kMaxRedBlocks = 4096;  // partial-sum capacity per slot
+
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 
 struct Inst {
@@ -135,6 +137,8 @@ struct Inst {
   double *pP, *pN, *pE, *pWk, *pb;
   // CG vectors
   double *x, *r, *z, *pd0, *pd1, *Ap;
+  // ping-pong buffers of the lexicographic sweep (old values of the previous sweep)
+  double *gsTmp0, *gsTmp1;
 
   // k-omega SST state and systems
   double *k, *w, *mut, *f1, *f2, *cd, *strain, *prod;

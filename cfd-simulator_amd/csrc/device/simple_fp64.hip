@@ -112,12 +112,6 @@ __global__ void __launch_bounds__(kBlock) k_grad(const Inst* insts) {
   }
   blockMax<1>(vmax, sm);
   if (threadIdx.x == 0) partialSlot(I, sVelScale)[blockIdx.x] = vmax[0];
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double s = waveMaxPartials(partialSlot(I, sVelScale), I.nBlocks);
-      if (threadIdx.x == 0) I.red[sVelScale] = (s > 0.0) ? s : 1.0;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -284,18 +278,6 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     partialSlot(I, sMomResY)[blockIdx.x] = part[1];
     partialSlot(I, sMomDiagL1)[blockIdx.x] = part[2];
   }
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double a = waveSumPartials(partialSlot(I, sMomResX), I.nBlocks);
-      const double b = waveSumPartials(partialSlot(I, sMomResY), I.nBlocks);
-      const double d = waveSumPartials(partialSlot(I, sMomDiagL1), I.nBlocks);
-      if (threadIdx.x == 0) {
-        I.red[sMomResX] = a;
-        I.red[sMomResY] = b;
-        I.red[sMomDiagL1] = d;
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -321,108 +303,188 @@ struct GsSys {
   int hasFloor;
 };
 
+// Everything a lane needs for one cell of its row that does not depend on
+// this sweep's updates: the row of the matrix, the right-hand side, the wake
+// partner and the old values of the cell, above and to the right. Old values
+// come from the previous sweep's buffer, which this sweep never writes, so
+// they can be read any number of steps ahead.
 template <int NC>
-__device__ void gsLexBody(const Inst& I, const GsSys& S, int sweeps) {
-  extern __shared__ double lds[];
+struct GsCell {
+  double aP, aS, aN, aW, aE;
+  double b[NC], xN[NC], xE[NC], xC[NC];
+  int pr;
+};
+
+template <int NC>
+__device__ __forceinline__ void gsFetch(const Inst& I, const GsSys& S, double* const* xOld, int jr, int k,
+                                        GsCell<NC>& g) {
+  const int nI = I.nI, nJ = I.nJ;
+  const int i = k - jr;
+  if (jr < nJ && i >= 0 && i < nI) {
+    const int c = jr * nI + i;
+    g.aP = S.aP[c];
+    g.aS = S.aS[c];
+    g.aN = S.aN[c];
+    g.aW = S.aW[c];
+    g.aE = S.aE[c];
+    g.pr = (jr == 0) ? I.partner[i] : -1;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      g.b[q] = S.b[q][c];
+      g.xC[q] = xOld[q][c];
+      g.xN[q] = (jr < nJ - 1) ? xOld[q][c + nI] : 0.0;
+      g.xE[q] = (i < nI - 1) ? xOld[q][c + 1] : 0.0;
+    }
+  }
+}
+
+// Barrier for the LDS hand-off between wavefront steps only: waits for this
+// wave's LDS writes, not for its outstanding global loads (the read-ahead),
+// which __syncthreads() would drain at every step. Global memory is not
+// shared between lanes within a sweep (old values are read from one buffer,
+// new values written to another), so no global fence is needed here.
+__device__ __forceinline__ void ldsBarrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt untouched
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <int NC, int CH>
+__device__ void gsLexBody(const Inst& I, const GsSys& S, double* const* tmp, int sweeps) {
+  HIP_DYNAMIC_SHARED(double, lds)
   const int nI = I.nI, nJ = I.nJ;
   const int R = static_cast<int>(blockDim.x);
   double* buf = lds;                 // [2][NC][R]: value just written by each row
   double* row0 = lds + 2 * NC * R;   // [NC][nI]: row j = 0, updated in place
   const int jr = static_cast<int>(threadIdx.x);
   const bool active = jr < nJ;
+  const int steps = nI + nJ - 1;
+  const int chunks = (steps + CH - 1) / CH;
   for (int sweep = 0; sweep < sweeps; ++sweep) {
+    // even sweeps read x and write tmp, odd sweeps the other way round
+    double* xOld[NC];
+    double* xNew[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      xOld[q] = (sweep & 1) ? tmp[q] : S.x[q];
+      xNew[q] = (sweep & 1) ? S.x[q] : tmp[q];
+    }
     for (int t = jr; t < nI; t += R) {
 #pragma unroll
-      for (int q = 0; q < NC; ++q) row0[q * nI + t] = S.x[q][t];
+      for (int q = 0; q < NC; ++q) row0[q * nI + t] = xOld[q][t];
     }
     __syncthreads();
     double xw[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) xw[q] = 0.0;
-    const int steps = nI + nJ - 1;
-    for (int k = 0; k < steps; ++k) {
-      const int i = k - jr;
-      if (active && i >= 0 && i < nI) {
-        const int c = jr * nI + i;
-        const double d = S.aP[c];
-        double xn[NC];
-        if (fabs(d) > 0.0) {
-          double off[NC];
+    GsCell<NC> cur[CH], nxt[CH];
 #pragma unroll
-          for (int q = 0; q < NC; ++q) off[q] = 0.0;
-          // below
-          if (jr > 0) {
-            const double a = S.aS[c];
+    for (int q = 0; q < CH; ++q) gsFetch<NC>(I, S, xOld, jr, q, cur[q]);
+    for (int ch = 0; ch < chunks; ++ch) {
+      if (ch + 1 < chunks) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) off[q] += a * buf[((k - 1) & 1) * NC * R + q * R + (jr - 1)];
-          } else {
-            const int pr = I.partner[i];
-            if (pr >= 0) {
-              const double a = S.aS[c];
+        for (int q = 0; q < CH; ++q) gsFetch<NC>(I, S, xOld, jr, (ch + 1) * CH + q, nxt[q]);
+      }
 #pragma unroll
-              for (int q = 0; q < NC; ++q) off[q] += a * row0[q * nI + pr];
+      for (int q = 0; q < CH; ++q) {
+        const int k = ch * CH + q;
+        if (k < steps) {
+          const int i = k - jr;
+          if (active && i >= 0 && i < nI) {
+            const GsCell<NC>& g = cur[q];
+            const int c = jr * nI + i;
+            double xn[NC];
+            if (fabs(g.aP) > 0.0) {
+              double off[NC];
+#pragma unroll
+              for (int m = 0; m < NC; ++m) off[m] = 0.0;
+              // cellFaces order: below, right, above, left (Mesh.cpp:206-213)
+              if (jr > 0) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) off[m] += g.aS * buf[((k - 1) & 1) * NC * R + m * R + (jr - 1)];
+              } else if (g.pr >= 0) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) off[m] += g.aS * row0[m * nI + g.pr];
+              }
+              if (i < nI - 1) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) off[m] += g.aE * g.xE[m];
+              }
+              if (jr < nJ - 1) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) off[m] += g.aN * g.xN[m];
+              }
+              if (i > 0) {
+#pragma unroll
+                for (int m = 0; m < NC; ++m) off[m] += g.aW * xw[m];
+              }
+#pragma unroll
+              for (int m = 0; m < NC; ++m) xn[m] = (g.b[m] - off[m]) / g.aP;
+            } else {
+#pragma unroll
+              for (int m = 0; m < NC; ++m) xn[m] = g.xC[m];  // row skipped (LinearSystem.cpp:86-88)
+            }
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+              xNew[m][c] = xn[m];
+              buf[(k & 1) * NC * R + m * R + jr] = xn[m];
+              xw[m] = xn[m];
+              if (jr == 0) row0[m * nI + i] = xn[m];
             }
           }
-          // right
-          if (i < nI - 1) {
-            const double a = S.aE[c];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) off[q] += a * ((jr == 0) ? row0[q * nI + i + 1] : S.x[q][c + 1]);
-          }
-          // above
-          if (jr < nJ - 1) {
-            const double a = S.aN[c];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) off[q] += a * S.x[q][c + nI];
-          }
-          // left
-          if (i > 0) {
-            const double a = S.aW[c];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) off[q] += a * xw[q];
-          }
-#pragma unroll
-          for (int q = 0; q < NC; ++q) xn[q] = (S.b[q][c] - off[q]) / d;
-        } else {
-#pragma unroll
-          for (int q = 0; q < NC; ++q) xn[q] = (jr == 0) ? row0[q * nI + i] : S.x[q][c];
-        }
-#pragma unroll
-        for (int q = 0; q < NC; ++q) {
-          S.x[q][c] = xn[q];
-          buf[(k & 1) * NC * R + q * R + jr] = xn[q];
-          xw[q] = xn[q];
-          if (jr == 0) row0[q * nI + i] = xn[q];
+          ldsBarrier();
         }
       }
-      __syncthreads();
-    }
-  }
-  if (S.hasFloor) {
-    for (int c = jr; c < I.N; c += R) {
 #pragma unroll
-      for (int q = 0; q < NC; ++q) S.x[q][c] = mx(S.x[q][c], S.floorValue);
+      for (int q = 0; q < CH; ++q) cur[q] = nxt[q];
+    }
+    __syncthreads();  // this sweep's writes are the next sweep's old values
+  }
+  // an odd number of sweeps leaves the result in tmp
+  const bool inTmp = (sweeps & 1) != 0;
+  for (int c = jr; c < I.N; c += R) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      double v = inTmp ? tmp[q][c] : S.x[q][c];
+      if (S.hasFloor) v = mx(v, S.floorValue);
+      S.x[q][c] = v;
     }
   }
 }
 
 // system 0: momentum (u, v); system 1: k; system 2: omega. blockIdx.x picks
-// the system among [first, first + count).
-__global__ void __launch_bounds__(1024) k_gs_lex(const Inst* insts, int first, int sweeps) {
+// the system among [first, first + count). Up to 256 rows: one lane per row
+// with a 4-cell (momentum) / 6-cell (k, omega) read-ahead held in registers.
+template <int CH2, int CH1>
+__device__ __forceinline__ void gsLexDispatch(const Inst* insts, int first, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped]) return;
   const int sys = first + static_cast<int>(blockIdx.x);
-  GsSys S{};
   if (sys == 0) {
-    S = GsSys{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
-    gsLexBody<2>(I, S, sweeps);
+    const GsSys S{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
+    double* const tmp[2] = {I.gsTmp0, I.gsTmp1};
+    gsLexBody<2, CH2>(I, S, tmp, sweeps);
   } else if (sys == 1) {
-    S = GsSys{I.kP, I.kS, I.kN, I.kW, I.kE, {I.kB, nullptr}, {I.k, nullptr}, 0.0, 1};
-    gsLexBody<1>(I, S, sweeps);
+    const GsSys S{I.kP, I.kS, I.kN, I.kW, I.kE, {I.kB, nullptr}, {I.k, nullptr}, 0.0, 1};
+    double* const tmp[2] = {I.gsTmp0, nullptr};
+    gsLexBody<1, CH1>(I, S, tmp, sweeps);
   } else {
-    S = GsSys{I.wP, I.wS, I.wN, I.wW, I.wE, {I.wB, nullptr}, {I.w, nullptr}, 1.0e-10, 1};
-    gsLexBody<1>(I, S, sweeps);
+    const GsSys S{I.wP, I.wS, I.wN, I.wW, I.wE, {I.wB, nullptr}, {I.w, nullptr}, 1.0e-10, 1};
+    double* const tmp[2] = {I.gsTmp1, nullptr};
+    gsLexBody<1, CH1>(I, S, tmp, sweeps);
   }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_gs_lex(const Inst* insts, int first, int sweeps) {
+  gsLexDispatch<4, 6>(insts, first, sweeps);
+}
+
+// More than 256 rows (up to 1024): no read-ahead, the register budget of a
+// 1024-lane block does not hold one.
+__global__ void __launch_bounds__(1024) k_gs_lex_wide(const Inst* insts, int first, int sweeps) {
+  gsLexDispatch<1, 1>(insts, first, sweeps);
 }
 
 // ---------------------------------------------------------------------------
@@ -645,18 +707,8 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
   }
   blockSum<2>(part, sm);
   if (threadIdx.x == 0) {
-    partialSlot(I, sNetFluxL1)[blockIdx.x] = part[0];
+    if (pass == 0) partialSlot(I, sNetFluxL1)[blockIdx.x] = part[0];
     partialSlot(I, sSourceSum)[blockIdx.x] = part[1];
-  }
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double a = waveSumPartials(partialSlot(I, sNetFluxL1), I.nBlocks);
-      const double b = waveSumPartials(partialSlot(I, sSourceSum), I.nBlocks);
-      if (threadIdx.x == 0) {
-        if (pass == 0) I.red[sNetFluxL1] = a;
-        I.red[sSourceSum] = b;
-      }
-    }
   }
 }
 
@@ -687,15 +739,16 @@ __device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c
 __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
   CFD_PROLOGUE
   __shared__ double sm[16];
+  double mean = 0.0;
+  if (!I.hasPRef) mean = blockSumPartials(partialSlot(I, sSourceSum), I.nBlocks, sm) / static_cast<double>(I.N);
   double part[2] = {0.0, 0.0};
   if (live) {
     double b = I.pb[c];
     if (!I.hasPRef) {
-      const double mean = I.red[sSourceSum] / static_cast<double>(I.N);
       b -= mean;
       I.pb[c] = b;
     }
-    const double r = b - 0.0;
+    const double r = b - 0.0;  // x0 = 0, so A x0 = 0
     const double d = I.pP[c];
     const double z = (fabs(d) > 0.0) ? r / d : r;
     I.r[c] = r;
@@ -707,29 +760,44 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
   }
   blockSum<2>(part, sm);
   if (threadIdx.x == 0) {
-    partialSlot(I, sCgNorm0)[blockIdx.x] = part[0];
+    partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
     partialSlot(I, sCgRz)[blockIdx.x] = part[1];
-  }
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double n0 = waveSumPartials(partialSlot(I, sCgNorm0), I.nBlocks);
-      const double rz = waveSumPartials(partialSlot(I, sCgRz), I.nBlocks);
-      if (threadIdx.x == 0) {
-        I.red[sCgNorm0] = n0;
-        I.red[sCgNorm] = n0;
-        I.red[sCgRz] = rz;
-        I.red[sCgBeta] = 0.0;
-        I.flags[fCgIters] = 0;
-        I.flags[fCgDone] = (n0 <= 0.0) ? 1 : 0;
-      }
+    if (blockIdx.x == 0) {
+      I.flags[fCgDone] = 0;
+      I.flags[fCgIters] = 0;
     }
   }
 }
 
+// Iteration `it`, launch A. Finishes the test of iteration it-1 from the
+// partials of launch B (|r|_1 <= tol |r0|_1, then the r.z guard), forms beta,
+// then p = z + beta p_old (neighbours' p recomputed locally, so no second
+// launch is needed before the stencil), Ap and p.Ap.
 __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
   CFD_PROLOGUE
   if (I.flags[fCgDone]) return;
   __shared__ double sm[16];
+  const double nrm = blockSumPartials(partialSlot(I, sCgNorm), I.nBlocks, sm);
+  const double rzNew = blockSumPartials(partialSlot(I, sCgRz), I.nBlocks, sm);
+  double beta = 0.0;
+  bool done = false;
+  if (it == 0) {
+    if (!(nrm > 0.0)) done = true;  // already solved (LinearSystem.cpp:144-149)
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+      I.red[sCgNorm0] = nrm;
+      I.red[sCgRz0] = rzNew;
+    }
+  } else {
+    const double rzOld = I.red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
+    if (nrm <= I.cgTol * I.red[sCgNorm0]) done = true;
+    else if (!(fabs(rzOld) > 0.0)) done = true;
+    else beta = rzNew / rzOld;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) I.red[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
+  }
+  if (done) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    return;
+  }
   double part[1] = {0.0};
   if (live) {
     double* pNew = (it & 1) ? I.pd1 : I.pd0;
@@ -740,7 +808,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
       pc = pNew[c];
       y = spmvPres(I, pNew, c, i, j);
     } else {
-      const double beta = I.red[sCgBeta];
       auto pv = [&](int q) { return I.z[q] + beta * pOld[q]; };
       pc = pv(c);
       y = I.pP[c] * pc;
@@ -759,26 +826,23 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
   }
   blockSum<1>(part, sm);
   if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double pap = waveSumPartials(partialSlot(I, sCgPAp), I.nBlocks);
-      if (threadIdx.x == 0) {
-        I.red[sCgPAp] = pap;
-        if (!(fabs(pap) > 0.0)) I.flags[fCgDone] = 1;
-        else I.red[sCgAlpha] = I.red[sCgRz] / pap;
-      }
-    }
-  }
 }
 
+// Iteration `it`, launch B: alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap,
+// z = r / diag, and the partials of |r|_1 and r.z for the next launch A.
 __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
   CFD_PROLOGUE
   if (I.flags[fCgDone]) return;
   __shared__ double sm[16];
+  const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.nBlocks, sm);
+  if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
+    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    return;
+  }
+  const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
   double part[2] = {0.0, 0.0};
   if (live) {
     const double* p = (it & 1) ? I.pd1 : I.pd0;
-    const double alpha = I.red[sCgAlpha];
     const double x = I.x[c] + alpha * p[c];
     const double r = I.r[c] - alpha * I.Ap[c];
     I.x[c] = x;
@@ -793,45 +857,21 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
     partialSlot(I, sCgRz)[blockIdx.x] = part[1];
-  }
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double nrm = waveSumPartials(partialSlot(I, sCgNorm), I.nBlocks);
-      const double rzn = waveSumPartials(partialSlot(I, sCgRz), I.nBlocks);
-      if (threadIdx.x == 0) {
-        I.red[sCgNorm] = nrm;
-        I.flags[fCgIters] = it + 1;
-        I.flags[fCgTotal] += 1;
-        if (nrm <= I.cgTol * I.red[sCgNorm0]) {
-          I.flags[fCgDone] = 1;
-        } else {
-          const double rz = I.red[sCgRz];
-          if (!(fabs(rz) > 0.0)) {
-            I.flags[fCgDone] = 1;
-          } else {
-            I.red[sCgBeta] = rzn / rz;
-            I.red[sCgRz] = rzn;
-          }
-        }
-      }
+    if (blockIdx.x == 0) {
+      I.flags[fCgIters] = it + 1;
+      I.flags[fCgTotal] += 1;
     }
   }
 }
 
-// Mean of the correction when no boundary anchors the pressure level
-// (SimpleSolver.cpp:735-745).
+// Partial sums of the correction for its mean when no boundary anchors the
+// pressure level (SimpleSolver.cpp:735-745); the correction kernels finish it.
 __global__ void __launch_bounds__(kBlock) k_corr_mean(const Inst* insts) {
   CFD_PROLOGUE
   __shared__ double sm[16];
   double part[1] = {live ? I.x[c] : 0.0};
   blockSum<1>(part, sm);
   if (threadIdx.x == 0) partialSlot(I, sCorrMean)[blockIdx.x] = part[0];
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double s = waveSumPartials(partialSlot(I, sCorrMean), I.nBlocks);
-      if (threadIdx.x == 0) I.red[sCorrMean] = s / static_cast<double>(I.N);
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -840,12 +880,15 @@ __global__ void __launch_bounds__(kBlock) k_corr_mean(const Inst* insts) {
 // the imposed-velocity test of the first still sees the uncorrected fluxes.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double corrAt(const Inst& I, int q) {
-  return I.hasPRef ? I.x[q] : I.x[q] - I.red[sCorrMean];
+__device__ __forceinline__ double corrMean(const Inst& I, double* sm) {
+  return I.hasPRef ? 0.0 : blockSumPartials(partialSlot(I, sCorrMean), I.nBlocks, sm) / static_cast<double>(I.N);
 }
 
 __global__ void __launch_bounds__(kBlock) k_correct_cells(const Inst* insts) {
   CFD_PROLOGUE
+  __shared__ double sm[4];
+  const double mean = corrMean(I, sm);
+  auto corrAt = [&](const Inst& J, int q) { return J.hasPRef ? J.x[q] : J.x[q] - mean; };
   if (!live) return;
   double gx = 0.0, gy = 0.0;
   const double pc = corrAt(I, c);
@@ -893,47 +936,52 @@ __global__ void __launch_bounds__(kBlock) k_correct_cells(const Inst* insts) {
   I.v[c] -= gy * mob;
 }
 
-__device__ __forceinline__ double corrFlux(const Inst& I, const FaceGeo& f, int o, int n) {
+__device__ __forceinline__ double corrFlux(const Inst& I, const FaceGeo& f, int o, int n, double mean) {
   const double w = f.w;
   const double dens = w * I.rho[o] + (1.0 - w) * I.rho[n];
   const double mob = w * (I.V[o] / I.aP[o]) + (1.0 - w) * (I.V[n] / I.aP[n]);
-  return dens * mob * f.g * (corrAt(I, n) - corrAt(I, o));
+  const double pn = I.hasPRef ? I.x[n] : I.x[n] - mean;
+  const double po = I.hasPRef ? I.x[o] : I.x[o] - mean;
+  return dens * mob * f.g * (pn - po);
 }
 
 __device__ __forceinline__ void corrBoundaryFlux(const Inst& I, const FaceGeo& f, const FaceBC& bc, double* F,
-                                                 int o) {
+                                                 int o, double mean) {
   if (imposes(bc, *F)) return;
   const double dens = I.rho[o];
   const double mob = I.V[o] / I.aP[o];
-  *F -= dens * mob * f.g * (0.0 - corrAt(I, o));
+  const double po = I.hasPRef ? I.x[o] : I.x[o] - mean;
+  *F -= dens * mob * f.g * (0.0 - po);
 }
 
 __global__ void __launch_bounds__(kBlock) k_correct_faces(const Inst* insts) {
   CFD_PROLOGUE
+  __shared__ double sm[4];
+  const double mean = corrMean(I, sm);
   if (!live) return;
   {
     const int fi = j * nI + i;
     const FaceGeo& f = I.jg[fi];
-    if (j > 0) I.Fj[fi] -= corrFlux(I, f, c - nI, c);
+    if (j > 0) I.Fj[fi] -= corrFlux(I, f, c - nI, c, mean);
     else {
       const int pr = I.partner[i];
-      if (pr >= 0) I.Fj[fi] -= corrFlux(I, f, c, pr);
-      else corrBoundaryFlux(I, f, I.bcB[i], &I.Fj[fi], c);
+      if (pr >= 0) I.Fj[fi] -= corrFlux(I, f, c, pr, mean);
+      else corrBoundaryFlux(I, f, I.bcB[i], &I.Fj[fi], c, mean);
     }
   }
   if (j == nJ - 1) {
     const int fi = (j + 1) * nI + i;
-    corrBoundaryFlux(I, I.jg[fi], I.bcT[i], &I.Fj[fi], c);
+    corrBoundaryFlux(I, I.jg[fi], I.bcT[i], &I.Fj[fi], c, mean);
   }
   {
     const int fi = j * (nI + 1) + i;
     const FaceGeo& f = I.ig[fi];
-    if (i > 0) I.Fi[fi] -= corrFlux(I, f, c - 1, c);
-    else corrBoundaryFlux(I, f, I.bcL[j], &I.Fi[fi], c);
+    if (i > 0) I.Fi[fi] -= corrFlux(I, f, c - 1, c, mean);
+    else corrBoundaryFlux(I, f, I.bcL[j], &I.Fi[fi], c, mean);
   }
   if (i == nI - 1) {
     const int fi = j * (nI + 1) + i + 1;
-    corrBoundaryFlux(I, I.ig[fi], I.bcR[j], &I.Fi[fi], c);
+    corrBoundaryFlux(I, I.ig[fi], I.bcR[j], &I.Fi[fi], c, mean);
   }
 }
 
@@ -964,12 +1012,6 @@ __global__ void __launch_bounds__(kBlock) k_divergence(const Inst* insts, double
   }
   blockMax<1>(part, sm);
   if (threadIdx.x == 0) partialSlot(I, sMaxDiv)[blockIdx.x] = part[0];
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double s = waveMaxPartials(partialSlot(I, sMaxDiv), I.nBlocks);
-      if (threadIdx.x == 0) I.red[sMaxDiv] = s;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -977,30 +1019,71 @@ __global__ void __launch_bounds__(kBlock) k_divergence(const Inst* insts, double
 // One thread per instance.
 // ---------------------------------------------------------------------------
 
-__global__ void k_monitor(const Inst* insts, int count) {
-  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (b >= count) return;
-  const Inst& I = insts[b];
+__device__ __forceinline__ double waveMaxOf(const double* p, int n, double init) {
+  double s = init;
+  for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) s = mx(s, p[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = mx(s, __shfl_down(s, o, 64));
+  return s;
+}
+
+__device__ __forceinline__ double waveMinOf(const double* p, int n, double init) {
+  double s = init;
+  for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) s = mn(s, p[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = mn(s, __shfl_down(s, o, 64));
+  return s;
+}
+
+// One 64-lane block per instance: finishes the reductions of the iteration's
+// kernels from their per-block partials, writes the monitor and applies the
+// stop rule.
+__global__ void __launch_bounds__(64) k_monitor(const Inst* insts) {
+  const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped]) return;
+  const int n = I.nBlocks;
+  const double vscale = waveMaxOf(partialSlot(I, sVelScale), n, 0.0);
+  const double resX = waveSumPartials(partialSlot(I, sMomResX), n);
+  const double resY = waveSumPartials(partialSlot(I, sMomResY), n);
+  const double diagL1 = waveSumPartials(partialSlot(I, sMomDiagL1), n);
+  const double net = waveSumPartials(partialSlot(I, sNetFluxL1), n);
+  const double maxDiv = waveMaxOf(partialSlot(I, sMaxDiv), n, 0.0);
+  double tk = 0.0, tkd = 0.0, tw = 0.0, twd = 0.0, mxk = 0.0, mxw = 0.0, mxm = 0.0, mxr = 0.0, mnk = 0.0, mnw = 0.0,
+         mnm = 0.0;
+  if (I.model == 2) {
+    tk = waveSumPartials(partialSlot(I, sTurbResK), n);
+    tkd = waveSumPartials(partialSlot(I, sTurbDiagK), n);
+    tw = waveSumPartials(partialSlot(I, sTurbResW), n);
+    twd = waveSumPartials(partialSlot(I, sTurbDiagW), n);
+    mxk = waveMaxOf(partialSlot(I, sMaxK), n, -INFINITY);
+    mxw = waveMaxOf(partialSlot(I, sMaxW), n, -INFINITY);
+    mxm = waveMaxOf(partialSlot(I, sMaxMut), n, -INFINITY);
+    mxr = waveMaxOf(partialSlot(I, sMaxEddyRatio), n, 0.0);
+    mnk = waveMinOf(partialSlot(I, sMinK), n, INFINITY);
+    mnw = waveMinOf(partialSlot(I, sMinW), n, INFINITY);
+    mnm = waveMinOf(partialSlot(I, sMinMut), n, INFINITY);
+  }
+  if (threadIdx.x != 0) return;
   constexpr double kTiny = 1e-300;
+  const double vs = (vscale > 0.0) ? vscale : 1.0;  // SimpleSolver.cpp:681-683
   DevMonitor m{};
-  m.continuity = I.red[sNetFluxL1] / I.refMassFlow;
-  m.massImbalance = I.red[sNetFluxL1];
-  const double scale = I.red[sMomDiagL1] * I.red[sVelScale];
-  m.momentumX = I.red[sMomResX] / mx(scale, kTiny);
-  m.momentumY = I.red[sMomResY] / mx(scale, kTiny);
-  m.maxDivergence = I.red[sMaxDiv];
+  m.continuity = net / I.refMassFlow;
+  m.massImbalance = net;
+  const double scale = diagL1 * vs;
+  m.momentumX = resX / mx(scale, kTiny);
+  m.momentumY = resY / mx(scale, kTiny);
+  m.maxDivergence = maxDiv;
   m.pressureIterations = I.flags[fCgTotal];
   if (I.model == 2) {
-    m.turbFirst = I.red[sTurbResK] / mx(I.red[sTurbDiagK] * mx(I.kIn, 1e-30), 1e-30);
-    m.turbSecond = I.red[sTurbResW] / mx(I.red[sTurbDiagW] * mx(I.wIn, 1e-30), 1e-30);
-    m.minK = I.red[sMinK];
-    m.maxK = I.red[sMaxK];
-    m.minW = I.red[sMinW];
-    m.maxW = I.red[sMaxW];
-    m.minMut = I.red[sMinMut];
-    m.maxMut = I.red[sMaxMut];
-    m.maxEddyRatio = I.red[sMaxEddyRatio];
+    m.turbFirst = tk / mx(tkd * mx(I.kIn, 1e-30), 1e-30);
+    m.turbSecond = tw / mx(twd * mx(I.wIn, 1e-30), 1e-30);
+    m.minK = mnk;
+    m.maxK = mxk;
+    m.minW = mnw;
+    m.maxW = mxw;
+    m.minMut = mnm;
+    m.maxMut = mxm;
+    m.maxEddyRatio = mxr;
   }
   *I.monitor = m;
   const int iter = I.flags[fIteration] + 1;

@@ -276,20 +276,6 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
     partialSlot(I, sTurbResW)[blockIdx.x] = part[2];
     partialSlot(I, sTurbDiagW)[blockIdx.x] = part[3];
   }
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const double a = waveSumPartials(partialSlot(I, sTurbResK), I.nBlocks);
-      const double b = waveSumPartials(partialSlot(I, sTurbDiagK), I.nBlocks);
-      const double e = waveSumPartials(partialSlot(I, sTurbResW), I.nBlocks);
-      const double f = waveSumPartials(partialSlot(I, sTurbDiagW), I.nBlocks);
-      if (threadIdx.x == 0) {
-        I.red[sTurbResK] = a;
-        I.red[sTurbDiagK] = b;
-        I.red[sTurbResW] = e;
-        I.red[sTurbDiagW] = f;
-      }
-    }
-  }
 }
 
 // KOmegaSST.cpp:86-92
@@ -365,37 +351,6 @@ __global__ void __launch_bounds__(kBlock) k_sst_mut(const Inst* insts) {
     partialSlot(I, sMinK)[blockIdx.x] = -neg[0];
     partialSlot(I, sMinW)[blockIdx.x] = -neg[1];
     partialSlot(I, sMinMut)[blockIdx.x] = -neg[2];
-  }
-  if (arriveLast(I.counter, static_cast<unsigned>(I.nBlocks))) {
-    if (threadIdx.x < 64) {
-      const int n = I.nBlocks;
-      double vmaxK = -INFINITY, vmaxW = -INFINITY, vmaxM = -INFINITY, vminK = INFINITY, vminW = INFINITY,
-             vminM = INFINITY, vr = 0.0;
-      for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) {
-        vmaxK = mx(vmaxK, partialSlot(I, sMaxK)[b]);
-        vmaxW = mx(vmaxW, partialSlot(I, sMaxW)[b]);
-        vmaxM = mx(vmaxM, partialSlot(I, sMaxMut)[b]);
-        vr = mx(vr, partialSlot(I, sMaxEddyRatio)[b]);
-        vminK = mn(vminK, partialSlot(I, sMinK)[b]);
-        vminW = mn(vminW, partialSlot(I, sMinW)[b]);
-        vminM = mn(vminM, partialSlot(I, sMinMut)[b]);
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        vmaxK = mx(vmaxK, __shfl_down(vmaxK, o, 64));
-        vmaxW = mx(vmaxW, __shfl_down(vmaxW, o, 64));
-        vmaxM = mx(vmaxM, __shfl_down(vmaxM, o, 64));
-        vr = mx(vr, __shfl_down(vr, o, 64));
-        vminK = mn(vminK, __shfl_down(vminK, o, 64));
-        vminW = mn(vminW, __shfl_down(vminW, o, 64));
-        vminM = mn(vminM, __shfl_down(vminM, o, 64));
-      }
-      if (threadIdx.x == 0) {
-        I.red[sMaxK] = vmaxK; I.red[sMaxW] = vmaxW; I.red[sMaxMut] = vmaxM;
-        I.red[sMinK] = vminK; I.red[sMinW] = vminW; I.red[sMinMut] = vminM;
-        I.red[sMaxEddyRatio] = vr;
-      }
-    }
   }
 }
 

@@ -186,7 +186,7 @@ void Engine::allocate() {
     I.igSst = cv.take<double>(nIf);
     double** cellArrays[] = {&I.u, &I.v, &I.p, &I.rho, &I.mu, &I.muMol, &I.aP, &I.aS, &I.aN, &I.aW, &I.aE,
                              &I.bx, &I.by, &I.pP, &I.pN, &I.pE, &I.pWk, &I.pb, &I.x, &I.r, &I.z, &I.pd0,
-                             &I.pd1, &I.Ap, &I.k, &I.w, &I.mut, &I.f1, &I.f2, &I.cd, &I.strain, &I.prod,
+                             &I.pd1, &I.Ap, &I.gsTmp0, &I.gsTmp1, &I.k, &I.w, &I.mut, &I.f1, &I.f2, &I.cd, &I.strain, &I.prod,
                              &I.kP, &I.kS, &I.kN, &I.kW, &I.kE, &I.kB, &I.wP, &I.wS, &I.wN, &I.wW, &I.wE, &I.wB};
     for (double** a : cellArrays) *a = cv.take<double>(N);
     double2** vecArrays[] = {&I.gU, &I.gV, &I.gP, &I.gK, &I.gW};

@@ -1,0 +1,146 @@
+// hip_runtime.h (CPU emulation) - runs the solver's HIP kernels on the host
+// for AddressSanitizer / bounds checking of their indexing. TEST
+// INFRASTRUCTURE ONLY: the emulated library (tests/emu/libcfdgpu_emu.so) is
+// built and loaded by tests/ alone; the product loads libcfdgpu.so and has no
+// CPU path.
+//
+// Model: blocks run one after another; the lanes of a block are worker
+// threads; __syncthreads / s_barrier are block-wide std::barrier waits and
+// __shfl_down exchanges through a per-wave buffer between two wave barriers.
+// Lanes that return early drop out of both barriers, as terminated waves do
+// on the hardware. Static __shared__ arrays become function statics (one
+// block at a time, so one copy suffices). Memory calls are malloc / memcpy.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+using std::isfinite;
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_uint3 {
+  unsigned x{0}, y{0}, z{0};
+};
+struct double2 {
+  double x, y;
+};
+inline double2 make_double2(double a, double b) { return double2{a, b}; }
+
+namespace cfdemu {
+struct BlockCtx {
+  unsigned threads{0};
+  std::unique_ptr<std::barrier<>> block;
+  std::vector<std::unique_ptr<std::barrier<>>> waves;
+  std::vector<double> xchg;
+  std::vector<char> lds;
+};
+inline thread_local emu_uint3 tIdx;
+inline thread_local emu_uint3 bIdx;
+inline thread_local dim3 bDim;
+inline thread_local BlockCtx* ctx = nullptr;
+inline void syncBlock() { ctx->block->arrive_and_wait(); }
+inline double shflDown(double v, int offset, int width) {
+  const unsigned lane = tIdx.x % static_cast<unsigned>(width);
+  const unsigned wave = tIdx.x / 64u;
+  auto& wb = *ctx->waves[wave];
+  ctx->xchg[tIdx.x] = v;
+  wb.arrive_and_wait();
+  const unsigned src = lane + static_cast<unsigned>(offset);
+  const unsigned waveLanes = std::min(64u, ctx->threads - wave * 64u);
+  const double r = (src < static_cast<unsigned>(width) && src < waveLanes) ? ctx->xchg[tIdx.x + offset] : v;
+  wb.arrive_and_wait();
+  return r;
+}
+template <class F>
+void runBlock(dim3 b, dim3 t, size_t shmem, F&& body) {
+  BlockCtx c;
+  c.threads = t.x * t.y * t.z;
+  c.block = std::make_unique<std::barrier<>>(static_cast<std::ptrdiff_t>(c.threads));
+  for (unsigned w = 0; w * 64 < c.threads; ++w)
+    c.waves.push_back(std::make_unique<std::barrier<>>(static_cast<std::ptrdiff_t>(std::min(64u, c.threads - w * 64))));
+  c.xchg.assign(c.threads, 0.0);
+  c.lds.assign(shmem + 64, 0);
+  std::vector<std::thread> lanes;
+  lanes.reserve(c.threads);
+  for (unsigned q = 0; q < c.threads; ++q) {
+    lanes.emplace_back([&, q] {
+      ctx = &c;
+      tIdx.x = q;
+      bIdx.x = b.x;
+      bIdx.y = b.y;
+      bDim = t;
+      body();
+      c.waves[q / 64]->arrive_and_drop();
+      c.block->arrive_and_drop();
+    });
+  }
+  for (auto& l : lanes) l.join();
+}
+}  // namespace cfdemu
+
+#define threadIdx (cfdemu::tIdx)
+#define blockIdx (cfdemu::bIdx)
+#define blockDim (cfdemu::bDim)
+#define __syncthreads() cfdemu::syncBlock()
+#define __builtin_amdgcn_s_barrier() cfdemu::syncBlock()
+#define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
+#define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(cfdemu::ctx->lds.data());
+inline double __shfl_down(double v, int offset, int width = 64) { return cfdemu::shflDown(v, offset, width); }
+
+// runtime API
+using hipError_t = int;
+struct emu_stream;
+using hipStream_t = emu_stream*;
+constexpr hipError_t hipSuccess = 0;
+enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
+constexpr unsigned hipStreamNonBlocking = 1;
+inline const char* hipGetErrorString(hipError_t) { return "emulation error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+template <class T>
+inline hipError_t hipMalloc(T** p, size_t bytes) {
+  *p = static_cast<T*>(std::malloc(bytes));
+  return *p ? hipSuccess : 2;
+}
+template <class T>
+inline hipError_t hipMallocAsync(T** p, size_t bytes, hipStream_t) { return hipMalloc(p, bytes); }
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  std::memmove(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+  static char token;
+  *s = reinterpret_cast<hipStream_t>(&token);
+  return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+
+template <class... P, class... A>
+void hipLaunchKernelGGL(void (*kernel)(P...), dim3 grid, dim3 block, size_t shmem, hipStream_t, A... args) {
+  for (unsigned by = 0; by < grid.y; ++by)
+    for (unsigned bx = 0; bx < grid.x; ++bx)
+      cfdemu::runBlock(dim3(bx, by), block, shmem, [&] { kernel(static_cast<P>(args)...); });
+}
+
