@@ -38,6 +38,20 @@ int guarded(F&& f) {
   }
 }
 
+// Device-path options the library runs. The mixed-precision path is switched
+// off: it made an illegal memory access on the GPU whose cause has not been
+// found, so it stays unreachable until one is.
+int checkPathOptions(const cfdgpu_settings& s) {
+  if (s.precision == CFDGPU_PRECISION_MIXED)
+    return fail(CFDGPU_NOT_IMPLEMENTED, "mixed precision is disabled in this build; use CFDGPU_PRECISION_FP64");
+  if (s.precision != CFDGPU_PRECISION_FP64) return fail(CFDGPU_INVALID_ARGUMENT, "unknown precision");
+  if (s.gs_ordering != CFDGPU_GS_LEXICOGRAPHIC && s.gs_ordering != CFDGPU_GS_REDBLACK)
+    return fail(CFDGPU_INVALID_ARGUMENT, "unknown Gauss-Seidel ordering");
+  if (s.pressure_solver != CFDGPU_PSOLVE_JACOBI_CG)
+    return fail(CFDGPU_NOT_IMPLEMENTED, "only the Jacobi-PCG pressure solver runs in FP64");
+  return CFDGPU_OK;
+}
+
 cfdgpu::CGridOptions toOptions(const cfdgpu_cgrid_options& o) {
   cfdgpu::CGridOptions r;
   r.upstream = o.upstream_chords;
@@ -320,6 +334,7 @@ int cfdgpu_create(cfdgpu_solver** out, const cfdgpu_mesh* mesh, const int* kind,
   if (!(settings->velocity_relaxation > 0.0) || settings->velocity_relaxation > 1.0 ||
       !(settings->pressure_relaxation > 0.0) || settings->pressure_relaxation > 1.0)
     return fail(CFDGPU_INVALID_ARGUMENT, "relaxation factors must lie in (0, 1]");
+  if (const int rc = checkPathOptions(*settings); rc != CFDGPU_OK) return rc;
   return guarded([&]() -> int {
     int devices = 0;
     if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
@@ -379,6 +394,7 @@ int cfdgpu_get_settings(const cfdgpu_solver* s, cfdgpu_settings* out) {
 int cfdgpu_set_settings(cfdgpu_solver* s, const cfdgpu_settings* settings) {
   SOLVER_GUARD(s)
   if (!settings) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  if (const int rc = checkPathOptions(*settings); rc != CFDGPU_OK) return rc;
   return guarded([&]() -> int {
     s->engine->setSettings(*settings);
     return CFDGPU_OK;

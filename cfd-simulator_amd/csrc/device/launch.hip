@@ -28,6 +28,14 @@ __global__ void k_sst_blend(const Inst*);
 __global__ void k_sst_asm(const Inst*);
 __global__ void k_sst_wall(const Inst*);
 __global__ void k_sst_mut(const Inst*);
+__global__ void k_fsweep(const Inst*, int, int, int);
+__global__ void k_cgf_rhs(const Inst*, int);
+__global__ void k_line_factor(const Inst*);
+__global__ void k_line_solve(const Inst*, int);
+__global__ void k_cgf_accumulate(const Inst*);
+template <int PRE> __global__ void k_cgf_init(const Inst*);
+template <int PRE> __global__ void k_cgf_a(const Inst*, int);
+template <int PRE> __global__ void k_cgf_b(const Inst*, int);
 
 int gsThreadsFor(int nJ) {
   int t = ((nJ + 63) / 64) * 64;
@@ -42,15 +50,47 @@ size_t gsLdsFor(int nJ, int nI, int threads) {
   return (static_cast<size_t>(2) * 2 * static_cast<size_t>(threads) + 2 * static_cast<size_t>(nI)) * sizeof(double);
 }
 
+namespace {
+
+// FP32 pressure correction with FP64 reductions, refined against the FP64
+// matrix (one pass = one FP32 CG solve of the FP64 residual).
+template <int PRE>
+void launchMixedPressure(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
+  const dim3 block(kBlock);
+  const dim3 lines(static_cast<unsigned>(cfg.maxLineBlocks), static_cast<unsigned>(cfg.batch));
+  if (PRE == 1) hipLaunchKernelGGL(k_line_factor, lines, dim3(64), 0, s, d);
+  for (int pass = 0; pass < h.refinePasses; ++pass) {
+    hipLaunchKernelGGL(k_cgf_rhs, grid, block, 0, s, d, pass);
+    hipLaunchKernelGGL(k_cgf_init<PRE>, grid, block, 0, s, d);
+    if (PRE == 1) hipLaunchKernelGGL(k_line_solve, lines, dim3(64), 0, s, d, -1);
+    for (int it = 0; it < h.cgMax; ++it) {
+      hipLaunchKernelGGL(k_cgf_a<PRE>, grid, block, 0, s, d, it);
+      hipLaunchKernelGGL(k_cgf_b<PRE>, grid, block, 0, s, d, it);
+      if (PRE == 1) hipLaunchKernelGGL(k_line_solve, lines, dim3(64), 0, s, d, it);
+    }
+    hipLaunchKernelGGL(k_cgf_accumulate, grid, block, 0, s, d);
+  }
+}
+
+}  // namespace
+
 void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
   const dim3 block(kBlock);
   const dim3 one((cfg.batch + 63) / 64);
   const bool lex = h.gsOrder == 0;
+  const bool mixed = h.precision == 1;
   hipLaunchKernelGGL(k_begin_iteration, one, dim3(64), 0, s, d, cfg.batch);
   hipLaunchKernelGGL(k_grad, grid, block, 0, s, d);
   hipLaunchKernelGGL(k_mom, grid, block, 0, s, d);
-  if (lex) {
+  if (mixed) {
+    // two-colour sweeps on the FP32 defect; the last adds it to u, v
+    for (int sw = 0; sw < h.momSweeps; ++sw) {
+      hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 0, 0);
+      hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 1, sw == h.momSweeps - 1 ? 1 : 0);
+    }
+  } else if (lex) {
     if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
     else hipLaunchKernelGGL(k_gs_lex_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
   } else {
@@ -64,10 +104,15 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   const int passes = h.nonOrthPasses < 1 ? 1 : h.nonOrthPasses;
   for (int pass = 0; pass < passes; ++pass) {
     hipLaunchKernelGGL(k_pressure, grid, block, 0, s, d, pass);
-    hipLaunchKernelGGL(k_cg_init, grid, block, 0, s, d);
-    for (int it = 0; it < h.cgMax; ++it) {
-      hipLaunchKernelGGL(k_cg_a, grid, block, 0, s, d, it);
-      hipLaunchKernelGGL(k_cg_b, grid, block, 0, s, d, it);
+    if (mixed) {
+      if (h.psolve == 1) launchMixedPressure<1>(d, cfg, h, s);
+      else launchMixedPressure<0>(d, cfg, h, s);
+    } else {
+      hipLaunchKernelGGL(k_cg_init, grid, block, 0, s, d);
+      for (int it = 0; it < h.cgMax; ++it) {
+        hipLaunchKernelGGL(k_cg_a, grid, block, 0, s, d, it);
+        hipLaunchKernelGGL(k_cg_b, grid, block, 0, s, d, it);
+      }
     }
     if (!h.hasPRef) hipLaunchKernelGGL(k_corr_mean, grid, block, 0, s, d);
     hipLaunchKernelGGL(k_correct_cells, grid, block, 0, s, d);
@@ -77,7 +122,15 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   if (h.model == 2) {
     hipLaunchKernelGGL(k_sst_blend, grid, block, 0, s, d);
     hipLaunchKernelGGL(k_sst_asm, grid, block, 0, s, d);
-    if (lex) {
+    if (mixed) {
+      for (int sw = 0; sw < 3; ++sw) {
+        const int last = (sw == 2) ? 1 : 0;
+        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 1, 0, 0);
+        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 1, 1, last);
+        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 0, 0);
+        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 1, last);
+      }
+    } else if (lex) {
       if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
       else hipLaunchKernelGGL(k_gs_lex_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
     } else {

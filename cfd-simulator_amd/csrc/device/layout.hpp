@@ -66,6 +66,7 @@ enum Slot {
   sMaxYPlus,
   sRefineNorm,     // mixed path: FP64 residual norm of the pressure correction
   sCgRz0, sCgRz1,  // CG: r.z of even / odd iterations (persisted by block 0)
+  sCgRzLine,       // CG: r.z partials of the line-preconditioner launch
   kSlots
 };
 
@@ -152,6 +153,10 @@ struct Inst {
   // mixed-precision mirrors (FP32 bulk)
   float *faP, *faS, *faN, *faW, *faE, *fbx, *fby, *fdx, *fdy;
   float *fpP, *fpN, *fpE, *fpWk, *fx, *fr, *fz, *fpd0, *fpd1, *fAp, *fb;
+  float *fkP, *fkS, *fkN, *fkW, *fkE, *fkR, *fkD, *fwP, *fwS, *fwN, *fwW, *fwE, *fwR, *fwD;
+  float *fLinv, *fLc;  // line preconditioner: 1/pivot and eliminated upper coefficient
+  int lineBlocks;      // 64-column blocks of the line preconditioner
+  int refinePasses;    // FP32 CG passes refined against the FP64 matrix
 
   // reductions
   double* red;        // kSlots scalars

@@ -271,6 +271,18 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     part[0] = fabs(row.bx - axu);
     part[1] = fabs(row.by - axv);
     part[2] = fabs(d);
+    if (I.precision == 1) {
+      // FP32 matrix and FP64 defect for the defect-correction sweeps
+      I.faP[c] = static_cast<float>(d);
+      I.faS[c] = static_cast<float>(aS);
+      I.faN[c] = static_cast<float>(aN);
+      I.faW[c] = static_cast<float>(aW);
+      I.faE[c] = static_cast<float>(aE);
+      I.fbx[c] = static_cast<float>(row.bx - axu);
+      I.fby[c] = static_cast<float>(row.by - axv);
+      I.fdx[c] = 0.0f;
+      I.fdy[c] = 0.0f;
+    }
   }
   blockSum<3>(part, sm);
   if (threadIdx.x == 0) {
@@ -702,6 +714,12 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
     I.pE[c] = cE;
     I.pWk[c] = cWk;
     I.pb[c] = -net;
+    if (I.precision == 1) {
+      I.fpP[c] = static_cast<float>(diag);
+      I.fpN[c] = static_cast<float>(cN);
+      I.fpE[c] = static_cast<float>(cE);
+      I.fpWk[c] = static_cast<float>(cWk);
+    }
     part[0] = fabs(net);
     part[1] = -net;
   }

@@ -206,8 +206,9 @@ __device__ __forceinline__ SstRow sstRow(const Inst& I, int c, int i, int j, dou
   return r;
 }
 
-__device__ __forceinline__ double sstResidualTerm(const Inst& I, const SstRow& r, const double* phi, int c, int i,
-                                                  int j) {
+// b - A phi of one row (the signed defect; its magnitude is the residual term)
+__device__ __forceinline__ double sstDefect(const Inst& I, const SstRow& r, const double* phi, int c, int i,
+                                            int j) {
   const int nI = I.nI, nJ = I.nJ;
   double ax = r.d * phi[c];
   if (j > 0) ax += r.aS * phi[c - nI];
@@ -218,7 +219,7 @@ __device__ __forceinline__ double sstResidualTerm(const Inst& I, const SstRow& r
   if (j < nJ - 1) ax += r.aN * phi[c + nI];
   if (i > 0) ax += r.aW * phi[c - 1];
   if (i < nI - 1) ax += r.aE * phi[c + 1];
-  return fabs(r.b - ax);
+  return r.b - ax;
 }
 
 __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
@@ -245,8 +246,14 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
         r.d = relaxed;
       }
       I.kP[c] = r.d; I.kS[c] = r.aS; I.kN[c] = r.aN; I.kW[c] = r.aW; I.kE[c] = r.aE; I.kB[c] = r.b;
-      part[0] = sstResidualTerm(I, r, I.k, c, i, j);
+      const double defect = sstDefect(I, r, I.k, c, i, j);
+      part[0] = fabs(defect);
       part[1] = fabs(r.d);
+      if (I.precision == 1) {
+        I.fkP[c] = static_cast<float>(r.d); I.fkS[c] = static_cast<float>(r.aS); I.fkN[c] = static_cast<float>(r.aN);
+        I.fkW[c] = static_cast<float>(r.aW); I.fkE[c] = static_cast<float>(r.aE);
+        I.fkR[c] = static_cast<float>(defect); I.fkD[c] = 0.0f;
+      }
     }
     // omega equation (wall value is omega_inflow: KOmegaSST.cpp:507-509)
     {
@@ -265,8 +272,14 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
         r.d = relaxed;
       }
       I.wP[c] = r.d; I.wS[c] = r.aS; I.wN[c] = r.aN; I.wW[c] = r.aW; I.wE[c] = r.aE; I.wB[c] = r.b;
-      part[2] = sstResidualTerm(I, r, I.w, c, i, j);
+      const double defect = sstDefect(I, r, I.w, c, i, j);
+      part[2] = fabs(defect);
       part[3] = fabs(r.d);
+      if (I.precision == 1) {
+        I.fwP[c] = static_cast<float>(r.d); I.fwS[c] = static_cast<float>(r.aS); I.fwN[c] = static_cast<float>(r.aN);
+        I.fwW[c] = static_cast<float>(r.aW); I.fwE[c] = static_cast<float>(r.aE);
+        I.fwR[c] = static_cast<float>(defect); I.fwD[c] = 0.0f;
+      }
     }
   }
   blockSum<4>(part, sm);

@@ -194,7 +194,9 @@ void Engine::allocate() {
     I.Fj = cv.take<double>(nJf);
     I.Fi = cv.take<double>(nIf);
     float** fArrays[] = {&I.faP, &I.faS, &I.faN, &I.faW, &I.faE, &I.fbx, &I.fby, &I.fdx, &I.fdy, &I.fpP, &I.fpN,
-                         &I.fpE, &I.fpWk, &I.fx, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.fb};
+                         &I.fpE, &I.fpWk, &I.fx, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.fb,
+                         &I.fkP, &I.fkS, &I.fkN, &I.fkW, &I.fkE, &I.fkR, &I.fkD, &I.fwP, &I.fwS, &I.fwN,
+                         &I.fwW, &I.fwE, &I.fwR, &I.fwD, &I.fLinv, &I.fLc};
     for (float** a : fArrays) *a = cv.take<float>(N);
     I.red = cv.take<double>(kSlots);
     I.partials = cv.take<double>(static_cast<size_t>(kSlots) * kMaxRedBlocks);
@@ -217,8 +219,10 @@ void Engine::allocate() {
   h_.nJ = grid_.nJ;
   h_.N = grid_.cells();
   h_.nBlocks = (h_.N + kBlock - 1) / kBlock;
+  h_.lineBlocks = (h_.nI + 63) / 64;
   check(hipMalloc(&dInst_, sizeof(Inst)), "hipMalloc (descriptor)");
   cfg_.maxBlocks = h_.nBlocks;
+  cfg_.maxLineBlocks = h_.lineBlocks;
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
   cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
@@ -257,6 +261,7 @@ void Engine::fillInstScalars() {
   h_.gsOrder = settings_.gs_ordering;
   h_.precision = settings_.precision;
   h_.psolve = settings_.pressure_solver;
+  h_.refinePasses = std::max(1, settings_.refinement_passes);
   h_.refMassFlow = refMassFlow_;
   h_.kIn = kIn_;
   h_.wIn = wIn_;

@@ -23,6 +23,7 @@ namespace cfdgpu {
 // Kernel launchers (defined in launch.hip).
 struct LaunchCfg {
   int maxBlocks;   // grid.x for cell-parallel kernels
+  int maxLineBlocks;  // grid.x for the column (line-preconditioner) kernels
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep

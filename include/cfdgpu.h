@@ -43,7 +43,8 @@ enum { CFDGPU_MODEL_NONE = 0, CFDGPU_MODEL_LAMINAR = 1, CFDGPU_MODEL_SST = 2 };
 /* arithmetic of the hot path */
 enum {
   CFDGPU_PRECISION_FP64 = 0, /* plain FP64 path: the drop-in and the judge */
-  CFDGPU_PRECISION_MIXED = 1 /* FP32 sweeps/CG bulk, FP64 state, residuals and reductions */
+  CFDGPU_PRECISION_MIXED = 1 /* FP32 sweeps/CG bulk, FP64 state and reductions; currently
+                                rejected with CFDGPU_NOT_IMPLEMENTED (disabled) */
 };
 /* Gauss-Seidel ordering of the momentum and turbulence sweeps */
 enum {
@@ -53,7 +54,8 @@ enum {
 /* pressure-correction solver */
 enum {
   CFDGPU_PSOLVE_JACOBI_CG = 0, /* reference: Jacobi-PCG, L1 stop (LinearSystem.cpp:118-210) */
-  CFDGPU_PSOLVE_LINE_CG = 1    /* CG preconditioned by wall-normal line (tridiagonal) solves */
+  CFDGPU_PSOLVE_LINE_CG = 1    /* CG preconditioned by wall-normal line (tridiagonal) solves;
+                                  mixed path only, so currently rejected (NOT_IMPLEMENTED) */
 };
 
 typedef struct cfdgpu_mesh cfdgpu_mesh;

@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <atomic>
 #include <barrier>
+#include <condition_variable>
+#include <mutex>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -69,6 +71,59 @@ inline double shflDown(double v, int offset, int width) {
   wb.arrive_and_wait();
   return r;
 }
+// A persistent pool of lane threads (created once, up to 1024), so long runs
+// do not create threads per launch.
+class LanePool {
+ public:
+  static LanePool& get() {
+    // never destroyed: the detached lanes wait on its condition variables
+    // until the process exits
+    static LanePool* pool = new LanePool;
+    return *pool;
+  }
+  void run(unsigned lanes, const std::function<void(unsigned)>& body) {
+    std::unique_lock<std::mutex> lock(m_);
+    ensure(lanes);
+    body_ = &body;
+    lanes_ = lanes;
+    pending_ = lanes;
+    ++generation_;
+    go_.notify_all();
+    done_.wait(lock, [this] { return pending_ == 0; });
+  }
+
+ private:
+  void ensure(unsigned lanes) {
+    while (threads_.size() < lanes) {
+      const unsigned id = static_cast<unsigned>(threads_.size());
+      threads_.emplace_back([this, id] { loop(id); });
+      threads_.back().detach();
+    }
+  }
+  void loop(unsigned id) {
+    unsigned long seen = 0;
+    for (;;) {
+      const std::function<void(unsigned)>* body;
+      {
+        std::unique_lock<std::mutex> lock(m_);
+        go_.wait(lock, [&] { return generation_ != seen; });
+        seen = generation_;
+        if (id >= lanes_) continue;
+        body = body_;
+      }
+      (*body)(id);
+      std::lock_guard<std::mutex> lock(m_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::mutex m_;
+  std::condition_variable go_, done_;
+  std::vector<std::thread> threads_;
+  const std::function<void(unsigned)>* body_{nullptr};
+  unsigned lanes_{0}, pending_{0};
+  unsigned long generation_{0};
+};
+
 template <class F>
 void runBlock(dim3 b, dim3 t, size_t shmem, F&& body) {
   BlockCtx c;
@@ -77,22 +132,18 @@ void runBlock(dim3 b, dim3 t, size_t shmem, F&& body) {
   for (unsigned w = 0; w * 64 < c.threads; ++w)
     c.waves.push_back(std::make_unique<std::barrier<>>(static_cast<std::ptrdiff_t>(std::min(64u, c.threads - w * 64))));
   c.xchg.assign(c.threads, 0.0);
-  c.lds.assign(shmem + 64, 0);
-  std::vector<std::thread> lanes;
-  lanes.reserve(c.threads);
-  for (unsigned q = 0; q < c.threads; ++q) {
-    lanes.emplace_back([&, q] {
-      ctx = &c;
-      tIdx.x = q;
-      bIdx.x = b.x;
-      bIdx.y = b.y;
-      bDim = t;
-      body();
-      c.waves[q / 64]->arrive_and_drop();
-      c.block->arrive_and_drop();
-    });
-  }
-  for (auto& l : lanes) l.join();
+  c.lds.assign(shmem, 0);
+  const std::function<void(unsigned)> lane = [&](unsigned q) {
+    ctx = &c;
+    tIdx.x = q;
+    bIdx.x = b.x;
+    bIdx.y = b.y;
+    bDim = t;
+    body();
+    c.waves[q / 64]->arrive_and_drop();
+    c.block->arrive_and_drop();
+  };
+  LanePool::get().run(c.threads, lane);
 }
 }  // namespace cfdemu
 
