@@ -1,0 +1,236 @@
+"""Benchmark of the MI355X SIMPLE outer iteration.
+
+Workload: a single RANS solve of NACA 2412 on a 1024 x 256 C-grid
+(surfacePoints 385, wakePoints 128, normalPoints 257, first layer 1e-6 chord,
+domain 15 chords up/vertical and 20 downstream), k-omega SST, Re = 1e6,
+alpha = 4 deg, Tu = 0.1 %, mu_t/mu = 10, all other SimpleSettings at the
+reference defaults (include/cfd/solver/SimpleSolver.hpp:96-131). The plain
+FP64 path with the reference's lexicographic Gauss-Seidel order runs it; that
+path reproduces the reference iteration for iteration.
+
+A step is one outer iteration (SimpleSolver::iterate, SimpleSolver.cpp:662-794)
+with the state resident in HBM. The headline value is converged solves/s:
+steps/s divided by the number of outer iterations this solve takes until Cl
+and Cd stay within 1e-3 and 1e-4 of their converged values. That count comes
+from `python bench.py --converge`, which runs the solve to its residual
+tolerance and prints the history it was read from.
+
+Beside the timed loop the line carries
+  roofline      the dominant stage's kernel: algorithmic HBM bytes per launch
+                (traffic_model()) over its mean launch time from HIP events
+                recorded on the solver's stream inside the timed region;
+  cpu_baseline  the FP64 oracle (a restatement of the reference solver, one
+                thread) timed on a few outer iterations of the same solve.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(_HERE, "cfd-simulator_amd"))
+
+METRIC = "converged solves/s batched-mixed vs plain FP64 at |ΔCl|<1e-3; p50/p99 latency"
+UNIT = "converged solves/s"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+# Outer iterations of the C2 solve until |Cl - Cl_conv| < 1e-3 and
+# |Cd - Cd_conv| < 1e-4 hold for good (python bench.py --converge).
+ITERATIONS_TO_CONVERGE = None
+
+WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
+                first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
+                reynolds=1e6, alpha=4.0, speed=50.0, density=1.225, intensity=0.001, viscosity_ratio=10.0)
+
+
+def traffic_model():
+    """Compulsory HBM words per cell for each unit of work (each array read or
+    written once per kernel, perfect neighbour reuse, geometry per cell)."""
+    return {
+        "cg_iteration": 16,
+        "gradients": 15,
+        "momentum_assembly": 36,
+        "momentum_sweeps": 33,      # 3 sweeps, u and v: 5 coefficients + 2 sources + 2 reads + 2 writes
+        "rhie_chow_pressure": 34,
+        "correction": 23,
+        "sst_update": 130,
+    }
+
+
+def stage_bytes(stage: str, cells: int, word: int = 8) -> float | None:
+    """Algorithmic bytes of one call of a stage."""
+    tm = traffic_model()
+    key = {"momentum_sweeps": "momentum_sweeps", "gradients": "gradients",
+           "momentum_assembly": "momentum_assembly", "correction": "correction"}.get(stage)
+    if key is None:
+        return None
+    return float(tm[key]) * cells * word
+
+
+def build_case(G):
+    w = WORKLOAD
+    m, p, t = w["section"]
+    opts = G.cgrid_options("coarse", upstream_chords=w["upstream"], vertical_chords=w["vertical"],
+                           downstream_chords=w["downstream"], surface_points=w["surface_points"],
+                           wake_points=w["wake_points"], normal_points=w["normal_points"],
+                           first_layer_height=w["first_layer_height"])
+    mesh = G.cgrid(m, p, t, options=opts)
+    kind, vel, pres = G.face_conditions(mesh, w["speed"], w["alpha"])
+    field = G.uniform_field(mesh.cells, speed=w["speed"], alpha_deg=w["alpha"], density=w["density"],
+                            reynolds=w["reynolds"])
+    return mesh, kind, vel, pres, field
+
+
+def gpu_solver(G, mesh, kind, vel, pres, field):
+    w = WORKLOAD
+    st = G.default_settings(intensity=w["intensity"], viscosity_ratio=w["viscosity_ratio"])
+    s = G.SimpleSolver(mesh, kind, vel, pres, st)
+    s.set_turbulence_model(G.MODEL_SST)
+    s.initialise(*field)
+    return s
+
+
+def coefficients(G, mesh, solver):
+    w = WORKLOAD
+    u, v, p, mu = solver.field()
+    f, _ = G.surface(mesh, u, v, p, mu, speed=w["speed"], alpha_deg=w["alpha"], density=w["density"])
+    return f
+
+
+def cpu_baseline(iterations: int, iters_per_solve):
+    """The oracle on the same solve, one thread, a bounded number of outer iterations."""
+    sys.path.insert(0, os.path.join(_HERE, "oracle"))
+    import oracle_py as O  # the checker: timed here as the CPU baseline only
+
+    w = WORKLOAD
+    m, p, t = w["section"]
+    opts = O.cgrid_options("coarse", upstream=w["upstream"], vertical=w["vertical"], downstream=w["downstream"],
+                           surface_points=w["surface_points"], wake_points=w["wake_points"],
+                           normal_points=w["normal_points"], first_layer=w["first_layer_height"])
+    mesh = O.cgrid(m, p, t, options=opts)
+    kind, vel, pres = O.face_conditions(mesh, w["speed"], w["alpha"])
+    st = O.default_settings()
+    st.intensity = w["intensity"]
+    st.viscosity_ratio = w["viscosity_ratio"]
+    s = O.Solver(mesh, kind, vel, pres, st)
+    s.set_model(2)
+    s.initialise(*O.uniform_field(mesh.cells, speed=w["speed"], alpha_deg=w["alpha"], density=w["density"],
+                                  reynolds=w["reynolds"]))
+    s.iterate(1)  # first iteration outside the sample (page faults, first touch)
+    t0 = time.perf_counter()
+    s.iterate(iterations)
+    sec = (time.perf_counter() - t0) / iterations
+    value = (1.0 / (sec * iters_per_solve)) if iters_per_solve else None
+    return {"value": value, "unit": UNIT if iters_per_solve else "outer iterations/s",
+            "iterations_per_s": 1.0 / sec, "s_per_iteration": sec, "cores": 1, "kind": "port",
+            "sample": f"{iterations} outer iterations of the same solve (FP64 oracle, 1 thread, -O2), "
+                      f"after 1 untimed"}
+
+
+def run_bench(args):
+    import cfdgpu as G
+
+    if G.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU: no HIP device visible")
+    mesh, kind, vel, pres, field = build_case(G)
+    s = gpu_solver(G, mesh, kind, vel, pres, field)
+    if args.warmup:
+        s.iterate(args.warmup)
+    t0 = time.perf_counter()
+    mon, stage_ms, stage_calls = s.iterate_timed(args.steps)  # returns after the stream has drained
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+
+    # dominant stage and its kernel
+    ranked = sorted(stage_ms.items(), key=lambda kv: -kv[1])
+    dom = ranked[0][0]
+    kernels = {"momentum_sweeps": "k_gs_lex_mom", "pressure_solve": "k_cg_a/k_cg_b",
+               "sst_sweeps": "k_gs_lex_turb", "momentum_assembly": "k_mom", "gradients": "k_grad"}
+    calls = max(stage_calls[dom], 1)
+    per_launch_ms = stage_ms[dom] / calls
+    algo = stage_bytes(dom, mesh.cells)
+    roofline = {"bound": "hbm", "kernel": kernels.get(dom, dom), "stage": dom,
+                "achieved": (algo / (per_launch_ms * 1e-3) / 1e9) if algo else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+                "algorithmic_bytes_per_launch": algo, "launch_ms": per_launch_ms}
+    if roofline["achieved"] is not None:
+        roofline["frac"] = roofline["achieved"] / HBM_PEAK_GBS
+
+    n_conv = ITERATIONS_TO_CONVERGE
+    value = (1000.0 / (ms_per_step * n_conv)) if n_conv else None
+    line = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "NACA 2412 C-grid 1024x256 (262,144 cells, y1 1e-6 c), k-omega SST, Re 1e6, "
+                               "alpha 4 deg, Tu 0.1%, mut/mu 10; plain FP64, lexicographic GS; "
+                               "one step = one SIMPLE outer iteration",
+                   "cells": mesh.cells, "iterations_per_solve": n_conv,
+                   "time_to_convergence_s": (ms_per_step * n_conv / 1000.0) if n_conv else None,
+                   "outer_iterations_per_s": 1000.0 / ms_per_step,
+                   "cg_iterations_last": mon.pressure_iterations,
+                   "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items() if stage_calls[k]}},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_iterations, n_conv)
+    print(json.dumps(line), flush=True)
+
+
+def run_converge(args):
+    """Run the solve until worst() < tol, printing Cl/Cd/Cm every chunk, then
+    report the first iteration after which |dCl| < 1e-3 and |dCd| < 1e-4 hold."""
+    import cfdgpu as G
+
+    mesh, kind, vel, pres, field = build_case(G)
+    s = gpu_solver(G, mesh, kind, vel, pres, field)
+    hist = []
+    done = 0
+    t0 = time.perf_counter()
+    while done < args.max_iterations:
+        mon = s.iterate(args.chunk)
+        done += args.chunk
+        f = coefficients(G, mesh, s)
+        hist.append((done, mon.worst(), f.cl, f.cd, f.cm))
+        print(f"{done} {time.perf_counter() - t0:8.1f}s worst={mon.worst():.3e} cl={f.cl:.8f} cd={f.cd:.8f} "
+              f"cm={f.cm:.8f} eddy={mon.max_eddy_ratio:.1f}", flush=True)
+        if mon.worst() < args.tolerance:
+            break
+    cl_f, cd_f = hist[-1][2], hist[-1][3]
+    n_conv = None
+    for it, _, cl, cd, _ in reversed(hist):
+        if abs(cl - cl_f) < 1e-3 and abs(cd - cd_f) < 1e-4:
+            n_conv = it
+        else:
+            break
+    print(json.dumps({"iterations": done, "final_worst": hist[-1][1], "cl": cl_f, "cd": cd_f,
+                      "cm": hist[-1][4], "iterations_to_converge": n_conv, "chunk": args.chunk}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-iterations", type=int, default=6, help="oracle iterations in the CPU sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--converge", action="store_true", help="run the solve to convergence instead")
+    ap.add_argument("--chunk", type=int, default=250)
+    ap.add_argument("--max-iterations", type=int, default=40000)
+    ap.add_argument("--tolerance", type=float, default=1e-7)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
+              "by instance, one solve per GPU). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.converge:
+        run_converge(args)
+    else:
+        run_bench(args)
+
+
+if __name__ == "__main__":
+    main()

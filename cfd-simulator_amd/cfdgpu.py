@@ -34,7 +34,7 @@ EXPORTS = [
     "cfdgpu_mesh_dims", "cfdgpu_mesh_nodes", "cfdgpu_mesh_cells", "cfdgpu_mesh_faces",
     "cfdgpu_mesh_quality", "cfdgpu_face_conditions", "cfdgpu_uniform_field", "cfdgpu_default_settings",
     "cfdgpu_default_sst", "cfdgpu_create", "cfdgpu_destroy", "cfdgpu_set_turbulence", "cfdgpu_initialise",
-    "cfdgpu_iterate", "cfdgpu_solve", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
+    "cfdgpu_iterate", "cfdgpu_iterate_timed", "cfdgpu_solve", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
     "cfdgpu_set_field", "cfdgpu_get_massflux", "cfdgpu_set_massflux", "cfdgpu_get_turbulence",
     "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_surface",
     "cfdgpu_sweep_angles", "cfdgpu_debug_array",
@@ -75,6 +75,12 @@ class Settings(C.Structure):
 class SstConstants(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("sigma_k1", "sigma_w1", "beta1", "sigma_k2", "sigma_w2", "beta2",
                                           "beta_star", "kappa", "a1", "production_limit")]
+
+
+# CFDGPU_STAGE_* in order
+STAGES = ("gradients", "momentum_assembly", "momentum_sweeps", "rhie_chow", "pressure_assembly",
+          "pressure_solve", "correction", "sst_blending", "sst_assembly", "sst_sweeps",
+          "sst_wall_viscosity", "monitor")
 
 
 class Monitor(C.Structure):
@@ -309,6 +315,14 @@ class SimpleSolver:
         m = Monitor()
         _check(lib().cfdgpu_iterate(self.h, count, C.byref(m)))
         return m
+
+    def iterate_timed(self, count: int = 1):
+        """iterate() with HIP events around every stage: (monitor, {stage: ms}, {stage: calls})."""
+        m = Monitor()
+        ms = (C.c_double * len(STAGES))()
+        calls = (C.c_int * len(STAGES))()
+        _check(lib().cfdgpu_iterate_timed(self.h, count, C.byref(m), ms, calls))
+        return m, dict(zip(STAGES, list(ms))), dict(zip(STAGES, list(calls)))
 
     def solve(self, tolerance: float, max_iterations: int, check_every: int = 25) -> SolveReport:
         rule = StopRule(tolerance, max_iterations, check_every)

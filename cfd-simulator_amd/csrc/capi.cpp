@@ -378,6 +378,12 @@ int cfdgpu_iterate(cfdgpu_solver* s, int count, cfdgpu_monitor* out) {
   return guarded([&]() -> int { return s->engine->iterate(count, out); });
 }
 
+int cfdgpu_iterate_timed(cfdgpu_solver* s, int count, cfdgpu_monitor* out, double* stage_ms, int* stage_calls) {
+  SOLVER_GUARD(s)
+  if (count < 0) return fail(CFDGPU_INVALID_ARGUMENT, "negative iteration count");
+  return guarded([&]() -> int { return s->engine->iterateTimed(count, out, stage_ms, stage_calls); });
+}
+
 int cfdgpu_solve(cfdgpu_solver* s, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out) {
   SOLVER_GUARD(s)
   if (!rule) return fail(CFDGPU_INVALID_ARGUMENT, "null stop rule");

@@ -123,6 +123,21 @@ __device__ __forceinline__ double waveMinPartials(const double* p, int n, double
   return s;
 }
 
+// Global-address-space views of device pointers. Loads and stores through a
+// generic pointer compile to FLAT instructions, which on CDNA count in
+// lgkmcnt as well as vmcnt: a wait for LDS traffic (lgkmcnt(0)) then also
+// waits for every outstanding HBM access. Kernels that overlap memory traffic
+// with LDS hand-offs go through these so their accesses are GLOBAL ones.
+#define CFD_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ CFD_GLOBAL T* gmem(T* p) {
+  return (CFD_GLOBAL T*)(p);
+}
+template <class T>
+__device__ __forceinline__ const CFD_GLOBAL T* gmem(const T* p) {
+  return (const CFD_GLOBAL T*)(p);
+}
+
 __device__ __forceinline__ double* partialSlot(const Inst& I, int slot) {
   return I.partials + static_cast<size_t>(slot) * kMaxRedBlocks;
 }

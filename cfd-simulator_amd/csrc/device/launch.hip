@@ -11,8 +11,10 @@ namespace cfdgpu {
 __global__ void k_begin_iteration(const Inst*, int);
 __global__ void k_grad(const Inst*);
 __global__ void k_mom(const Inst*);
-__global__ void k_gs_lex(const Inst*, int, int);
-__global__ void k_gs_lex_wide(const Inst*, int, int);
+__global__ void k_gs_lex_mom(const Inst*, int);
+__global__ void k_gs_lex_turb(const Inst*, int);
+__global__ void k_gs_lex_mom_wide(const Inst*, int);
+__global__ void k_gs_lex_turb_wide(const Inst*, int);
 __global__ void k_gs_color(const Inst*, int, int, int);
 __global__ void k_rhiechow(const Inst*);
 __global__ void k_pressure(const Inst*, int);
@@ -75,15 +77,22 @@ void launchMixedPressure(const Inst* d, const LaunchCfg& cfg, const Inst& h, hip
 
 }  // namespace
 
-void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s) {
+void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s, StageRecorder* rec) {
   const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
   const dim3 block(kBlock);
   const dim3 one((cfg.batch + 63) / 64);
   const bool lex = h.gsOrder == 0;
   const bool mixed = h.precision == 1;
+  auto begin = [&](int stage) { if (rec) rec->begin(stage, s); };
+  auto end = [&](int stage) { if (rec) rec->end(stage, s); };
+  begin(CFDGPU_STAGE_GRADIENTS);
   hipLaunchKernelGGL(k_begin_iteration, one, dim3(64), 0, s, d, cfg.batch);
   hipLaunchKernelGGL(k_grad, grid, block, 0, s, d);
+  end(CFDGPU_STAGE_GRADIENTS);
+  begin(CFDGPU_STAGE_MOMENTUM_ASSEMBLY);
   hipLaunchKernelGGL(k_mom, grid, block, 0, s, d);
+  end(CFDGPU_STAGE_MOMENTUM_ASSEMBLY);
+  begin(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   if (mixed) {
     // two-colour sweeps on the FP32 defect; the last adds it to u, v
     for (int sw = 0; sw < h.momSweeps; ++sw) {
@@ -91,19 +100,25 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
       hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 1, sw == h.momSweeps - 1 ? 1 : 0);
     }
   } else if (lex) {
-    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
-    else hipLaunchKernelGGL(k_gs_lex_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 0, h.momSweeps);
+    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_mom, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    else hipLaunchKernelGGL(k_gs_lex_mom_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
   } else {
     for (int sw = 0; sw < h.momSweeps; ++sw) {
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 0, 0);
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 1, 0);
     }
   }
+  end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
+  begin(CFDGPU_STAGE_RHIE_CHOW);
   hipLaunchKernelGGL(k_rhiechow, grid, block, 0, s, d);
+  end(CFDGPU_STAGE_RHIE_CHOW);
   // SimpleSolver.cpp:716-748
   const int passes = h.nonOrthPasses < 1 ? 1 : h.nonOrthPasses;
   for (int pass = 0; pass < passes; ++pass) {
+    begin(CFDGPU_STAGE_PRESSURE_ASSEMBLY);
     hipLaunchKernelGGL(k_pressure, grid, block, 0, s, d, pass);
+    end(CFDGPU_STAGE_PRESSURE_ASSEMBLY);
+    begin(CFDGPU_STAGE_PRESSURE_SOLVE);
     if (mixed) {
       if (h.psolve == 1) launchMixedPressure<1>(d, cfg, h, s);
       else launchMixedPressure<0>(d, cfg, h, s);
@@ -114,14 +129,24 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         hipLaunchKernelGGL(k_cg_b, grid, block, 0, s, d, it);
       }
     }
+    end(CFDGPU_STAGE_PRESSURE_SOLVE);
+    begin(CFDGPU_STAGE_CORRECTION);
     if (!h.hasPRef) hipLaunchKernelGGL(k_corr_mean, grid, block, 0, s, d);
     hipLaunchKernelGGL(k_correct_cells, grid, block, 0, s, d);
     hipLaunchKernelGGL(k_correct_faces, grid, block, 0, s, d);
+    end(CFDGPU_STAGE_CORRECTION);
   }
+  begin(CFDGPU_STAGE_CORRECTION);
   hipLaunchKernelGGL(k_divergence, grid, block, 0, s, d, static_cast<double*>(nullptr));
+  end(CFDGPU_STAGE_CORRECTION);
   if (h.model == 2) {
+    begin(CFDGPU_STAGE_SST_BLENDING);
     hipLaunchKernelGGL(k_sst_blend, grid, block, 0, s, d);
+    end(CFDGPU_STAGE_SST_BLENDING);
+    begin(CFDGPU_STAGE_SST_ASSEMBLY);
     hipLaunchKernelGGL(k_sst_asm, grid, block, 0, s, d);
+    end(CFDGPU_STAGE_SST_ASSEMBLY);
+    begin(CFDGPU_STAGE_SST_SWEEPS);
     if (mixed) {
       for (int sw = 0; sw < 3; ++sw) {
         const int last = (sw == 2) ? 1 : 0;
@@ -131,8 +156,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 1, last);
       }
     } else if (lex) {
-      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
-      else hipLaunchKernelGGL(k_gs_lex_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 1, 3);
+      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      else hipLaunchKernelGGL(k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
     } else {
       for (int sw = 0; sw < 3; ++sw) {
         const int floor = (sw == 2) ? 1 : 0;
@@ -142,10 +167,15 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 2, 1, floor);
       }
     }
+    end(CFDGPU_STAGE_SST_SWEEPS);
+    begin(CFDGPU_STAGE_SST_WALL_VISCOSITY);
     hipLaunchKernelGGL(k_sst_wall, dim3(1, cfg.batch), block, 0, s, d);
     hipLaunchKernelGGL(k_sst_mut, grid, block, 0, s, d);
+    end(CFDGPU_STAGE_SST_WALL_VISCOSITY);
   }
+  begin(CFDGPU_STAGE_MONITOR);
   hipLaunchKernelGGL(k_monitor, dim3(1, cfg.batch), dim3(64), 0, s, d);
+  end(CFDGPU_STAGE_MONITOR);
 }
 
 void launchDivergence(const Inst* d, const LaunchCfg& cfg, double* out, hipStream_t s) {

@@ -315,6 +315,24 @@ struct GsSys {
   int hasFloor;
 };
 
+// The lexicographic sweep's view of a system: every pointer in the global
+// address space (see gmem), so the per-step LDS barrier never waits for the
+// read-ahead loads or the row's stores.
+struct GsSysG {
+  const CFD_GLOBAL double *aP, *aS, *aN, *aW, *aE;
+  const CFD_GLOBAL double* b[2];
+  CFD_GLOBAL double* x[2];
+  const CFD_GLOBAL int* partner;
+  double floorValue;
+  int hasFloor;
+};
+
+__device__ __forceinline__ GsSysG globalView(const GsSys& S, const int* partner) {
+  return GsSysG{gmem(S.aP), gmem(S.aS), gmem(S.aN), gmem(S.aW), gmem(S.aE),
+                {gmem(S.b[0]), gmem(S.b[1])}, {gmem(S.x[0]), gmem(S.x[1])}, gmem(partner),
+                S.floorValue, S.hasFloor};
+}
+
 // Everything a lane needs for one cell of its row that does not depend on
 // this sweep's updates: the row of the matrix, the right-hand side, the wake
 // partner and the old values of the cell, above and to the right. Old values
@@ -327,26 +345,30 @@ struct GsCell {
   int pr;
 };
 
+// Fetch the cell of row jr on anti-diagonal k. The loads are unconditional
+// (indices clamped into the mesh) so every lane issues the same number on
+// every step: the compiler can then wait for the oldest fetch only and keep
+// the newer ones in flight. Values of cells off the wavefront are never used.
 template <int NC>
-__device__ __forceinline__ void gsFetch(const Inst& I, const GsSys& S, double* const* xOld, int jr, int k,
-                                        GsCell<NC>& g) {
-  const int nI = I.nI, nJ = I.nJ;
-  const int i = k - jr;
-  if (jr < nJ && i >= 0 && i < nI) {
-    const int c = jr * nI + i;
-    g.aP = S.aP[c];
-    g.aS = S.aS[c];
-    g.aN = S.aN[c];
-    g.aW = S.aW[c];
-    g.aE = S.aE[c];
-    g.pr = (jr == 0) ? I.partner[i] : -1;
+__device__ __forceinline__ void gsFetch(int nI, int nJ, const GsSysG& S, CFD_GLOBAL double* const* xOld, int jr,
+                                        int k, GsCell<NC>& g) {
+  const int i = min(max(k - jr, 0), nI - 1);
+  const int j = min(jr, nJ - 1);
+  const int c = j * nI + i;
+  const int cN = (j < nJ - 1) ? c + nI : c;
+  const int cE = (i < nI - 1) ? c + 1 : c;
+  g.aP = S.aP[c];
+  g.aS = S.aS[c];
+  g.aN = S.aN[c];
+  g.aW = S.aW[c];
+  g.aE = S.aE[c];
+  g.pr = S.partner[i];  // used on row 0 only
 #pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      g.b[q] = S.b[q][c];
-      g.xC[q] = xOld[q][c];
-      g.xN[q] = (jr < nJ - 1) ? xOld[q][c + nI] : 0.0;
-      g.xE[q] = (i < nI - 1) ? xOld[q][c + 1] : 0.0;
-    }
+  for (int q = 0; q < NC; ++q) {
+    g.b[q] = S.b[q][c];
+    g.xC[q] = xOld[q][c];
+    g.xN[q] = xOld[q][cN];
+    g.xE[q] = xOld[q][cE];
   }
 }
 
@@ -362,8 +384,14 @@ __device__ __forceinline__ void ldsBarrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-template <int NC, int CH>
-__device__ void gsLexBody(const Inst& I, const GsSys& S, double* const* tmp, int sweeps) {
+// D is the read-ahead depth: a ring of D cells per lane, slot q holding the
+// cell of step k0 + q; it is refilled with step k0 + q + D right after use.
+template <int NC, int D>
+__device__ void gsLexBody(const Inst& I, const GsSys& Sg, double* const* tmpg, int sweeps) {
+  const GsSysG S = globalView(Sg, I.partner);
+  CFD_GLOBAL double* tmp[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) tmp[q] = gmem(tmpg[q]);
   HIP_DYNAMIC_SHARED(double, lds)
   const int nI = I.nI, nJ = I.nJ;
   const int R = static_cast<int>(blockDim.x);
@@ -371,12 +399,14 @@ __device__ void gsLexBody(const Inst& I, const GsSys& S, double* const* tmp, int
   double* row0 = lds + 2 * NC * R;   // [NC][nI]: row j = 0, updated in place
   const int jr = static_cast<int>(threadIdx.x);
   const bool active = jr < nJ;
+  // Steps past nI + nJ - 1 touch no cell (k - jr >= nI for every row), so the
+  // count is rounded up to whole rings and the loop body has no tail.
   const int steps = nI + nJ - 1;
-  const int chunks = (steps + CH - 1) / CH;
+  const int rings = (steps + D - 1) / D;
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     // even sweeps read x and write tmp, odd sweeps the other way round
-    double* xOld[NC];
-    double* xNew[NC];
+    CFD_GLOBAL double* xOld[NC];
+    CFD_GLOBAL double* xNew[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
       xOld[q] = (sweep & 1) ? tmp[q] : S.x[q];
@@ -390,72 +420,60 @@ __device__ void gsLexBody(const Inst& I, const GsSys& S, double* const* tmp, int
     double xw[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) xw[q] = 0.0;
-    GsCell<NC> cur[CH], nxt[CH];
+    GsCell<NC> ring[D];
 #pragma unroll
-    for (int q = 0; q < CH; ++q) gsFetch<NC>(I, S, xOld, jr, q, cur[q]);
-    for (int ch = 0; ch < chunks; ++ch) {
-      if (ch + 1 < chunks) {
+    for (int q = 0; q < D; ++q) gsFetch<NC>(nI, nJ, S, xOld, jr, q, ring[q]);
+    for (int r = 0; r < rings; ++r) {
 #pragma unroll
-        for (int q = 0; q < CH; ++q) gsFetch<NC>(I, S, xOld, jr, (ch + 1) * CH + q, nxt[q]);
-      }
+      for (int q = 0; q < D; ++q) {
+        const int k = r * D + q;
+        const int i = k - jr;
+        const bool on = active && i >= 0 && i < nI;
+        const GsCell<NC>& g = ring[q];
+        // the value below: the row under this one (LDS, written last step),
+        // or on row 0 the wake partner's current value (row0, in place)
+        const bool hasBelow = (jr > 0) || (g.pr >= 0);
+        const int belowRow = (jr > 0) ? jr - 1 : 0;
+        const int pr = (g.pr >= 0) ? g.pr : 0;
+        double xn[NC];
 #pragma unroll
-      for (int q = 0; q < CH; ++q) {
-        const int k = ch * CH + q;
-        if (k < steps) {
-          const int i = k - jr;
-          if (active && i >= 0 && i < nI) {
-            const GsCell<NC>& g = cur[q];
-            const int c = jr * nI + i;
-            double xn[NC];
-            if (fabs(g.aP) > 0.0) {
-              double off[NC];
-#pragma unroll
-              for (int m = 0; m < NC; ++m) off[m] = 0.0;
-              // cellFaces order: below, right, above, left (Mesh.cpp:206-213)
-              if (jr > 0) {
-#pragma unroll
-                for (int m = 0; m < NC; ++m) off[m] += g.aS * buf[((k - 1) & 1) * NC * R + m * R + (jr - 1)];
-              } else if (g.pr >= 0) {
-#pragma unroll
-                for (int m = 0; m < NC; ++m) off[m] += g.aS * row0[m * nI + g.pr];
-              }
-              if (i < nI - 1) {
-#pragma unroll
-                for (int m = 0; m < NC; ++m) off[m] += g.aE * g.xE[m];
-              }
-              if (jr < nJ - 1) {
-#pragma unroll
-                for (int m = 0; m < NC; ++m) off[m] += g.aN * g.xN[m];
-              }
-              if (i > 0) {
-#pragma unroll
-                for (int m = 0; m < NC; ++m) off[m] += g.aW * xw[m];
-              }
-#pragma unroll
-              for (int m = 0; m < NC; ++m) xn[m] = (g.b[m] - off[m]) / g.aP;
-            } else {
-#pragma unroll
-              for (int m = 0; m < NC; ++m) xn[m] = g.xC[m];  // row skipped (LinearSystem.cpp:86-88)
-            }
-#pragma unroll
-            for (int m = 0; m < NC; ++m) {
-              xNew[m][c] = xn[m];
-              buf[(k & 1) * NC * R + m * R + jr] = xn[m];
-              xw[m] = xn[m];
-              if (jr == 0) row0[m * nI + i] = xn[m];
-            }
-          }
-          ldsBarrier();
+        for (int m = 0; m < NC; ++m) {
+          const double below = (jr > 0) ? buf[((k - 1) & 1) * NC * R + m * R + belowRow] : row0[m * nI + pr];
+          // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
+          // a missing neighbour's term is skipped, not added as zero
+          double off = 0.0;
+          off = hasBelow ? off + g.aS * below : off;
+          off = (i < nI - 1) ? off + g.aE * g.xE[m] : off;
+          off = (jr < nJ - 1) ? off + g.aN * g.xN[m] : off;
+          off = (i > 0) ? off + g.aW * xw[m] : off;
+          // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88)
+          xn[m] = (fabs(g.aP) > 0.0) ? (g.b[m] - off) / g.aP : g.xC[m];
         }
-      }
+        if (on) {
+          const int c = jr * nI + i;
 #pragma unroll
-      for (int q = 0; q < CH; ++q) cur[q] = nxt[q];
+          for (int m = 0; m < NC; ++m) {
+            xNew[m][c] = xn[m];
+            if (jr == 0) row0[m * nI + i] = xn[m];
+          }
+        }
+        // Off the wavefront these are never read: the row above is off it on
+        // the next step too, and this row's next cell has i - 1 < 0 or i >= nI.
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+          buf[(k & 1) * NC * R + m * R + jr] = xn[m];
+          xw[m] = xn[m];
+        }
+        gsFetch<NC>(nI, nJ, S, xOld, jr, k + D, ring[q]);
+        ldsBarrier();
+      }
     }
     __syncthreads();  // this sweep's writes are the next sweep's old values
   }
   // an odd number of sweeps leaves the result in tmp
   const bool inTmp = (sweeps & 1) != 0;
-  for (int c = jr; c < I.N; c += R) {
+  const int N = nI * nJ;
+  for (int c = jr; c < N; c += R) {
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
       double v = inTmp ? tmp[q][c] : S.x[q][c];
@@ -465,38 +483,50 @@ __device__ void gsLexBody(const Inst& I, const GsSys& S, double* const* tmp, int
   }
 }
 
-// system 0: momentum (u, v); system 1: k; system 2: omega. blockIdx.x picks
-// the system among [first, first + count). Up to 256 rows: one lane per row
-// with a 4-cell (momentum) / 6-cell (k, omega) read-ahead held in registers.
-template <int CH2, int CH1>
-__device__ __forceinline__ void gsLexDispatch(const Inst* insts, int first, int sweeps) {
+// Momentum: u and v share one matrix, one workgroup. Turbulence: blockIdx.x
+// 0 sweeps k, 1 sweeps omega (independent systems, run side by side).
+template <int D>
+__device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped]) return;
-  const int sys = first + static_cast<int>(blockIdx.x);
-  if (sys == 0) {
-    const GsSys S{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
-    double* const tmp[2] = {I.gsTmp0, I.gsTmp1};
-    gsLexBody<2, CH2>(I, S, tmp, sweeps);
-  } else if (sys == 1) {
+  const GsSys S{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
+  double* const tmp[2] = {I.gsTmp0, I.gsTmp1};
+  gsLexBody<2, D>(I, S, tmp, sweeps);
+}
+
+template <int D>
+__device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
+  const Inst& I = insts[blockIdx.y];
+  if (I.flags[fStopped]) return;
+  if (blockIdx.x == 0) {
     const GsSys S{I.kP, I.kS, I.kN, I.kW, I.kE, {I.kB, nullptr}, {I.k, nullptr}, 0.0, 1};
     double* const tmp[2] = {I.gsTmp0, nullptr};
-    gsLexBody<1, CH1>(I, S, tmp, sweeps);
+    gsLexBody<1, D>(I, S, tmp, sweeps);
   } else {
     const GsSys S{I.wP, I.wS, I.wN, I.wW, I.wE, {I.wB, nullptr}, {I.w, nullptr}, 1.0e-10, 1};
     double* const tmp[2] = {I.gsTmp1, nullptr};
-    gsLexBody<1, CH1>(I, S, tmp, sweeps);
+    gsLexBody<1, D>(I, S, tmp, sweeps);
   }
 }
 
+// Up to 256 rows: one wave per 64 rows, each lane holding a ring of cells
+// fetched ahead (6 for u and v, 8 for k or omega).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_gs_lex(const Inst* insts, int first, int sweeps) {
-  gsLexDispatch<4, 6>(insts, first, sweeps);
+k_gs_lex_mom(const Inst* insts, int sweeps) {
+  gsLexMomentum<6>(insts, sweeps);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_gs_lex_turb(const Inst* insts, int sweeps) {
+  gsLexTurbulence<8>(insts, sweeps);
 }
 
-// More than 256 rows (up to 1024): no read-ahead, the register budget of a
-// 1024-lane block does not hold one.
-__global__ void __launch_bounds__(1024) k_gs_lex_wide(const Inst* insts, int first, int sweeps) {
-  gsLexDispatch<1, 1>(insts, first, sweeps);
+// More than 256 rows (up to 1024): the register budget of a 1024-lane block
+// holds a ring of two cells.
+__global__ void __launch_bounds__(1024) k_gs_lex_mom_wide(const Inst* insts, int sweeps) {
+  gsLexMomentum<2>(insts, sweeps);
+}
+__global__ void __launch_bounds__(1024) k_gs_lex_turb_wide(const Inst* insts, int sweeps) {
+  gsLexTurbulence<2>(insts, sweeps);
 }
 
 // ---------------------------------------------------------------------------

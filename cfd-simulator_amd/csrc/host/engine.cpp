@@ -34,6 +34,51 @@ struct Carver {
 
 }  // namespace
 
+// Event pairs around each stage occurrence, drawn from a pool that grows to
+// the largest timed call and is reused after that.
+class StageClock final : public StageRecorder {
+ public:
+  ~StageClock() {
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+  void reset() {
+    used_ = 0;
+    marks_.clear();
+  }
+  void begin(int stage, hipStream_t s) override {
+    marks_.push_back({stage, used_});
+    record(s);
+  }
+  void end(int, hipStream_t s) override { record(s); }
+  // after the stream has drained
+  void collect(double* ms, int* calls) const {
+    for (int q = 0; q < CFDGPU_STAGE_COUNT; ++q) {
+      if (ms) ms[q] = 0.0;
+      if (calls) calls[q] = 0;
+    }
+    for (const auto& m : marks_) {
+      float t = 0.0f;
+      check(hipEventElapsedTime(&t, pool_[m.second], pool_[m.second + 1]), "stage timing");
+      if (ms) ms[m.first] += static_cast<double>(t);
+      if (calls) calls[m.first] += 1;
+    }
+  }
+
+ private:
+  void record(hipStream_t s) {
+    if (used_ == pool_.size()) {
+      hipEvent_t e{};
+      check(hipEventCreate(&e), "hipEventCreate");
+      pool_.push_back(e);
+    }
+    check(hipEventRecord(pool_[used_], s), "hipEventRecord");
+    ++used_;
+  }
+  std::vector<hipEvent_t> pool_;
+  size_t used_{0};
+  std::vector<std::pair<int, size_t>> marks_;
+};
+
 DeviceArena::~DeviceArena() {
   if (base) (void)hipFree(base);
 }
@@ -542,6 +587,21 @@ int Engine::iterate(int count, cfdgpu_monitor* out) {
   check(hipGetLastError(), "launch");
   if (out) readMonitor(out);
   else check(hipStreamSynchronize(stream_), "iterate sync");
+  return CFDGPU_OK;
+}
+
+int Engine::iterateTimed(int count, cfdgpu_monitor* out, double* stageMs, int* stageCalls) {
+  if (!initialised_) return CFDGPU_INITIALIZATION_FAILURE;
+  if (!clock_) clock_ = std::make_unique<StageClock>();
+  StageClock& clock = *clock_;
+  clock.reset();
+  int zero[2] = {0, 0};
+  check(hipMemcpyAsync(h_.flags + fStopped, zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
+  for (int n = 0; n < count; ++n) launchIteration(dInst_, cfg_, h_, stream_, &clock);
+  check(hipGetLastError(), "launch");
+  check(hipStreamSynchronize(stream_), "iterate sync");
+  if (out) readMonitor(out);
+  clock.collect(stageMs, stageCalls);
   return CFDGPU_OK;
 }
 

@@ -28,10 +28,21 @@ struct LaunchCfg {
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
 };
-void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hipStream_t s);
+// Optional hook around each stage of an outer iteration (CFDGPU_STAGE_*),
+// called on the host as the launches are queued.
+struct StageRecorder {
+  virtual void begin(int stage, hipStream_t s) = 0;
+  virtual void end(int stage, hipStream_t s) = 0;
+ protected:
+  ~StageRecorder() = default;
+};
+void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hipStream_t s,
+                     StageRecorder* rec = nullptr);
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
 int gsThreadsFor(int nJ);
 size_t gsLdsFor(int nJ, int nI, int threads);
+
+class StageClock;
 
 // Device arena for one instance: every array of Inst carved out of one allocation.
 struct DeviceArena {
@@ -51,6 +62,9 @@ class Engine {
   int setModel(int model, const cfdgpu_sst_constants* constants);
   int initialise(const double* u, const double* v, const double* p, const double* rho, const double* mu);
   int iterate(int count, cfdgpu_monitor* out);
+  // iterate() with HIP events around every stage; per-stage device time summed
+  // over the iterations (stageMs) and how often each stage ran (stageCalls)
+  int iterateTimed(int count, cfdgpu_monitor* out, double* stageMs, int* stageCalls);
   int solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out);
   void setSettings(const cfdgpu_settings& s);
   const cfdgpu_settings& settings() const { return settings_; }
@@ -107,6 +121,7 @@ class Engine {
   Inst* dInst_{nullptr};
   hipStream_t stream_{nullptr};
   LaunchCfg cfg_{};
+  std::unique_ptr<StageClock> clock_;
 };
 
 }  // namespace cfdgpu

@@ -58,6 +58,23 @@ enum {
                                   mixed path only, so currently rejected (NOT_IMPLEMENTED) */
 };
 
+/* stages of one outer iteration, for cfdgpu_iterate_timed */
+enum {
+  CFDGPU_STAGE_GRADIENTS = 0,          /* face values + Green-Gauss of u, v, p (SimpleSolver.cpp:268-317) */
+  CFDGPU_STAGE_MOMENTUM_ASSEMBLY = 1,  /* assembleMomentum + applyRelaxation + residual (:319-443) */
+  CFDGPU_STAGE_MOMENTUM_SWEEPS = 2,    /* 3 Gauss-Seidel sweeps of u and v (LinearSystem.cpp:77-116) */
+  CFDGPU_STAGE_RHIE_CHOW = 3,          /* computeRhieChowFluxes (:445-496) */
+  CFDGPU_STAGE_PRESSURE_ASSEMBLY = 4,  /* assemblePressureCorrection (:498-580) */
+  CFDGPU_STAGE_PRESSURE_SOLVE = 5,     /* conjugateGradient (LinearSystem.cpp:118-210) */
+  CFDGPU_STAGE_CORRECTION = 6,         /* correct (:582-660) + divergence monitor (:796-817) */
+  CFDGPU_STAGE_SST_BLENDING = 7,       /* strain, gradients of k and omega, F1, F2, CD, production */
+  CFDGPU_STAGE_SST_ASSEMBLY = 8,       /* k and omega rows, relaxation, residuals */
+  CFDGPU_STAGE_SST_SWEEPS = 9,         /* 3 Gauss-Seidel sweeps of k and of omega */
+  CFDGPU_STAGE_SST_WALL_VISCOSITY = 10,/* wall omega + eddy viscosity + effective viscosity */
+  CFDGPU_STAGE_MONITOR = 11,           /* reductions, SolverMonitor, stop rule */
+  CFDGPU_STAGE_COUNT = 12
+};
+
 typedef struct cfdgpu_mesh cfdgpu_mesh;
 typedef struct cfdgpu_solver cfdgpu_solver;
 
@@ -186,6 +203,12 @@ int cfdgpu_initialise(cfdgpu_solver* solver, const double* u, const double* v, c
                       const double* rho, const double* mu);
 /* SimpleSolver::iterate (SimpleSolver.cpp:662-794), count times; monitor of the last */
 int cfdgpu_iterate(cfdgpu_solver* solver, int count, cfdgpu_monitor* out);
+/* cfdgpu_iterate with HIP events recorded on the solver's stream around every
+   stage: stage_ms[CFDGPU_STAGE_COUNT] receives each stage's device time summed
+   over the iterations, stage_calls[CFDGPU_STAGE_COUNT] how often it ran (either
+   may be NULL). No reference counterpart: the measurement hook of this path. */
+int cfdgpu_iterate_timed(cfdgpu_solver* solver, int count, cfdgpu_monitor* out, double* stage_ms,
+                         int* stage_calls);
 /* the SolverWorker loop (SolverWorker.cpp:134-277) run device-resident */
 int cfdgpu_solve(cfdgpu_solver* solver, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out);
 int cfdgpu_get_settings(const cfdgpu_solver* solver, cfdgpu_settings* out);

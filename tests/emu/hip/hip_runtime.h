@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <barrier>
 #include <condition_variable>
 #include <mutex>
@@ -33,6 +34,9 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 using std::isfinite;
+// HIP device code has integer min / max overloads in the global namespace.
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a < b ? b : a; }
 
 struct dim3 {
   unsigned x, y, z;
@@ -186,6 +190,18 @@ inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
   return hipSuccess;
 }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+// Launches run to completion when queued, so an event is a host timestamp.
+struct emu_event {
+  std::chrono::steady_clock::time_point t;
+};
+using hipEvent_t = emu_event*;
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new emu_event(); return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = std::chrono::steady_clock::now(); return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+  *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
+  return hipSuccess;
+}
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 
 template <class... P, class... A>
