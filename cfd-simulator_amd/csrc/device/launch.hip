@@ -39,17 +39,13 @@ template <int PRE> __global__ void k_cgf_init(const Inst*);
 template <int PRE> __global__ void k_cgf_a(const Inst*, int);
 template <int PRE> __global__ void k_cgf_b(const Inst*, int);
 
-int gsThreadsFor(int nJ) {
-  int t = ((nJ + 63) / 64) * 64;
-  if (t < 64) t = 64;
-  if (t > 1024) t = 1024;
-  return t;
-}
+int gsThreadsFor(int nJ) { return gsLanes(nJ); }
 
 size_t gsLdsFor(int nJ, int nI, int threads) {
   (void)nJ;
-  // [2][2][threads] step buffer + [2][nI] row 0
-  return (static_cast<size_t>(2) * 2 * static_cast<size_t>(threads) + 2 * static_cast<size_t>(nI)) * sizeof(double);
+  // [2][2][threads] step buffer + [2][nI] row 0 (doubles) + [nI] partner map (ints)
+  return (static_cast<size_t>(2) * 2 * static_cast<size_t>(threads) + 2 * static_cast<size_t>(nI)) * sizeof(double) +
+         static_cast<size_t>(nI) * sizeof(int);
 }
 
 namespace {

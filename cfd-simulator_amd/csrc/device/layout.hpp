@@ -95,6 +95,19 @@ struct DevMonitor {
 
 constexpr int kMaxRedBlocks = 4096;  // partial-sum capacity per slot
 
+// Skewed layout of the lexicographic sweep. Every index a sweep forms is
+// (k * R + lane) or ((k + 1) * R + lane + 1) with the step k clamped to
+// [0, steps - 1] and lane < R, so it stays below skewLength.
+constexpr int kSkMom = 11;
+constexpr int kSkTurb = 8;
+// Lanes of the sweep for nJ rows: one per row, rounded up to whole waves.
+__host__ __device__ constexpr int gsLanes(int nJ) {
+  return nJ <= 64 ? 64 : (nJ >= 1024 ? 1024 : ((nJ + 63) / 64) * 64);
+}
+__host__ __device__ constexpr int skewSteps(int nI, int nJ) { return nI + nJ - 1; }
+__host__ __device__ constexpr int skewIndex(int i, int j, int R) { return (i + j) * R + j; }
+__host__ __device__ constexpr int skewLength(int nI, int nJ, int R) { return (skewSteps(nI, nJ) + 1) * R + R; }
+
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 
 struct Inst {
@@ -138,8 +151,19 @@ struct Inst {
   double *pP, *pN, *pE, *pWk, *pb;
   // CG vectors
   double *x, *r, *z, *pd0, *pd1, *Ap;
-  // ping-pong buffers of the lexicographic sweep (old values of the previous sweep)
+  // ping-pong buffers of the two-colour sweep's callers (unused by the lexicographic one)
   double *gsTmp0, *gsTmp1;
+
+  // Lexicographic sweep in the anti-diagonal ("skewed") layout: cell (i, j) is
+  // stored at skewIndex(i, j, gsR), so the cells of one wavefront step are
+  // contiguous and a wave's loads coalesce. The assembly kernels write the
+  // systems here; the sweeps ping-pong between x0 and x1 and the last sweep
+  // writes the row-major field.
+  int gsR;        // lanes of the sweep (rows rounded up to 64) = stride of the layout
+  int gsSkewLen;  // elements of each skewed array: skewLength(nI, nJ, gsR)
+  double* skM[kSkMom];  // momentum: aP aS aN aW aE bx by u0 v0 u1 v1
+  double* skK[kSkTurb]; // k: aP aS aN aW aE b x0 x1
+  double* skW[kSkTurb]; // omega: aP aS aN aW aE b x0 x1
 
   // k-omega SST state and systems
   double *k, *w, *mut, *f1, *f2, *cd, *strain, *prod;

@@ -261,6 +261,19 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     I.aE[c] = aE;
     I.bx[c] = row.bx;
     I.by[c] = row.by;
+    if (I.gsOrder == 0) {
+      // the lexicographic sweep's copy, anti-diagonal layout
+      const int sk = skewIndex(i, j, I.gsR);
+      I.skM[0][sk] = d;
+      I.skM[1][sk] = aS;
+      I.skM[2][sk] = aN;
+      I.skM[3][sk] = aW;
+      I.skM[4][sk] = aE;
+      I.skM[5][sk] = row.bx;
+      I.skM[6][sk] = row.by;
+      I.skM[7][sk] = uc;
+      I.skM[8][sk] = vc;
+    }
     // residual with the starting field (LinearSystem.cpp:18-54, face order)
     double axu = d * uc, axv = d * vc;
     if (j > 0) { axu += aS * I.u[c - nI]; axv += aS * I.v[c - nI]; }
@@ -315,65 +328,57 @@ struct GsSys {
   int hasFloor;
 };
 
-// The lexicographic sweep's view of a system: every pointer in the global
-// address space (see gmem), so the per-step LDS barrier never waits for the
-// read-ahead loads or the row's stores.
-struct GsSysG {
+// One system of the lexicographic sweep in the skewed layout (see Inst::gsR):
+// matrix rows and right-hand sides, the two ping-pong copies of the unknowns
+// (x0 holds the starting values, written by the assembly kernel) and the
+// row-major field that the last sweep writes. All pointers are global-space
+// (see gmem), so the per-step LDS barrier never waits for HBM traffic.
+template <int NC>
+struct GsSkew {
   const CFD_GLOBAL double *aP, *aS, *aN, *aW, *aE;
-  const CFD_GLOBAL double* b[2];
-  CFD_GLOBAL double* x[2];
-  const CFD_GLOBAL int* partner;
-  double floorValue;
+  const CFD_GLOBAL double* b[NC];
+  CFD_GLOBAL double* x0[NC];
+  CFD_GLOBAL double* x1[NC];
+  CFD_GLOBAL double* out[NC];
+  double floorValue;  // applied to the values the last sweep writes, when hasFloor
   int hasFloor;
 };
 
-__device__ __forceinline__ GsSysG globalView(const GsSys& S, const int* partner) {
-  return GsSysG{gmem(S.aP), gmem(S.aS), gmem(S.aN), gmem(S.aW), gmem(S.aE),
-                {gmem(S.b[0]), gmem(S.b[1])}, {gmem(S.x[0]), gmem(S.x[1])}, gmem(partner),
-                S.floorValue, S.hasFloor};
-}
-
-// Everything a lane needs for one cell of its row that does not depend on
-// this sweep's updates: the row of the matrix, the right-hand side, the wake
-// partner and the old values of the cell, above and to the right. Old values
-// come from the previous sweep's buffer, which this sweep never writes, so
-// they can be read any number of steps ahead.
+// What a lane needs for its cell on one wavefront step that does not depend
+// on this sweep's updates: the matrix row, the right-hand side, and the old
+// values of the cell and of the cell above. The old value of the cell to the
+// right is the next step's xC, so it is not fetched separately.
 template <int NC>
 struct GsCell {
   double aP, aS, aN, aW, aE;
-  double b[NC], xN[NC], xE[NC], xC[NC];
-  int pr;
+  double b[NC], xC[NC], xN[NC];
 };
 
-// Fetch the cell of row jr on anti-diagonal k. The loads are unconditional
-// (indices clamped into the mesh) so every lane issues the same number on
-// every step: the compiler can then wait for the oldest fetch only and keep
-// the newer ones in flight. Values of cells off the wavefront are never used.
+// Fetch lane's cell on step k. Loads are unconditional with the step clamped
+// into [0, steps - 1], so every lane issues the same loads on every step and
+// the compiler can keep the newer ones in flight while it waits for the
+// oldest; values of cells off the wavefront are never used. Indices are
+// k * R + lane and (k + 1) * R + lane + 1, below skewLength(nI, nJ, R).
 template <int NC>
-__device__ __forceinline__ void gsFetch(int nI, int nJ, const GsSysG& S, CFD_GLOBAL double* const* xOld, int jr,
-                                        int k, GsCell<NC>& g) {
-  const int i = min(max(k - jr, 0), nI - 1);
-  const int j = min(jr, nJ - 1);
-  const int c = j * nI + i;
-  const int cN = (j < nJ - 1) ? c + nI : c;
-  const int cE = (i < nI - 1) ? c + 1 : c;
-  g.aP = S.aP[c];
-  g.aS = S.aS[c];
-  g.aN = S.aN[c];
-  g.aW = S.aW[c];
-  g.aE = S.aE[c];
-  g.pr = S.partner[i];  // used on row 0 only
+__device__ __forceinline__ void gsFetch(const GsSkew<NC>& S, const CFD_GLOBAL double* const* xOld, int R,
+                                        int steps, int lane, int k, GsCell<NC>& g) {
+  const int at = min(k, steps - 1) * R + lane;
+  const int atN = min(k + 1, steps - 1) * R + lane + 1;
+  g.aP = S.aP[at];
+  g.aS = S.aS[at];
+  g.aN = S.aN[at];
+  g.aW = S.aW[at];
+  g.aE = S.aE[at];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
-    g.b[q] = S.b[q][c];
-    g.xC[q] = xOld[q][c];
-    g.xN[q] = xOld[q][cN];
-    g.xE[q] = xOld[q][cE];
+    g.b[q] = S.b[q][at];
+    g.xC[q] = xOld[q][at];
+    g.xN[q] = xOld[q][atN];
   }
 }
 
 // Barrier for the LDS hand-off between wavefront steps only: waits for this
-// wave's LDS writes, not for its outstanding global loads (the read-ahead),
+// wave's LDS traffic, not for its outstanding global loads (the read-ahead),
 // which __syncthreads() would drain at every step. Global memory is not
 // shared between lanes within a sweep (old values are read from one buffer,
 // new values written to another), so no global fence is needed here.
@@ -384,37 +389,41 @@ __device__ __forceinline__ void ldsBarrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// D is the read-ahead depth: a ring of D cells per lane, slot q holding the
-// cell of step k0 + q; it is refilled with step k0 + q + D right after use.
+// Lexicographic Gauss-Seidel on the anti-diagonal schedule: lane j owns row j
+// and handles cell (k - j, j) on step k. The value below comes from the lane
+// underneath through LDS (written on step k - 1); row 0 takes its wake
+// partner's current value from an LDS copy of the row, updated in place, so
+// the partner coupling follows the reference's order for any partner map.
+// D is the read-ahead: a ring of D cells per lane, slot q holding step
+// r * D + q and refilled with the step D ahead right after use (D >= 2, the
+// next step's cell supplies the east neighbour).
 template <int NC, int D>
-__device__ void gsLexBody(const Inst& I, const GsSys& Sg, double* const* tmpg, int sweeps) {
-  const GsSysG S = globalView(Sg, I.partner);
-  CFD_GLOBAL double* tmp[NC];
-#pragma unroll
-  for (int q = 0; q < NC; ++q) tmp[q] = gmem(tmpg[q]);
+__device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSkew<NC>& S, int sweeps) {
+  static_assert(D >= 2, "the ring must hold the next step");
   HIP_DYNAMIC_SHARED(double, lds)
-  const int nI = I.nI, nJ = I.nJ;
-  const int R = static_cast<int>(blockDim.x);
-  double* buf = lds;                 // [2][NC][R]: value just written by each row
-  double* row0 = lds + 2 * NC * R;   // [NC][nI]: row j = 0, updated in place
-  const int jr = static_cast<int>(threadIdx.x);
-  const bool active = jr < nJ;
-  // Steps past nI + nJ - 1 touch no cell (k - jr >= nI for every row), so the
+  double* buf = lds;                                        // [2][NC][R]: value just written by each row
+  double* row0 = lds + 2 * NC * R;                          // [NC][nI]: row j = 0, updated in place
+  int* partner = reinterpret_cast<int*>(row0 + NC * nI);    // [nI]
+  const int lane = static_cast<int>(threadIdx.x);
+  const bool active = lane < nJ;
+  const int steps = skewSteps(nI, nJ);
+  // Steps past nI + nJ - 2 touch no cell (k - j >= nI for every row), so the
   // count is rounded up to whole rings and the loop body has no tail.
-  const int steps = nI + nJ - 1;
   const int rings = (steps + D - 1) / D;
+  const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
+  for (int t = lane; t < nI; t += R) partner[t] = partnerSrc[t];
   for (int sweep = 0; sweep < sweeps; ++sweep) {
-    // even sweeps read x and write tmp, odd sweeps the other way round
-    CFD_GLOBAL double* xOld[NC];
+    const bool last = sweep == sweeps - 1;
+    const CFD_GLOBAL double* xOld[NC];
     CFD_GLOBAL double* xNew[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-      xOld[q] = (sweep & 1) ? tmp[q] : S.x[q];
-      xNew[q] = (sweep & 1) ? S.x[q] : tmp[q];
+      xOld[q] = (sweep & 1) ? S.x1[q] : S.x0[q];
+      xNew[q] = (sweep & 1) ? S.x0[q] : S.x1[q];
     }
-    for (int t = jr; t < nI; t += R) {
+    for (int t = lane; t < nI; t += R) {
 #pragma unroll
-      for (int q = 0; q < NC; ++q) row0[q * nI + t] = xOld[q][t];
+      for (int q = 0; q < NC; ++q) row0[q * nI + t] = xOld[q][skewIndex(t, 0, R)];
     }
     __syncthreads();
     double xw[NC];
@@ -422,106 +431,97 @@ __device__ void gsLexBody(const Inst& I, const GsSys& Sg, double* const* tmpg, i
     for (int q = 0; q < NC; ++q) xw[q] = 0.0;
     GsCell<NC> ring[D];
 #pragma unroll
-    for (int q = 0; q < D; ++q) gsFetch<NC>(nI, nJ, S, xOld, jr, q, ring[q]);
+    for (int q = 0; q < D; ++q) gsFetch<NC>(S, xOld, R, steps, lane, q, ring[q]);
     for (int r = 0; r < rings; ++r) {
 #pragma unroll
       for (int q = 0; q < D; ++q) {
         const int k = r * D + q;
-        const int i = k - jr;
+        const int i = k - lane;
         const bool on = active && i >= 0 && i < nI;
         const GsCell<NC>& g = ring[q];
-        // the value below: the row under this one (LDS, written last step),
-        // or on row 0 the wake partner's current value (row0, in place)
-        const bool hasBelow = (jr > 0) || (g.pr >= 0);
-        const int belowRow = (jr > 0) ? jr - 1 : 0;
-        const int pr = (g.pr >= 0) ? g.pr : 0;
+        const GsCell<NC>& east = ring[(q + 1) % D];
+        const int pr = partner[min(max(i, 0), nI - 1)];
+        const bool hasBelow = (lane > 0) || (pr >= 0);
         double xn[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-          const double below = (jr > 0) ? buf[((k - 1) & 1) * NC * R + m * R + belowRow] : row0[m * nI + pr];
+          const double below =
+              (lane > 0) ? buf[((k - 1) & 1) * NC * R + m * R + (lane - 1)] : row0[m * nI + max(pr, 0)];
           // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
           // a missing neighbour's term is skipped, not added as zero
           double off = 0.0;
           off = hasBelow ? off + g.aS * below : off;
-          off = (i < nI - 1) ? off + g.aE * g.xE[m] : off;
-          off = (jr < nJ - 1) ? off + g.aN * g.xN[m] : off;
+          off = (i < nI - 1) ? off + g.aE * east.xC[m] : off;
+          off = (lane < nJ - 1) ? off + g.aN * g.xN[m] : off;
           off = (i > 0) ? off + g.aW * xw[m] : off;
           // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88)
           xn[m] = (fabs(g.aP) > 0.0) ? (g.b[m] - off) / g.aP : g.xC[m];
         }
         if (on) {
-          const int c = jr * nI + i;
 #pragma unroll
           for (int m = 0; m < NC; ++m) {
-            xNew[m][c] = xn[m];
-            if (jr == 0) row0[m * nI + i] = xn[m];
+            if (last) {
+              const double v = S.hasFloor ? mx(xn[m], S.floorValue) : xn[m];
+              S.out[m][lane * nI + i] = v;
+            } else {
+              xNew[m][k * R + lane] = xn[m];
+            }
+            if (lane == 0) row0[m * nI + i] = xn[m];
           }
         }
         // Off the wavefront these are never read: the row above is off it on
-        // the next step too, and this row's next cell has i - 1 < 0 or i >= nI.
+        // the next step too, and this row's next cell has i + 1 <= 0 or >= nI.
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-          buf[(k & 1) * NC * R + m * R + jr] = xn[m];
+          buf[(k & 1) * NC * R + m * R + lane] = xn[m];
           xw[m] = xn[m];
         }
-        gsFetch<NC>(nI, nJ, S, xOld, jr, k + D, ring[q]);
+        gsFetch<NC>(S, xOld, R, steps, lane, k + D, ring[q]);
         ldsBarrier();
       }
     }
     __syncthreads();  // this sweep's writes are the next sweep's old values
   }
-  // an odd number of sweeps leaves the result in tmp
-  const bool inTmp = (sweeps & 1) != 0;
-  const int N = nI * nJ;
-  for (int c = jr; c < N; c += R) {
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      double v = inTmp ? tmp[q][c] : S.x[q][c];
-      if (S.hasFloor) v = mx(v, S.floorValue);
-      S.x[q][c] = v;
-    }
-  }
 }
 
 // Momentum: u and v share one matrix, one workgroup. Turbulence: blockIdx.x
-// 0 sweeps k, 1 sweeps omega (independent systems, run side by side).
+// 0 sweeps k, 1 sweeps omega (independent systems, run side by side). The
+// launch gives the block I.gsR lanes (the layout's stride).
 template <int D>
 __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
-  if (I.flags[fStopped]) return;
-  const GsSys S{I.aP, I.aS, I.aN, I.aW, I.aE, {I.bx, I.by}, {I.u, I.v}, 0.0, 0};
-  double* const tmp[2] = {I.gsTmp0, I.gsTmp1};
-  gsLexBody<2, D>(I, S, tmp, sweeps);
+  if (I.flags[fStopped] || sweeps <= 0) return;
+  double* const* k = I.skM;
+  const GsSkew<2> S{gmem(k[0]), gmem(k[1]), gmem(k[2]), gmem(k[3]), gmem(k[4]),
+                    {gmem(k[5]), gmem(k[6])}, {gmem(k[7]), gmem(k[8])}, {gmem(k[9]), gmem(k[10])},
+                    {gmem(I.u), gmem(I.v)}, 0.0, 0};
+  gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
 }
 
 template <int D>
 __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
-  if (I.flags[fStopped]) return;
-  if (blockIdx.x == 0) {
-    const GsSys S{I.kP, I.kS, I.kN, I.kW, I.kE, {I.kB, nullptr}, {I.k, nullptr}, 0.0, 1};
-    double* const tmp[2] = {I.gsTmp0, nullptr};
-    gsLexBody<1, D>(I, S, tmp, sweeps);
-  } else {
-    const GsSys S{I.wP, I.wS, I.wN, I.wW, I.wE, {I.wB, nullptr}, {I.w, nullptr}, 1.0e-10, 1};
-    double* const tmp[2] = {I.gsTmp1, nullptr};
-    gsLexBody<1, D>(I, S, tmp, sweeps);
-  }
+  if (I.flags[fStopped] || sweeps <= 0) return;
+  const bool isK = blockIdx.x == 0;
+  double* const* k = isK ? I.skK : I.skW;
+  const GsSkew<1> S{gmem(k[0]), gmem(k[1]), gmem(k[2]), gmem(k[3]), gmem(k[4]),
+                    {gmem(k[5])}, {gmem(k[6])}, {gmem(k[7])}, {gmem(isK ? I.k : I.w)},
+                    isK ? 0.0 : 1.0e-10, 1};
+  gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
 }
 
 // Up to 256 rows: one wave per 64 rows, each lane holding a ring of cells
-// fetched ahead (6 for u and v, 8 for k or omega).
+// fetched ahead (4 for u and v, 6 for k or omega).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_lex_mom(const Inst* insts, int sweeps) {
-  gsLexMomentum<6>(insts, sweeps);
+  gsLexMomentum<4>(insts, sweeps);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_lex_turb(const Inst* insts, int sweeps) {
-  gsLexTurbulence<8>(insts, sweeps);
+  gsLexTurbulence<6>(insts, sweeps);
 }
 
-// More than 256 rows (up to 1024): the register budget of a 1024-lane block
-// holds a ring of two cells.
+// More than 256 rows (up to 1024): a ring of two cells.
 __global__ void __launch_bounds__(1024) k_gs_lex_mom_wide(const Inst* insts, int sweeps) {
   gsLexMomentum<2>(insts, sweeps);
 }

@@ -24,7 +24,8 @@ namespace cfdgpu {
   const int nI = I.nI;                                        \
   const int nJ = I.nJ;                                        \
   const int i = live ? c % nI : 0;                            \
-  const int j = live ? c / nI : 0;
+  const int j = live ? c / nI : 0;                            \
+  (void)nJ;
 
 constexpr double kMinOmega = 1.0e-10;
 constexpr double kTinySst = 1.0e-30;
@@ -222,6 +223,18 @@ __device__ __forceinline__ double sstDefect(const Inst& I, const SstRow& r, cons
   return r.b - ax;
 }
 
+// The lexicographic sweep's copy of one row and the current value, in the
+// anti-diagonal layout (slots aP aS aN aW aE b x0).
+__device__ __forceinline__ void skewRow(double* const* sk, int at, const SstRow& r, double phi) {
+  sk[0][at] = r.d;
+  sk[1][at] = r.aS;
+  sk[2][at] = r.aN;
+  sk[3][at] = r.aW;
+  sk[4][at] = r.aE;
+  sk[5][at] = r.b;
+  sk[6][at] = phi;
+}
+
 __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
   SST_PROLOGUE
   __shared__ double sm[16];
@@ -246,6 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
         r.d = relaxed;
       }
       I.kP[c] = r.d; I.kS[c] = r.aS; I.kN[c] = r.aN; I.kW[c] = r.aW; I.kE[c] = r.aE; I.kB[c] = r.b;
+      if (I.gsOrder == 0) skewRow(I.skK, skewIndex(i, j, I.gsR), r, phi);
       const double defect = sstDefect(I, r, I.k, c, i, j);
       part[0] = fabs(defect);
       part[1] = fabs(r.d);
@@ -272,6 +286,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
         r.d = relaxed;
       }
       I.wP[c] = r.d; I.wS[c] = r.aS; I.wN[c] = r.aN; I.wW[c] = r.aW; I.wE[c] = r.aE; I.wB[c] = r.b;
+      if (I.gsOrder == 0) skewRow(I.skW, skewIndex(i, j, I.gsR), r, phi);
       const double defect = sstDefect(I, r, I.w, c, i, j);
       part[2] = fabs(defect);
       part[3] = fabs(r.d);

@@ -107,6 +107,7 @@ Engine::Engine(const Grid& grid, std::vector<int> kind, std::vector<Vec2> bcVelo
   precomputeGeometry();
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   allocate();
+  checkOrdering(settings_);
   uploadStatic();
 }
 
@@ -243,6 +244,10 @@ void Engine::allocate() {
                          &I.fkP, &I.fkS, &I.fkN, &I.fkW, &I.fkE, &I.fkR, &I.fkD, &I.fwP, &I.fwS, &I.fwN,
                          &I.fwW, &I.fwE, &I.fwR, &I.fwD, &I.fLinv, &I.fLc};
     for (float** a : fArrays) *a = cv.take<float>(N);
+    const size_t skew = static_cast<size_t>(skewLength(grid_.nI, grid_.nJ, gsThreadsFor(grid_.nJ)));
+    for (double*& a : I.skM) a = cv.take<double>(skew);
+    for (double*& a : I.skK) a = cv.take<double>(skew);
+    for (double*& a : I.skW) a = cv.take<double>(skew);
     I.red = cv.take<double>(kSlots);
     I.partials = cv.take<double>(static_cast<size_t>(kSlots) * kMaxRedBlocks);
     I.counter = cv.take<unsigned>(16);
@@ -254,7 +259,7 @@ void Engine::allocate() {
   Carver sizing;
   Inst dummy{};
   layout(sizing, dummy);
-  arena_.bytes = sizing.offset + 256;
+  arena_.bytes = sizing.offset;
   check(hipMalloc(&arena_.base, arena_.bytes), "hipMalloc (solver state)");
   check(hipMemsetAsync(arena_.base, 0, arena_.bytes, stream_), "hipMemset");
   Carver real;
@@ -265,12 +270,15 @@ void Engine::allocate() {
   h_.N = grid_.cells();
   h_.nBlocks = (h_.N + kBlock - 1) / kBlock;
   h_.lineBlocks = (h_.nI + 63) / 64;
+  h_.gsR = gsThreadsFor(grid_.nJ);
+  h_.gsSkewLen = skewLength(grid_.nI, grid_.nJ, h_.gsR);
   check(hipMalloc(&dInst_, sizeof(Inst)), "hipMalloc (descriptor)");
   cfg_.maxBlocks = h_.nBlocks;
   cfg_.maxLineBlocks = h_.lineBlocks;
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
   cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
+  if (cfg_.gsThreads != h_.gsR) throw std::logic_error("sweep lanes differ from the skewed layout's stride");
 }
 
 void Engine::uploadStatic() {
@@ -325,7 +333,15 @@ void Engine::uploadInst() {
   check(hipStreamSynchronize(stream_), "descriptor sync");
 }
 
+void Engine::checkOrdering(const cfdgpu_settings& s) const {
+  constexpr size_t kLdsLimit = 64 * 1024;
+  if (s.gs_ordering == CFDGPU_GS_LEXICOGRAPHIC && cfg_.gsLds > kLdsLimit)
+    throw std::invalid_argument("mesh too wide for the lexicographic sweep (row 0 must fit in 64 KiB of LDS); "
+                                "use CFDGPU_GS_REDBLACK");
+}
+
 void Engine::setSettings(const cfdgpu_settings& s) {
+  checkOrdering(s);
   settings_ = s;
   if (initialised_) uploadInst();
 }

@@ -93,6 +93,7 @@ class Engine {
   bool sstInitialise();
   void applyEddyViscosity();
   void fillInstScalars();
+  void checkOrdering(const cfdgpu_settings& s) const;
   void readMonitor(cfdgpu_monitor* out) const;
 
   const Grid& grid_;

@@ -1,0 +1,82 @@
+// Enumerates every index the lexicographic sweep and its writers form in the
+// skewed layout, for a set of mesh shapes, and checks each against its
+// allocation (device arrays of skewLength, row-major fields of nI*nJ, the
+// LDS step buffer, row-0 copy and partner map) and that the cell each fetch
+// reads is the cell the step needs. Mirrors the index expressions of
+// gsFetch / gsLexSkew (simple_fp64.hip) and the skewIndex writers of k_mom and
+// k_sst_asm. Prints "ok" or the first violation.
+#include <cstdio>
+#include <set>
+#include <vector>
+
+#include "../../cfd-simulator_amd/csrc/device/layout.hpp"
+
+using namespace cfdgpu;
+
+static int fail(const char* what, int nI, int nJ, long v, long lim) {
+  std::printf("FAIL %s nI=%d nJ=%d index=%ld limit=%ld\n", what, nI, nJ, v, lim);
+  return 1;
+}
+
+static int check(int nI, int nJ, int D, int NC) {
+  const int R = gsLanes(nJ);
+  if (R < nJ) return fail("lanes < rows", nI, nJ, R, nJ);
+  const long len = skewLength(nI, nJ, R);
+  const int steps = skewSteps(nI, nJ);
+  const long N = static_cast<long>(nI) * nJ;
+  // writers: one slot per cell, all distinct
+  std::set<long> seen;
+  for (int j = 0; j < nJ; ++j)
+    for (int i = 0; i < nI; ++i) {
+      const long s = skewIndex(i, j, R);
+      if (s < 0 || s >= len) return fail("writer", nI, nJ, s, len);
+      if (!seen.insert(s).second) return fail("writer collision", nI, nJ, s, len);
+    }
+  // row-0 copy
+  for (int t = 0; t < nI; ++t)
+    if (skewIndex(t, 0, R) >= len) return fail("row0 load", nI, nJ, skewIndex(t, 0, R), len);
+  const int rings = (steps + D - 1) / D;
+  const int kmax = rings * D - 1 + D;  // last refill
+  for (int k = 0; k <= kmax; ++k)
+    for (int lane = 0; lane < R; ++lane) {
+      const long at = static_cast<long>(k < steps - 1 ? k : steps - 1) * R + lane;
+      const long atN = static_cast<long>(k + 1 < steps - 1 ? k + 1 : steps - 1) * R + lane + 1;
+      if (at < 0 || at >= len) return fail("fetch", nI, nJ, at, len);
+      if (atN < 0 || atN >= len) return fail("fetch north", nI, nJ, atN, len);
+      const int i = k - lane;
+      const bool on = lane < nJ && i >= 0 && i < nI && k < rings * D;
+      // LDS: step buffer, row 0, partner map (partner values lie in [-1, nI))
+      const long bufRead = static_cast<long>(((k - 1) & 1)) * NC * R + (NC - 1) * R + (lane > 0 ? lane - 1 : 0);
+      if (bufRead >= 2L * NC * R) return fail("lds buf read", nI, nJ, bufRead, 2L * NC * R);
+      const long bufWrite = static_cast<long>(k & 1) * NC * R + (NC - 1) * R + lane;
+      if (bufWrite >= 2L * NC * R) return fail("lds buf write", nI, nJ, bufWrite, 2L * NC * R);
+      const int pIdx = i < 0 ? 0 : (i > nI - 1 ? nI - 1 : i);
+      if (pIdx < 0 || pIdx >= nI) return fail("lds partner", nI, nJ, pIdx, nI);
+      if (!on) continue;
+      // the fetched slots hold the cell, the cell above and (next step) the cell to the right
+      if (at != skewIndex(i, lane, R)) return fail("fetch cell", nI, nJ, at, skewIndex(i, lane, R));
+      if (lane + 1 < nJ && atN != skewIndex(i, lane + 1, R)) return fail("fetch north cell", nI, nJ, atN, 0);
+      if (i + 1 < nI) {
+        const long atE = static_cast<long>(k + 1 < steps - 1 ? k + 1 : steps - 1) * R + lane;
+        if (atE != skewIndex(i + 1, lane, R)) return fail("east cell", nI, nJ, atE, 0);
+      }
+      const long store = static_cast<long>(k) * R + lane;
+      if (store != skewIndex(i, lane, R) || store >= len) return fail("store", nI, nJ, store, len);
+      const long out = static_cast<long>(lane) * nI + i;
+      if (out >= N) return fail("final store", nI, nJ, out, N);
+      if (lane == 0 && (NC - 1) * nI + i >= NC * nI) return fail("lds row0 write", nI, nJ, i, nI);
+    }
+  return 0;
+}
+
+int main() {
+  const int shapes[][2] = {{1, 1}, {2, 1}, {1, 5}, {26, 5}, {26, 79}, {118, 27}, {64, 64}, {65, 63},
+                           {256, 64}, {512, 128}, {1024, 256}, {300, 257}, {40, 1000}, {7, 1024}};
+  for (const auto& s : shapes) {
+    const int nI = s[0], nJ = s[1];
+    const int Dm = gsLanes(nJ) <= 256 ? 4 : 2, Dt = gsLanes(nJ) <= 256 ? 6 : 2;
+    if (check(nI, nJ, Dm, 2) || check(nI, nJ, Dt, 1)) return 1;
+  }
+  std::printf("ok\n");
+  return 0;
+}
