@@ -79,50 +79,6 @@ __device__ __forceinline__ void blockMax(double (&v)[NV], double* sm) {
   __syncthreads();
 }
 
-// Cross-block scalars: each kernel writes one partial per block and the
-// kernel that needs the total sums the partials itself, in a fixed order, at
-// its start (the kernel boundary makes them visible). No fences, no atomics:
-// deterministic, and nothing forces the per-XCD L2s to write back mid-kernel.
-// Every thread of the block gets the same value.
-__device__ __forceinline__ double blockSumPartials(const double* p, int n, double* sm) {
-  if (threadIdx.x < 64) {
-    double s = 0.0;
-    for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) s += p[b];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (threadIdx.x == 0) sm[0] = s;
-  }
-  __syncthreads();
-  const double r = sm[0];
-  __syncthreads();
-  return r;
-}
-
-// Sum partials[0..n) of one slot with wave 0 in a fixed order; result in lane 0.
-__device__ __forceinline__ double waveSumPartials(const double* p, int n) {
-  double s = 0.0;
-  for (int b = threadIdx.x; b < n; b += 64) s += p[b];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  return s;
-}
-
-__device__ __forceinline__ double waveMaxPartials(const double* p, int n) {
-  double s = 0.0;
-  for (int b = threadIdx.x; b < n; b += 64) s = mx(s, p[b]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s = mx(s, __shfl_down(s, o, 64));
-  return s;
-}
-
-__device__ __forceinline__ double waveMinPartials(const double* p, int n, double init) {
-  double s = init;
-  for (int b = threadIdx.x; b < n; b += 64) s = mn(s, p[b]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s = mn(s, __shfl_down(s, o, 64));
-  return s;
-}
-
 // Global-address-space views of device pointers. Loads and stores through a
 // generic pointer compile to FLAT instructions, which on CDNA count in
 // lgkmcnt as well as vmcnt: a wait for LDS traffic (lgkmcnt(0)) then also
@@ -136,6 +92,102 @@ __device__ __forceinline__ CFD_GLOBAL T* gmem(T* p) {
 template <class T>
 __device__ __forceinline__ const CFD_GLOBAL T* gmem(const T* p) {
   return (const CFD_GLOBAL T*)(p);
+}
+
+// Cross-block scalars: each kernel writes one partial per block and the
+// kernel that needs the total sums the partials itself, in a fixed order, at
+// its start (the kernel boundary makes them visible). No fences, no atomics:
+// deterministic, and nothing forces the per-XCD L2s to write back mid-kernel.
+//
+// Wave 0 folds p[0..n): lane l combines p[l], p[l + 64], p[l + 128], ... in
+// that order, then a shuffle tree combines the lanes. A lane issues up to
+// kPartialUnroll loads before its first combine, so up to 64 * kPartialUnroll
+// partials cost one memory round trip. `identity` is op's neutral element and
+// stands in for the slots past n.
+constexpr int kPartialUnroll = 16;
+
+struct OpSum {
+  __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct OpMax {
+  __device__ double operator()(double a, double b) const { return mx(a, b); }
+};
+struct OpMin {
+  __device__ double operator()(double a, double b) const { return mn(a, b); }
+};
+
+template <int NS, class Op>
+__device__ __forceinline__ void wavePartials(const double* const (&p)[NS], int n, double identity, Op op,
+                                             double (&out)[NS]) {
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) out[q] = identity;
+  for (int b0 = 0; b0 < n; b0 += 64 * kPartialUnroll) {
+    double t[NS][kPartialUnroll];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const CFD_GLOBAL double* g = gmem(p[q]);
+#pragma unroll
+      for (int u = 0; u < kPartialUnroll; ++u) {
+        const int b = b0 + u * 64 + lane;
+        t[q][u] = (b < n) ? g[b] : identity;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+      for (int u = 0; u < kPartialUnroll; ++u) out[q] = op(out[q], t[q][u]);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) out[q] = op(out[q], __shfl_down(out[q], o, 64));
+  }
+}
+
+// Single-slot forms for wave 0; the result is in lane 0.
+__device__ __forceinline__ double waveSumPartials(const double* p, int n) {
+  const double* const s[1] = {p};
+  double r[1];
+  wavePartials<1>(s, n, 0.0, OpSum{}, r);
+  return r[0];
+}
+__device__ __forceinline__ double waveMaxPartials(const double* p, int n, double identity = 0.0) {
+  const double* const s[1] = {p};
+  double r[1];
+  wavePartials<1>(s, n, identity, OpMax{}, r);
+  return r[0];
+}
+__device__ __forceinline__ double waveMinPartials(const double* p, int n, double identity) {
+  const double* const s[1] = {p};
+  double r[1];
+  wavePartials<1>(s, n, identity, OpMin{}, r);
+  return r[0];
+}
+
+// Sums of NS slots' partials, broadcast to every thread of the block.
+template <int NS>
+__device__ __forceinline__ void blockSumPartials(const double* const (&p)[NS], int n, double* sm,
+                                                 double (&out)[NS]) {
+  if (threadIdx.x < 64) {
+    wavePartials<NS>(p, n, 0.0, OpSum{}, out);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) sm[q] = out[q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NS; ++q) out[q] = sm[q];
+  __syncthreads();
+}
+
+__device__ __forceinline__ double blockSumPartials(const double* p, int n, double* sm) {
+  const double* const s[1] = {p};
+  double r[1];
+  blockSumPartials<1>(s, n, sm, r);
+  return r[0];
 }
 
 __device__ __forceinline__ double* partialSlot(const Inst& I, int slot) {

@@ -820,13 +820,48 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
 // Iteration `it`, launch A. Finishes the test of iteration it-1 from the
 // partials of launch B (|r|_1 <= tol |r0|_1, then the r.z guard), forms beta,
 // then p = z + beta p_old (neighbours' p recomputed locally, so no second
-// launch is needed before the stencil), Ap and p.Ap.
+// launch is needed before the stencil), Ap and p.Ap. The cell's operands do
+// not depend on beta, so they are loaded before the partials are summed and
+// the two memory round trips overlap.
 __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
   CFD_PROLOGUE
   if (I.flags[fCgDone]) return;
   __shared__ double sm[16];
-  const double nrm = blockSumPartials(partialSlot(I, sCgNorm), I.nBlocks, sm);
-  const double rzNew = blockSumPartials(partialSlot(I, sCgRz), I.nBlocks, sm);
+  // operands: z and p_old at the cell and its four neighbours, the matrix row
+  const CFD_GLOBAL double* z = gmem(static_cast<const double*>(I.z));
+  const CFD_GLOBAL double* pOld = gmem(static_cast<const double*>((it & 1) ? I.pd0 : I.pd1));
+  const CFD_GLOBAL double* pFirst = gmem(static_cast<const double*>(I.pd0));
+  const CFD_GLOBAL double* src = (it == 0) ? pFirst : pOld;  // iteration 0 uses p0 = z (in pd0)
+  double zc = 0.0, zS = 0.0, zN = 0.0, zW = 0.0, zE = 0.0;
+  double oc = 0.0, oS = 0.0, oN = 0.0, oW = 0.0, oE = 0.0;
+  double aP = 0.0, aS = 0.0, aN = 0.0, aW = 0.0, aE = 0.0;
+  bool hasS = false;
+  if (live) {
+    const int pr = (j == 0) ? I.partner[i] : -1;
+    hasS = (j > 0) || (pr >= 0);
+    const int cS = (j > 0) ? c - nI : (pr >= 0 ? pr : c);
+    const int cN = (j < nJ - 1) ? c + nI : c;
+    const int cW = (i > 0) ? c - 1 : c;
+    const int cE = (i < nI - 1) ? c + 1 : c;
+    const CFD_GLOBAL double* pP = gmem(static_cast<const double*>(I.pP));
+    const CFD_GLOBAL double* pN = gmem(static_cast<const double*>(I.pN));
+    const CFD_GLOBAL double* pE = gmem(static_cast<const double*>(I.pE));
+    const CFD_GLOBAL double* pWk = gmem(static_cast<const double*>(I.pWk));
+    aP = pP[c];
+    aS = (j > 0) ? pN[c - nI] : pWk[c];
+    aN = pN[c];
+    aW = pE[cW == c ? c : c - 1];
+    aE = pE[c];
+    oc = src[c]; oS = src[cS]; oN = src[cN]; oW = src[cW]; oE = src[cE];
+    if (it > 0) { zc = z[c]; zS = z[cS]; zN = z[cN]; zW = z[cW]; zE = z[cE]; }
+  }
+  double sums[2];
+  {
+    const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
+    blockSumPartials<2>(slots, I.nBlocks, sm, sums);
+  }
+  const double nrm = sums[0];
+  const double rzNew = sums[1];
   double beta = 0.0;
   bool done = false;
   if (it == 0) {
@@ -848,28 +883,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
   }
   double part[1] = {0.0};
   if (live) {
-    double* pNew = (it & 1) ? I.pd1 : I.pd0;
-    const double* pOld = (it & 1) ? I.pd0 : I.pd1;
-    double y;
-    double pc;
-    if (it == 0) {
-      pc = pNew[c];
-      y = spmvPres(I, pNew, c, i, j);
-    } else {
-      auto pv = [&](int q) { return I.z[q] + beta * pOld[q]; };
-      pc = pv(c);
-      y = I.pP[c] * pc;
-      if (j > 0) y += I.pN[c - nI] * pv(c - nI);
-      else {
-        const int pr = I.partner[i];
-        if (pr >= 0) y += I.pWk[c] * pv(pr);
-      }
-      if (j < nJ - 1) y += I.pN[c] * pv(c + nI);
-      if (i > 0) y += I.pE[c - 1] * pv(c - 1);
-      if (i < nI - 1) y += I.pE[c] * pv(c + 1);
-      pNew[c] = pc;
-    }
-    I.Ap[c] = y;
+    // p = z + beta p_old (LinearSystem.cpp:199-201); iteration 0 has p = z
+    auto pv = [&](double zq, double oq) { return (it == 0) ? oq : zq + beta * oq; };
+    const double pc = pv(zc, oc);
+    // LinearSystem::multiply order: diagonal, then the faces of the cell
+    // (below/wake, above, left, right as the structured sweep visits them)
+    double y = aP * pc;
+    if (hasS) y += aS * pv(zS, oS);
+    if (j < nJ - 1) y += aN * pv(zN, oN);
+    if (i > 0) y += aW * pv(zW, oW);
+    if (i < nI - 1) y += aE * pv(zE, oE);
+    if (it > 0) gmem((it & 1) ? I.pd1 : I.pd0)[c] = pc;
+    gmem(I.Ap)[c] = y;
     part[0] = pc * y;
   }
   blockSum<1>(part, sm);
@@ -877,11 +902,20 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
 }
 
 // Iteration `it`, launch B: alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap,
-// z = r / diag, and the partials of |r|_1 and r.z for the next launch A.
+// z = r / diag, and the partials of |r|_1 and r.z for the next launch A. The
+// cell's operands are loaded before the p.Ap partials are summed.
 __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
   CFD_PROLOGUE
   if (I.flags[fCgDone]) return;
   __shared__ double sm[16];
+  double xc = 0.0, pc = 0.0, rc = 0.0, apc = 0.0, d = 0.0;
+  if (live) {
+    xc = gmem(static_cast<const double*>(I.x))[c];
+    pc = gmem(static_cast<const double*>((it & 1) ? I.pd1 : I.pd0))[c];
+    rc = gmem(static_cast<const double*>(I.r))[c];
+    apc = gmem(static_cast<const double*>(I.Ap))[c];
+    d = gmem(static_cast<const double*>(I.pP))[c];
+  }
   const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.nBlocks, sm);
   if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
     if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
@@ -890,14 +924,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
   const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
   double part[2] = {0.0, 0.0};
   if (live) {
-    const double* p = (it & 1) ? I.pd1 : I.pd0;
-    const double x = I.x[c] + alpha * p[c];
-    const double r = I.r[c] - alpha * I.Ap[c];
-    I.x[c] = x;
-    I.r[c] = r;
-    const double d = I.pP[c];
+    const double x = xc + alpha * pc;
+    const double r = rc - alpha * apc;
+    gmem(I.x)[c] = x;
+    gmem(I.r)[c] = r;
     const double z = (fabs(d) > 0.0) ? r / d : r;
-    I.z[c] = z;
+    gmem(I.z)[c] = z;
     part[0] = fabs(r);
     part[1] = r * z;
   }
@@ -1067,22 +1099,6 @@ __global__ void __launch_bounds__(kBlock) k_divergence(const Inst* insts, double
 // One thread per instance.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double waveMaxOf(const double* p, int n, double init) {
-  double s = init;
-  for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) s = mx(s, p[b]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s = mx(s, __shfl_down(s, o, 64));
-  return s;
-}
-
-__device__ __forceinline__ double waveMinOf(const double* p, int n, double init) {
-  double s = init;
-  for (int b = static_cast<int>(threadIdx.x); b < n; b += 64) s = mn(s, p[b]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s = mn(s, __shfl_down(s, o, 64));
-  return s;
-}
-
 // One 64-lane block per instance: finishes the reductions of the iteration's
 // kernels from their per-block partials, writes the monitor and applies the
 // stop rule.
@@ -1090,26 +1106,32 @@ __global__ void __launch_bounds__(64) k_monitor(const Inst* insts) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped]) return;
   const int n = I.nBlocks;
-  const double vscale = waveMaxOf(partialSlot(I, sVelScale), n, 0.0);
-  const double resX = waveSumPartials(partialSlot(I, sMomResX), n);
-  const double resY = waveSumPartials(partialSlot(I, sMomResY), n);
-  const double diagL1 = waveSumPartials(partialSlot(I, sMomDiagL1), n);
-  const double net = waveSumPartials(partialSlot(I, sNetFluxL1), n);
-  const double maxDiv = waveMaxOf(partialSlot(I, sMaxDiv), n, 0.0);
+  double sums[4], maxs[2];
+  {
+    const double* const s4[4] = {partialSlot(I, sMomResX), partialSlot(I, sMomResY), partialSlot(I, sMomDiagL1),
+                                 partialSlot(I, sNetFluxL1)};
+    wavePartials<4>(s4, n, 0.0, OpSum{}, sums);
+    const double* const m2[2] = {partialSlot(I, sVelScale), partialSlot(I, sMaxDiv)};
+    wavePartials<2>(m2, n, 0.0, OpMax{}, maxs);
+  }
+  const double resX = sums[0], resY = sums[1], diagL1 = sums[2], net = sums[3];
+  const double vscale = maxs[0], maxDiv = maxs[1];
   double tk = 0.0, tkd = 0.0, tw = 0.0, twd = 0.0, mxk = 0.0, mxw = 0.0, mxm = 0.0, mxr = 0.0, mnk = 0.0, mnw = 0.0,
          mnm = 0.0;
   if (I.model == 2) {
-    tk = waveSumPartials(partialSlot(I, sTurbResK), n);
-    tkd = waveSumPartials(partialSlot(I, sTurbDiagK), n);
-    tw = waveSumPartials(partialSlot(I, sTurbResW), n);
-    twd = waveSumPartials(partialSlot(I, sTurbDiagW), n);
-    mxk = waveMaxOf(partialSlot(I, sMaxK), n, -INFINITY);
-    mxw = waveMaxOf(partialSlot(I, sMaxW), n, -INFINITY);
-    mxm = waveMaxOf(partialSlot(I, sMaxMut), n, -INFINITY);
-    mxr = waveMaxOf(partialSlot(I, sMaxEddyRatio), n, 0.0);
-    mnk = waveMinOf(partialSlot(I, sMinK), n, INFINITY);
-    mnw = waveMinOf(partialSlot(I, sMinW), n, INFINITY);
-    mnm = waveMinOf(partialSlot(I, sMinMut), n, INFINITY);
+    double t4[4], hi[3], lo[3], er[1];
+    const double* const s4[4] = {partialSlot(I, sTurbResK), partialSlot(I, sTurbDiagK), partialSlot(I, sTurbResW),
+                                 partialSlot(I, sTurbDiagW)};
+    wavePartials<4>(s4, n, 0.0, OpSum{}, t4);
+    const double* const h3[3] = {partialSlot(I, sMaxK), partialSlot(I, sMaxW), partialSlot(I, sMaxMut)};
+    wavePartials<3>(h3, n, -INFINITY, OpMax{}, hi);
+    const double* const e1[1] = {partialSlot(I, sMaxEddyRatio)};
+    wavePartials<1>(e1, n, 0.0, OpMax{}, er);
+    const double* const l3[3] = {partialSlot(I, sMinK), partialSlot(I, sMinW), partialSlot(I, sMinMut)};
+    wavePartials<3>(l3, n, INFINITY, OpMin{}, lo);
+    tk = t4[0]; tkd = t4[1]; tw = t4[2]; twd = t4[3];
+    mxk = hi[0]; mxw = hi[1]; mxm = hi[2]; mxr = er[0];
+    mnk = lo[0]; mnw = lo[1]; mnm = lo[2];
   }
   if (threadIdx.x != 0) return;
   constexpr double kTiny = 1e-300;
