@@ -94,6 +94,13 @@ __device__ __forceinline__ const CFD_GLOBAL T* gmem(const T* p) {
   return (const CFD_GLOBAL T*)(p);
 }
 
+// A pair of doubles as one 16-byte vector: loads and stores of it through a
+// global pointer are single dwordx4 accesses. (HIP's double2 wrapper cannot be
+// copied out of an address-space-qualified location.)
+typedef double dpair __attribute__((vector_size(16)));
+__device__ __forceinline__ const CFD_GLOBAL dpair* gpair(const double2* p) { return (const CFD_GLOBAL dpair*)(p); }
+__device__ __forceinline__ CFD_GLOBAL dpair* gpair(double2* p) { return (CFD_GLOBAL dpair*)(p); }
+
 // Cross-block scalars: each kernel writes one partial per block and the
 // kernel that needs the total sums the partials itself, in a fixed order, at
 // its start (the kernel boundary makes them visible). No fences, no atomics:

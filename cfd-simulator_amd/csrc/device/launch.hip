@@ -15,6 +15,7 @@ __global__ void k_gs_lex_mom(const Inst*, int);
 __global__ void k_gs_lex_turb(const Inst*, int);
 __global__ void k_gs_lex_mom_wide(const Inst*, int);
 __global__ void k_gs_lex_turb_wide(const Inst*, int);
+__global__ void k_gs_unskew(const Inst*, int, int);
 __global__ void k_gs_color(const Inst*, int, int, int);
 __global__ void k_rhiechow(const Inst*);
 __global__ void k_pressure(const Inst*, int);
@@ -98,6 +99,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   } else if (lex) {
     if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_mom, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
     else hipLaunchKernelGGL(k_gs_lex_mom_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 0, h.momSweeps);
   } else {
     for (int sw = 0; sw < h.momSweeps; ++sw) {
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 0, 0);
@@ -154,6 +156,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     } else if (lex) {
       if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
       else hipLaunchKernelGGL(k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 1, 3);
     } else {
       for (int sw = 0; sw < 3; ++sw) {
         const int floor = (sw == 2) ? 1 : 0;

@@ -98,8 +98,6 @@ constexpr int kMaxRedBlocks = 4096;  // partial-sum capacity per slot
 // Skewed layout of the lexicographic sweep. Every index a sweep forms is
 // (k * R + lane) or ((k + 1) * R + lane + 1) with the step k clamped to
 // [0, steps - 1] and lane < R, so it stays below skewLength.
-constexpr int kSkMom = 11;
-constexpr int kSkTurb = 8;
 // Lanes of the sweep for nJ rows: one per row, rounded up to whole waves.
 __host__ __device__ constexpr int gsLanes(int nJ) {
   return nJ <= 64 ? 64 : (nJ >= 1024 ? 1024 : ((nJ + 63) / 64) * 64);
@@ -161,9 +159,15 @@ struct Inst {
   // writes the row-major field.
   int gsR;        // lanes of the sweep (rows rounded up to 64) = stride of the layout
   int gsSkewLen;  // elements of each skewed array: skewLength(nI, nJ, gsR)
-  double* skM[kSkMom];  // momentum: aP aS aN aW aE bx by u0 v0 u1 v1
-  double* skK[kSkTurb]; // k: aP aS aN aW aE b x0 x1
-  double* skW[kSkTurb]; // omega: aP aS aN aW aE b x0 x1
+  // Matrix rows in pairs, so a lane fetches a row with three 16-byte loads:
+  // row[0] = (aP, aS), row[1] = (aN, aW), row[2] = (aE, b); momentum keeps
+  // b_y in its own array. Unknowns in pairs too: (u, v) for momentum, (phi, -)
+  // for k and omega; x[0] holds the starting values, x[1] the ping-pong copy.
+  double2* skMomRow[3];
+  double* skMomBy;
+  double2* skMomX[2];
+  double2* skTurbRow[2][3];  // [k, omega][pair]
+  double2* skTurbX[2][2];    // [k, omega][copy]
 
   // k-omega SST state and systems
   double *k, *w, *mut, *f1, *f2, *cd, *strain, *prod;

@@ -262,17 +262,13 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     I.bx[c] = row.bx;
     I.by[c] = row.by;
     if (I.gsOrder == 0) {
-      // the lexicographic sweep's copy, anti-diagonal layout
+      // the lexicographic sweep's copy, anti-diagonal layout, in pairs
       const int sk = skewIndex(i, j, I.gsR);
-      I.skM[0][sk] = d;
-      I.skM[1][sk] = aS;
-      I.skM[2][sk] = aN;
-      I.skM[3][sk] = aW;
-      I.skM[4][sk] = aE;
-      I.skM[5][sk] = row.bx;
-      I.skM[6][sk] = row.by;
-      I.skM[7][sk] = uc;
-      I.skM[8][sk] = vc;
+      I.skMomRow[0][sk] = make_double2(d, aS);
+      I.skMomRow[1][sk] = make_double2(aN, aW);
+      I.skMomRow[2][sk] = make_double2(aE, row.bx);
+      I.skMomBy[sk] = row.by;
+      I.skMomX[0][sk] = make_double2(uc, vc);
     }
     // residual with the starting field (LinearSystem.cpp:18-54, face order)
     double axu = d * uc, axv = d * vc;
@@ -329,51 +325,63 @@ struct GsSys {
 };
 
 // One system of the lexicographic sweep in the skewed layout (see Inst::gsR):
-// matrix rows and right-hand sides, the two ping-pong copies of the unknowns
-// (x0 holds the starting values, written by the assembly kernel) and the
-// row-major field that the last sweep writes. All pointers are global-space
-// (see gmem), so the per-step LDS barrier never waits for HBM traffic.
+// matrix rows as three pairs (aP, aS), (aN, aW), (aE, b[0]) plus b[1] for
+// momentum, and the two ping-pong copies of the unknowns as pairs (x[0] holds
+// the starting values, written by the assembly kernel; after n sweeps the
+// result is in x[n & 1], which k_gs_unskew copies to the field). All pointers
+// are global-space (see gmem), so the per-step LDS barrier never waits for
+// HBM traffic.
 template <int NC>
 struct GsSkew {
-  const CFD_GLOBAL double *aP, *aS, *aN, *aW, *aE;
-  const CFD_GLOBAL double* b[NC];
-  CFD_GLOBAL double* x0[NC];
-  CFD_GLOBAL double* x1[NC];
-  CFD_GLOBAL double* out[NC];
-  double floorValue;  // applied to the values the last sweep writes, when hasFloor
-  int hasFloor;
+  const CFD_GLOBAL dpair* row[3];
+  const CFD_GLOBAL double* b1;  // second right-hand side (NC == 2)
+  CFD_GLOBAL dpair* x[2];
 };
 
 // What a lane needs for its cell on one wavefront step that does not depend
 // on this sweep's updates: the matrix row, the right-hand side, and the old
-// values of the cell and of the cell above. The old value of the cell to the
-// right is the next step's xC, so it is not fetched separately.
+// values of the cell, the cell above and the cell to the right. (The last is
+// also the next step's xC; fetching it again keeps each ring slot
+// self-contained, so the wait for a slot never covers the slot after it.)
 template <int NC>
 struct GsCell {
   double aP, aS, aN, aW, aE;
-  double b[NC], xC[NC], xN[NC];
+  double b[NC], xC[NC], xN[NC], xE[NC];
 };
 
-// Fetch lane's cell on step k. Loads are unconditional with the step clamped
+// Fetch lane's cell on step k: three (NC = 1) or four (NC = 2) loads of the
+// row and three of the unknowns. Loads are unconditional with the step clamped
 // into [0, steps - 1], so every lane issues the same loads on every step and
 // the compiler can keep the newer ones in flight while it waits for the
 // oldest; values of cells off the wavefront are never used. Indices are
-// k * R + lane and (k + 1) * R + lane + 1, below skewLength(nI, nJ, R).
+// k * R + lane, (k + 1) * R + lane and (k + 1) * R + lane + 1, below
+// skewLength(nI, nJ, R).
 template <int NC>
-__device__ __forceinline__ void gsFetch(const GsSkew<NC>& S, const CFD_GLOBAL double* const* xOld, int R,
-                                        int steps, int lane, int k, GsCell<NC>& g) {
+__device__ __forceinline__ void gsFetch(const GsSkew<NC>& S, const CFD_GLOBAL dpair* xOld, int R, int steps,
+                                        int lane, int k, GsCell<NC>& g) {
   const int at = min(k, steps - 1) * R + lane;
-  const int atN = min(k + 1, steps - 1) * R + lane + 1;
-  g.aP = S.aP[at];
-  g.aS = S.aS[at];
-  g.aN = S.aN[at];
-  g.aW = S.aW[at];
-  g.aE = S.aE[at];
-#pragma unroll
-  for (int q = 0; q < NC; ++q) {
-    g.b[q] = S.b[q][at];
-    g.xC[q] = xOld[q][at];
-    g.xN[q] = xOld[q][atN];
+  const int atE = min(k + 1, steps - 1) * R + lane;
+  const int atN = atE + 1;
+  const dpair r0 = S.row[0][at];
+  const dpair r1 = S.row[1][at];
+  const dpair r2 = S.row[2][at];
+  g.aP = r0[0];
+  g.aS = r0[1];
+  g.aN = r1[0];
+  g.aW = r1[1];
+  g.aE = r2[0];
+  g.b[0] = r2[1];
+  if constexpr (NC == 2) g.b[1] = S.b1[at];
+  const dpair c = xOld[at];
+  const dpair e = xOld[atE];
+  const dpair n = xOld[atN];
+  g.xC[0] = c[0];
+  g.xE[0] = e[0];
+  g.xN[0] = n[0];
+  if constexpr (NC == 2) {
+    g.xC[1] = c[1];
+    g.xE[1] = e[1];
+    g.xN[1] = n[1];
   }
 }
 
@@ -395,11 +403,9 @@ __device__ __forceinline__ void ldsBarrier() {
 // partner's current value from an LDS copy of the row, updated in place, so
 // the partner coupling follows the reference's order for any partner map.
 // D is the read-ahead: a ring of D cells per lane, slot q holding step
-// r * D + q and refilled with the step D ahead right after use (D >= 2, the
-// next step's cell supplies the east neighbour).
+// r * D + q and refilled with the step D ahead right after use.
 template <int NC, int D>
 __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSkew<NC>& S, int sweeps) {
-  static_assert(D >= 2, "the ring must hold the next step");
   HIP_DYNAMIC_SHARED(double, lds)
   double* buf = lds;                                        // [2][NC][R]: value just written by each row
   double* row0 = lds + 2 * NC * R;                          // [NC][nI]: row j = 0, updated in place
@@ -413,17 +419,13 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
   const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
   for (int t = lane; t < nI; t += R) partner[t] = partnerSrc[t];
   for (int sweep = 0; sweep < sweeps; ++sweep) {
-    const bool last = sweep == sweeps - 1;
-    const CFD_GLOBAL double* xOld[NC];
-    CFD_GLOBAL double* xNew[NC];
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      xOld[q] = (sweep & 1) ? S.x1[q] : S.x0[q];
-      xNew[q] = (sweep & 1) ? S.x0[q] : S.x1[q];
-    }
+    // selects, not S.x[sweep & 1]: a run-time index would put S in scratch
+    const CFD_GLOBAL dpair* xOld = (sweep & 1) ? S.x[1] : S.x[0];
+    CFD_GLOBAL dpair* xNew = (sweep & 1) ? S.x[0] : S.x[1];
     for (int t = lane; t < nI; t += R) {
-#pragma unroll
-      for (int q = 0; q < NC; ++q) row0[q * nI + t] = xOld[q][skewIndex(t, 0, R)];
+      const dpair v = xOld[skewIndex(t, 0, R)];
+      row0[t] = v[0];
+      if constexpr (NC == 2) row0[nI + t] = v[1];
     }
     __syncthreads();
     double xw[NC];
@@ -439,7 +441,6 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
         const int i = k - lane;
         const bool on = active && i >= 0 && i < nI;
         const GsCell<NC>& g = ring[q];
-        const GsCell<NC>& east = ring[(q + 1) % D];
         const int pr = partner[min(max(i, 0), nI - 1)];
         const bool hasBelow = (lane > 0) || (pr >= 0);
         double xn[NC];
@@ -451,23 +452,24 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
           // a missing neighbour's term is skipped, not added as zero
           double off = 0.0;
           off = hasBelow ? off + g.aS * below : off;
-          off = (i < nI - 1) ? off + g.aE * east.xC[m] : off;
+          off = (i < nI - 1) ? off + g.aE * g.xE[m] : off;
           off = (lane < nJ - 1) ? off + g.aN * g.xN[m] : off;
           off = (i > 0) ? off + g.aW * xw[m] : off;
           // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88)
           xn[m] = (fabs(g.aP) > 0.0) ? (g.b[m] - off) / g.aP : g.xC[m];
         }
-        if (on) {
+        {
+          // Every lane stores on every step, so the number of memory operations
+          // in flight is the same on every path and the compiler's waits stay
+          // partial. Lanes off the wavefront write the padding row past the
+          // last step, which no fetch reads.
+          const int at = on ? k * R + lane : (steps + 1) * R + lane;
+          const dpair v = {xn[0], NC == 2 ? xn[NC - 1] : 0.0};
+          xNew[at] = v;
+        }
+        if (on && lane == 0) {
 #pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            if (last) {
-              const double v = S.hasFloor ? mx(xn[m], S.floorValue) : xn[m];
-              S.out[m][lane * nI + i] = v;
-            } else {
-              xNew[m][k * R + lane] = xn[m];
-            }
-            if (lane == 0) row0[m * nI + i] = xn[m];
-          }
+          for (int m = 0; m < NC; ++m) row0[m * nI + i] = xn[m];
         }
         // Off the wavefront these are never read: the row above is off it on
         // the next step too, and this row's next cell has i + 1 <= 0 or >= nI.
@@ -491,10 +493,10 @@ template <int D>
 __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped] || sweeps <= 0) return;
-  double* const* k = I.skM;
-  const GsSkew<2> S{gmem(k[0]), gmem(k[1]), gmem(k[2]), gmem(k[3]), gmem(k[4]),
-                    {gmem(k[5]), gmem(k[6])}, {gmem(k[7]), gmem(k[8])}, {gmem(k[9]), gmem(k[10])},
-                    {gmem(I.u), gmem(I.v)}, 0.0, 0};
+  const GsSkew<2> S{{gpair(static_cast<const double2*>(I.skMomRow[0])), gpair(static_cast<const double2*>(I.skMomRow[1])),
+                     gpair(static_cast<const double2*>(I.skMomRow[2]))},
+                    gmem(static_cast<const double*>(I.skMomBy)),
+                    {gpair(I.skMomX[0]), gpair(I.skMomX[1])}};
   gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
 }
 
@@ -502,11 +504,12 @@ template <int D>
 __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped] || sweeps <= 0) return;
-  const bool isK = blockIdx.x == 0;
-  double* const* k = isK ? I.skK : I.skW;
-  const GsSkew<1> S{gmem(k[0]), gmem(k[1]), gmem(k[2]), gmem(k[3]), gmem(k[4]),
-                    {gmem(k[5])}, {gmem(k[6])}, {gmem(k[7])}, {gmem(isK ? I.k : I.w)},
-                    isK ? 0.0 : 1.0e-10, 1};
+  const int sys = (blockIdx.x == 0) ? 0 : 1;  // 0: k, 1: omega
+  double2* const* row = I.skTurbRow[sys];
+  const GsSkew<1> S{{gpair(static_cast<const double2*>(row[0])), gpair(static_cast<const double2*>(row[1])),
+                     gpair(static_cast<const double2*>(row[2]))},
+                    nullptr,
+                    {gpair(I.skTurbX[sys][0]), gpair(I.skTurbX[sys][1])}};
   gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
 }
 
@@ -519,6 +522,26 @@ k_gs_lex_mom(const Inst* insts, int sweeps) {
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_lex_turb(const Inst* insts, int sweeps) {
   gsLexTurbulence<6>(insts, sweeps);
+}
+
+// After `sweeps` sweeps the result is in the skewed copy x[sweeps & 1]:
+// copy it to the row-major field. System 0: u and v; system 1: k and omega
+// with their floors (KOmegaSST.cpp:468-471).
+__global__ void __launch_bounds__(kBlock) k_gs_unskew(const Inst* insts, int system, int sweeps) {
+  CFD_PROLOGUE
+  if (!live || sweeps <= 0) return;
+  const int sk = skewIndex(i, j, I.gsR);
+  const int src = sweeps & 1;
+  if (system == 0) {
+    const dpair v = gpair(static_cast<const double2*>(I.skMomX[src]))[sk];
+    I.u[c] = v[0];
+    I.v[c] = v[1];
+  } else {
+    const dpair k = gpair(static_cast<const double2*>(I.skTurbX[0][src]))[sk];
+    const dpair w = gpair(static_cast<const double2*>(I.skTurbX[1][src]))[sk];
+    I.k[c] = mx(k[0], 0.0);
+    I.w[c] = mx(w[0], 1.0e-10);
+  }
 }
 
 // More than 256 rows (up to 1024): a ring of two cells.

@@ -224,15 +224,12 @@ __device__ __forceinline__ double sstDefect(const Inst& I, const SstRow& r, cons
 }
 
 // The lexicographic sweep's copy of one row and the current value, in the
-// anti-diagonal layout (slots aP aS aN aW aE b x0).
-__device__ __forceinline__ void skewRow(double* const* sk, int at, const SstRow& r, double phi) {
-  sk[0][at] = r.d;
-  sk[1][at] = r.aS;
-  sk[2][at] = r.aN;
-  sk[3][at] = r.aW;
-  sk[4][at] = r.aE;
-  sk[5][at] = r.b;
-  sk[6][at] = phi;
+// anti-diagonal layout: pairs (aP, aS), (aN, aW), (aE, b) and (phi, -).
+__device__ __forceinline__ void skewRow(const Inst& I, int sys, int at, const SstRow& r, double phi) {
+  I.skTurbRow[sys][0][at] = make_double2(r.d, r.aS);
+  I.skTurbRow[sys][1][at] = make_double2(r.aN, r.aW);
+  I.skTurbRow[sys][2][at] = make_double2(r.aE, r.b);
+  I.skTurbX[sys][0][at] = make_double2(phi, 0.0);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
@@ -259,7 +256,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
         r.d = relaxed;
       }
       I.kP[c] = r.d; I.kS[c] = r.aS; I.kN[c] = r.aN; I.kW[c] = r.aW; I.kE[c] = r.aE; I.kB[c] = r.b;
-      if (I.gsOrder == 0) skewRow(I.skK, skewIndex(i, j, I.gsR), r, phi);
+      if (I.gsOrder == 0) skewRow(I, 0, skewIndex(i, j, I.gsR), r, phi);
       const double defect = sstDefect(I, r, I.k, c, i, j);
       part[0] = fabs(defect);
       part[1] = fabs(r.d);
@@ -286,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
         r.d = relaxed;
       }
       I.wP[c] = r.d; I.wS[c] = r.aS; I.wN[c] = r.aN; I.wW[c] = r.aW; I.wE[c] = r.aE; I.wB[c] = r.b;
-      if (I.gsOrder == 0) skewRow(I.skW, skewIndex(i, j, I.gsR), r, phi);
+      if (I.gsOrder == 0) skewRow(I, 1, skewIndex(i, j, I.gsR), r, phi);
       const double defect = sstDefect(I, r, I.w, c, i, j);
       part[2] = fabs(defect);
       part[3] = fabs(r.d);

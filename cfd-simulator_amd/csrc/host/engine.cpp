@@ -245,9 +245,13 @@ void Engine::allocate() {
                          &I.fwW, &I.fwE, &I.fwR, &I.fwD, &I.fLinv, &I.fLc};
     for (float** a : fArrays) *a = cv.take<float>(N);
     const size_t skew = static_cast<size_t>(skewLength(grid_.nI, grid_.nJ, gsThreadsFor(grid_.nJ)));
-    for (double*& a : I.skM) a = cv.take<double>(skew);
-    for (double*& a : I.skK) a = cv.take<double>(skew);
-    for (double*& a : I.skW) a = cv.take<double>(skew);
+    for (double2*& a : I.skMomRow) a = cv.take<double2>(skew);
+    I.skMomBy = cv.take<double>(skew);
+    for (double2*& a : I.skMomX) a = cv.take<double2>(skew);
+    for (auto& sys : I.skTurbRow)
+      for (double2*& a : sys) a = cv.take<double2>(skew);
+    for (auto& sys : I.skTurbX)
+      for (double2*& a : sys) a = cv.take<double2>(skew);
     I.red = cv.take<double>(kSlots);
     I.partials = cv.take<double>(static_cast<size_t>(kSlots) * kMaxRedBlocks);
     I.counter = cv.take<unsigned>(16);

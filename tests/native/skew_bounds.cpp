@@ -43,6 +43,8 @@ static int check(int nI, int nJ, int D, int NC) {
       const long atN = static_cast<long>(k + 1 < steps - 1 ? k + 1 : steps - 1) * R + lane + 1;
       if (at < 0 || at >= len) return fail("fetch", nI, nJ, at, len);
       if (atN < 0 || atN >= len) return fail("fetch north", nI, nJ, atN, len);
+      const long atE0 = atN - 1;
+      if (atE0 < 0 || atE0 >= len) return fail("fetch east", nI, nJ, atE0, len);
       const int i = k - lane;
       const bool on = lane < nJ && i >= 0 && i < nI && k < rings * D;
       // LDS: step buffer, row 0, partner map (partner values lie in [-1, nI))
@@ -52,6 +54,10 @@ static int check(int nI, int nJ, int D, int NC) {
       if (bufWrite >= 2L * NC * R) return fail("lds buf write", nI, nJ, bufWrite, 2L * NC * R);
       const int pIdx = i < 0 ? 0 : (i > nI - 1 ? nI - 1 : i);
       if (pIdx < 0 || pIdx >= nI) return fail("lds partner", nI, nJ, pIdx, nI);
+      // every lane stores every step: on the wavefront to its cell, off it
+      // to the padding row past the last step
+      const long sink = static_cast<long>(steps + 1) * R + lane;
+      if (sink >= len || sink < static_cast<long>(steps + 1) * R) return fail("sink store", nI, nJ, sink, len);
       if (!on) continue;
       // the fetched slots hold the cell, the cell above and (next step) the cell to the right
       if (at != skewIndex(i, lane, R)) return fail("fetch cell", nI, nJ, at, skewIndex(i, lane, R));
@@ -74,7 +80,7 @@ int main() {
                            {256, 64}, {512, 128}, {1024, 256}, {300, 257}, {40, 1000}, {7, 1024}};
   for (const auto& s : shapes) {
     const int nI = s[0], nJ = s[1];
-    const int Dm = gsLanes(nJ) <= 256 ? 4 : 2, Dt = gsLanes(nJ) <= 256 ? 6 : 2;
+    const int Dm = gsLanes(nJ) <= 256 ? 6 : 2, Dt = gsLanes(nJ) <= 256 ? 8 : 2;
     if (check(nI, nJ, Dm, 2) || check(nI, nJ, Dt, 1)) return 1;
   }
   std::printf("ok\n");
