@@ -181,8 +181,11 @@ def run_bench(args):
 
 
 def run_converge(args):
-    """Run the solve until worst() < tol, printing Cl/Cd/Cm every chunk, then
-    report the first iteration after which |dCl| < 1e-3 and |dCd| < 1e-4 hold."""
+    """Run the solve for --max-iterations (or until worst() < tol with --stop-at-tolerance),
+    printing the residual and Cl/Cd/Cm every chunk. Reports the first chunk whose
+    residual is below the tolerance (the reference's stop rule) and the first
+    iteration after which |Cl - Cl_end| < 1e-3 and |Cd - Cd_end| < 1e-4 hold for
+    every later chunk, with Cl_end, Cd_end the coefficients of the last chunk."""
     import cfdgpu as G
 
     mesh, kind, vel, pres, field = build_case(G)
@@ -190,6 +193,7 @@ def run_converge(args):
     hist = []
     done = 0
     t0 = time.perf_counter()
+    at_tol = None
     while done < args.max_iterations:
         mon = s.iterate(args.chunk)
         done += args.chunk
@@ -197,8 +201,10 @@ def run_converge(args):
         hist.append((done, mon.worst(), f.cl, f.cd, f.cm))
         print(f"{done} {time.perf_counter() - t0:8.1f}s worst={mon.worst():.3e} cl={f.cl:.8f} cd={f.cd:.8f} "
               f"cm={f.cm:.8f} eddy={mon.max_eddy_ratio:.1f}", flush=True)
-        if mon.worst() < args.tolerance:
-            break
+        if at_tol is None and mon.worst() < args.tolerance:
+            at_tol = done
+            if args.stop_at_tolerance:
+                break
     cl_f, cd_f = hist[-1][2], hist[-1][3]
     n_conv = None
     for it, _, cl, cd, _ in reversed(hist):
@@ -207,7 +213,26 @@ def run_converge(args):
         else:
             break
     print(json.dumps({"iterations": done, "final_worst": hist[-1][1], "cl": cl_f, "cd": cd_f,
-                      "cm": hist[-1][4], "iterations_to_converge": n_conv, "chunk": args.chunk}), flush=True)
+                      "cm": hist[-1][4], "iterations_to_tolerance": at_tol, "tolerance": args.tolerance,
+                      "iterations_to_coefficients": n_conv, "chunk": args.chunk,
+                      "seconds": time.perf_counter() - t0}), flush=True)
+
+
+def run_solve(args):
+    """One C2 solve from the uniform field to the reference's stop rule
+    (residuals.worst() < tolerance, checked after every outer iteration on the
+    device), timed end to end on the GPU."""
+    import cfdgpu as G
+
+    mesh, kind, vel, pres, field = build_case(G)
+    s = gpu_solver(G, mesh, kind, vel, pres, field)
+    t0 = time.perf_counter()
+    rep = s.solve(args.tolerance, args.max_iterations, check_every=args.check_every)
+    sec = time.perf_counter() - t0
+    f = coefficients(G, mesh, s)
+    print(json.dumps({"iterations": rep.iterations, "status": rep.status, "seconds": sec,
+                      "ms_per_iteration": 1000.0 * sec / max(rep.iterations, 1), "worst": rep.last.worst(),
+                      "tolerance": args.tolerance, "cl": f.cl, "cd": f.cd, "cm": f.cm}), flush=True)
 
 
 def main():
@@ -221,12 +246,17 @@ def main():
     ap.add_argument("--chunk", type=int, default=250)
     ap.add_argument("--max-iterations", type=int, default=40000)
     ap.add_argument("--tolerance", type=float, default=1e-7)
+    ap.add_argument("--stop-at-tolerance", action="store_true")
+    ap.add_argument("--solve", action="store_true", help="one solve to the residual tolerance, timed")
+    ap.add_argument("--check-every", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
               "by instance, one solve per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    if args.converge:
+    if args.solve:
+        run_solve(args)
+    elif args.converge:
         run_converge(args)
     else:
         run_bench(args)

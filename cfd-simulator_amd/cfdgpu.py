@@ -34,7 +34,8 @@ EXPORTS = [
     "cfdgpu_mesh_dims", "cfdgpu_mesh_nodes", "cfdgpu_mesh_cells", "cfdgpu_mesh_faces",
     "cfdgpu_mesh_quality", "cfdgpu_face_conditions", "cfdgpu_uniform_field", "cfdgpu_default_settings",
     "cfdgpu_default_sst", "cfdgpu_create", "cfdgpu_destroy", "cfdgpu_set_turbulence", "cfdgpu_initialise",
-    "cfdgpu_iterate", "cfdgpu_iterate_timed", "cfdgpu_solve", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
+    "cfdgpu_iterate", "cfdgpu_iterate_timed", "cfdgpu_solve", "cfdgpu_batch_create", "cfdgpu_batch_destroy",
+    "cfdgpu_batch_iterate", "cfdgpu_batch_solve", "cfdgpu_polar_sweep", "cfdgpu_polar_csv", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
     "cfdgpu_set_field", "cfdgpu_get_massflux", "cfdgpu_set_massflux", "cfdgpu_get_turbulence",
     "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_surface",
     "cfdgpu_sweep_angles", "cfdgpu_debug_array",
@@ -102,6 +103,17 @@ class SolveReport(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("last", Monitor), ("seconds", C.c_double)]
 
 
+class PolarPoint(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("alpha_deg", "cl", "cd", "cd_pressure", "cd_friction", "cm", "l_over_d",
+                                          "upper_separation", "lower_separation")] + \
+               [("status", C.c_int), ("iterations", C.c_int), ("continuity", C.c_double), ("retried_cold", C.c_int)]
+
+
+class PolarHeader(C.Structure):
+    _fields_ = [("section", C.c_char_p), ("mesh", C.c_char_p), ("reynolds", C.c_double), ("speed", C.c_double),
+                ("chord", C.c_double), ("moment_reference", C.c_double), ("continued", C.c_int)]
+
+
 class Forces(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("cl", "cd", "cm", "cd_pressure", "cd_friction", "lift", "drag",
                                           "pitching_moment", "fx_pressure", "fy_pressure", "fx_friction",
@@ -151,7 +163,12 @@ def load(path: str = LIB_PATH):
         "cfdgpu_set_turbulence": (C.c_int, [vp, C.c_int, C.c_void_p]),
         "cfdgpu_initialise": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "cfdgpu_iterate": (C.c_int, [vp, C.c_int, C.POINTER(Monitor)]),
+        "cfdgpu_iterate_timed": (C.c_int, [vp, C.c_int, C.POINTER(Monitor), D, C.POINTER(C.c_int)]),
         "cfdgpu_solve": (C.c_int, [vp, C.POINTER(StopRule), C.POINTER(SolveReport)]),
+        "cfdgpu_batch_create": (C.c_int, [pp, pp, C.c_int]),
+        "cfdgpu_batch_destroy": (None, [vp]),
+        "cfdgpu_batch_iterate": (C.c_int, [vp, C.c_int, C.POINTER(Monitor)]),
+        "cfdgpu_batch_solve": (C.c_int, [vp, C.POINTER(StopRule), C.POINTER(SolveReport)]),
         "cfdgpu_get_settings": (C.c_int, [vp, C.POINTER(Settings)]),
         "cfdgpu_set_settings": (C.c_int, [vp, C.POINTER(Settings)]),
         "cfdgpu_get_field": (C.c_int, [vp, D, D, D, D]),
@@ -165,6 +182,10 @@ def load(path: str = LIB_PATH):
         "cfdgpu_surface": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(Forces)]),
         "cfdgpu_sweep_angles": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, C.c_int]),
+        "cfdgpu_polar_sweep": (C.c_int, [vp, C.POINTER(Settings), C.c_int, C.c_void_p] + [C.c_double] * 5 +
+                               [_dp, C.c_int, C.c_int, C.POINTER(StopRule), C.POINTER(PolarPoint)]),
+        "cfdgpu_polar_csv": (C.c_int, [C.POINTER(PolarPoint), C.c_int, C.POINTER(PolarHeader), C.c_char_p,
+                                       C.c_int]),
         "cfdgpu_debug_array": (C.c_int, [vp, C.c_char_p, _dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -388,6 +409,34 @@ class SimpleSolver:
         return out[:n].copy()
 
 
+class Batch:
+    """Several SimpleSolvers advanced together (cfdgpu_batch_*): one launch per
+    stage for all members. The solvers must stay alive while the batch is used."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = C.c_void_p()
+        _check(lib().cfdgpu_batch_create(C.byref(h), arr, len(self.solvers)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cfdgpu_batch_destroy(self.h)
+            self.h = None
+
+    def iterate(self, count: int = 1):
+        mons = (Monitor * len(self.solvers))()
+        _check(lib().cfdgpu_batch_iterate(self.h, count, mons))
+        return list(mons)
+
+    def solve(self, tolerance: float, max_iterations: int, check_every: int = 25):
+        rule = StopRule(tolerance, max_iterations, check_every)
+        reps = (SolveReport * len(self.solvers))()
+        _check(lib().cfdgpu_batch_solve(self.h, C.byref(rule), reps))
+        return list(reps)
+
+
 def surface(mesh: Mesh, u, v, p, mu, speed=50.0, alpha_deg=0.0, density=1.225, reference_pressure=0.0, chord=1.0):
     """extractSurface + integrateForces in FP64. Returns (Forces, rows[n,12])."""
     cap = 12 * mesh.ni
@@ -398,6 +447,32 @@ def surface(mesh: Mesh, u, v, p, mu, speed=50.0, alpha_deg=0.0, density=1.225, r
     _check(lib().cfdgpu_surface(mesh.h, g(u), g(v), g(p), g(mu), speed, alpha_deg, density, reference_pressure,
                                 chord, rows, cap, C.byref(n), C.byref(f)))
     return f, rows[:12 * n.value].reshape(n.value, 12)
+
+
+def polar_sweep(mesh: Mesh, alphas, settings: Settings | None = None, model: int = 0, speed=50.0, density=1.225,
+                reynolds=1e6, chord=1.0, reference_pressure=0.0, continued=True, tolerance=1e-6,
+                max_iterations=10000, check_every=25):
+    """The reference's continued polar sweep on the device (cfdgpu_polar_sweep)."""
+    a = np.ascontiguousarray(alphas, dtype=np.float64)
+    out = (PolarPoint * len(a))()
+    rule = StopRule(tolerance, max_iterations, check_every)
+    _check(lib().cfdgpu_polar_sweep(mesh.h, C.byref(settings or default_settings()), model, None, speed, density,
+                                    reynolds, chord, reference_pressure, a, len(a), 1 if continued else 0,
+                                    C.byref(rule), out))
+    return list(out)
+
+
+def polar_csv(points, section="unknown", mesh="unknown", reynolds=1e6, speed=50.0, chord=1.0,
+              moment_reference=0.25, continued=True) -> str:
+    """post::toCsv (Polar.cpp:126-163)."""
+    arr = (PolarPoint * len(points))(*points)
+    h = PolarHeader(section.encode(), mesh.encode(), reynolds, speed, chord, moment_reference, 1 if continued else 0)
+    n = lib().cfdgpu_polar_csv(arr, len(points), C.byref(h), None, 0)
+    if n < 0:
+        raise CfdError(-n, lib().cfdgpu_last_error().decode())
+    buf = C.create_string_buffer(n + 1)
+    lib().cfdgpu_polar_csv(arr, len(points), C.byref(h), buf, n + 1)
+    return buf.value.decode()
 
 
 def sweep_angles(start: float, end: float, step: float):

@@ -12,6 +12,7 @@
 
 #include "../../include/cfdgpu.h"
 #include "host/engine.hpp"
+#include "host/polar.hpp"
 #include "host/post.hpp"
 
 namespace {
@@ -260,22 +261,7 @@ int cfdgpu_face_conditions(const cfdgpu_mesh* m, double speed, double alpha, dou
   if (!m || !kind || !vxy || !pressure) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
   if (!std::isfinite(speed) || speed <= 0.0) return fail(CFDGPU_INVALID_ARGUMENT, "freestream speed must be positive");
   if (!std::isfinite(alpha) || std::abs(alpha) > 90.0) return fail(CFDGPU_INVALID_ARGUMENT, "angle of attack must lie within +/-90 degrees");
-  const double a = alpha * 3.141592653589793238462643383279502884 / 180.0;
-  const double sx = speed * std::cos(a), sy = speed * std::sin(a);
-  const cfdgpu::Grid& g = m->grid;
-  for (int f = 0; f < g.faces(); ++f) {
-    int k = CFDGPU_K_INTERNAL;
-    switch (g.faceType(f)) {
-      case cfdgpu::kWallB: k = CFDGPU_K_WALL; break;
-      case cfdgpu::kFarfieldB: k = CFDGPU_K_FARFIELD; break;
-      case cfdgpu::kOutletB: k = CFDGPU_K_OUTLET; break;
-      default: k = CFDGPU_K_INTERNAL; break;
-    }
-    kind[f] = k;
-    pressure[f] = pref;
-    vxy[2 * f] = (k == CFDGPU_K_WALL) ? 0.0 : sx;
-    vxy[2 * f + 1] = (k == CFDGPU_K_WALL) ? 0.0 : sy;
-  }
+  cfdgpu::defaultFaceConditions(m->grid, speed, alpha, pref, kind, vxy, pressure);
   return CFDGPU_OK;
 }
 
@@ -390,6 +376,41 @@ int cfdgpu_solve(cfdgpu_solver* s, const cfdgpu_stop_rule* rule, cfdgpu_solve_re
   return guarded([&]() -> int { return s->engine->solve(*rule, out); });
 }
 
+int cfdgpu_batch_create(cfdgpu_batch** out, cfdgpu_solver* const* solvers, int count) {
+  if (!out || !solvers || count <= 0) return fail(CFDGPU_INVALID_ARGUMENT, "null argument or empty batch");
+  return guarded([&]() -> int {
+    std::vector<cfdgpu::Engine*> members;
+    members.reserve(static_cast<size_t>(count));
+    for (int q = 0; q < count; ++q) {
+      if (!solvers[q] || !solvers[q]->engine) return fail(CFDGPU_INVALID_ARGUMENT, "null solver in batch");
+      members.push_back(solvers[q]->engine.get());
+    }
+    auto* b = new cfdgpu_batch();
+    try {
+      b->batch = std::make_unique<cfdgpu::Batch>(std::move(members));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+    return CFDGPU_OK;
+  });
+}
+
+void cfdgpu_batch_destroy(cfdgpu_batch* b) { delete b; }
+
+int cfdgpu_batch_iterate(cfdgpu_batch* b, int count, cfdgpu_monitor* out) {
+  if (!b || !b->batch) return fail(CFDGPU_INVALID_ARGUMENT, "null batch");
+  if (count < 0) return fail(CFDGPU_INVALID_ARGUMENT, "negative iteration count");
+  return guarded([&]() -> int { return b->batch->iterate(count, out); });
+}
+
+int cfdgpu_batch_solve(cfdgpu_batch* b, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out) {
+  if (!b || !b->batch) return fail(CFDGPU_INVALID_ARGUMENT, "null batch");
+  if (!rule) return fail(CFDGPU_INVALID_ARGUMENT, "null stop rule");
+  return guarded([&]() -> int { return b->batch->solve(*rule, out); });
+}
+
 int cfdgpu_get_settings(const cfdgpu_solver* s, cfdgpu_settings* out) {
   SOLVER_GUARD(s)
   if (!out) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
@@ -460,6 +481,47 @@ int cfdgpu_surface(const cfdgpu_mesh* m, const double* u, const double* v, const
                                             count, forces);
     return rc == 0 ? CFDGPU_OK : fail(rc, "the mesh has no aerofoil surface to extract");
   });
+}
+
+int cfdgpu_polar_sweep(const cfdgpu_mesh* m, const cfdgpu_settings* settings, int model,
+                       const cfdgpu_sst_constants* sst, double speed, double density, double reynolds, double chord,
+                       double pref, const double* alphas, int count, int continued, const cfdgpu_stop_rule* rule,
+                       cfdgpu_polar_point* out) {
+  if (!m || !settings || !alphas || !rule || !out || count < 0) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  if (!(speed > 0.0) || !(density > 0.0) || !(reynolds > 0.0) || !(chord > 0.0))
+    return fail(CFDGPU_INVALID_ARGUMENT, "freestream must be positive and finite");
+  if (model < CFDGPU_MODEL_NONE || model > CFDGPU_MODEL_SST) return fail(CFDGPU_INVALID_ARGUMENT, "unknown model");
+  if (const int rc = checkPathOptions(*settings); rc != CFDGPU_OK) return rc;
+  return guarded([&]() -> int {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
+      return fail(CFDGPU_INITIALIZATION_FAILURE, "no HIP device: the MI355X path needs a GPU");
+    cfdgpu::PolarRequest rq;
+    rq.grid = &m->grid;
+    rq.settings = *settings;
+    rq.model = model;
+    rq.sst = sst;
+    rq.speed = speed;
+    rq.density = density;
+    rq.reynolds = reynolds;
+    rq.chord = chord;
+    rq.referencePressure = pref;
+    rq.alphas.assign(alphas, alphas + count);
+    rq.continued = continued != 0;
+    rq.rule = *rule;
+    std::vector<cfdgpu_polar_point> pts;
+    cfdgpu::polarSweep(rq, pts);
+    for (size_t q = 0; q < pts.size(); ++q) out[q] = pts[q];
+    return CFDGPU_OK;
+  });
+}
+
+int cfdgpu_polar_csv(const cfdgpu_polar_point* points, int count, const cfdgpu_polar_header* header, char* buf,
+                     int capacity) {
+  if ((!points && count > 0) || !header || count < 0) return -fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  const std::string text = cfdgpu::polarCsv(points, count, *header);
+  if (buf && capacity > static_cast<int>(text.size())) std::memcpy(buf, text.c_str(), text.size() + 1);
+  return static_cast<int>(text.size());
 }
 
 int cfdgpu_sweep_angles(double start, double end, double step, double* out, int capacity) {

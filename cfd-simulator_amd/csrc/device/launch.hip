@@ -177,6 +177,17 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   end(CFDGPU_STAGE_MONITOR);
 }
 
+namespace {
+__global__ void k_resume(const Inst* insts, int count) {
+  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b < count) insts[b].flags[fStopped] = 0;
+}
+}  // namespace
+
+void launchResumeAll(const Inst* d, int count, hipStream_t s) {
+  hipLaunchKernelGGL(k_resume, dim3((count + 63) / 64), dim3(64), 0, s, d, count);
+}
+
 void launchDivergence(const Inst* d, const LaunchCfg& cfg, double* out, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
   hipLaunchKernelGGL(k_divergence, grid, dim3(kBlock), 0, s, d, out);

@@ -405,24 +405,30 @@ __device__ __forceinline__ void ldsBarrier() {
 // D is the read-ahead: a ring of D cells per lane, slot q holding step
 // r * D + q and refilled with the step D ahead right after use.
 template <int NC, int D>
+// R is the instance's layout stride (its lane count); the block may be wider
+// (a batch launches with its widest member's lanes). Lanes past R take no
+// cell; their global addresses are clamped to lane R - 1 and LDS is laid out
+// by the block width B.
 __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSkew<NC>& S, int sweeps) {
   HIP_DYNAMIC_SHARED(double, lds)
-  double* buf = lds;                                        // [2][NC][R]: value just written by each row
-  double* row0 = lds + 2 * NC * R;                          // [NC][nI]: row j = 0, updated in place
+  const int B = static_cast<int>(blockDim.x);
+  double* buf = lds;                                        // [2][NC][B]: value just written by each row
+  double* row0 = lds + 2 * NC * B;                          // [NC][nI]: row j = 0, updated in place
   int* partner = reinterpret_cast<int*>(row0 + NC * nI);    // [nI]
   const int lane = static_cast<int>(threadIdx.x);
+  const int laneG = min(lane, R - 1);
   const bool active = lane < nJ;
   const int steps = skewSteps(nI, nJ);
   // Steps past nI + nJ - 2 touch no cell (k - j >= nI for every row), so the
   // count is rounded up to whole rings and the loop body has no tail.
   const int rings = (steps + D - 1) / D;
   const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
-  for (int t = lane; t < nI; t += R) partner[t] = partnerSrc[t];
+  for (int t = lane; t < nI; t += B) partner[t] = partnerSrc[t];
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     // selects, not S.x[sweep & 1]: a run-time index would put S in scratch
     const CFD_GLOBAL dpair* xOld = (sweep & 1) ? S.x[1] : S.x[0];
     CFD_GLOBAL dpair* xNew = (sweep & 1) ? S.x[0] : S.x[1];
-    for (int t = lane; t < nI; t += R) {
+    for (int t = lane; t < nI; t += B) {
       const dpair v = xOld[skewIndex(t, 0, R)];
       row0[t] = v[0];
       if constexpr (NC == 2) row0[nI + t] = v[1];
@@ -433,7 +439,7 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
     for (int q = 0; q < NC; ++q) xw[q] = 0.0;
     GsCell<NC> ring[D];
 #pragma unroll
-    for (int q = 0; q < D; ++q) gsFetch<NC>(S, xOld, R, steps, lane, q, ring[q]);
+    for (int q = 0; q < D; ++q) gsFetch<NC>(S, xOld, R, steps, laneG, q, ring[q]);
     for (int r = 0; r < rings; ++r) {
 #pragma unroll
       for (int q = 0; q < D; ++q) {
@@ -447,7 +453,7 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
           const double below =
-              (lane > 0) ? buf[((k - 1) & 1) * NC * R + m * R + (lane - 1)] : row0[m * nI + max(pr, 0)];
+              (lane > 0) ? buf[((k - 1) & 1) * NC * B + m * B + (lane - 1)] : row0[m * nI + max(pr, 0)];
           // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
           // a missing neighbour's term is skipped, not added as zero
           double off = 0.0;
@@ -463,7 +469,7 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
           // in flight is the same on every path and the compiler's waits stay
           // partial. Lanes off the wavefront write the padding row past the
           // last step, which no fetch reads.
-          const int at = on ? k * R + lane : (steps + 1) * R + lane;
+          const int at = on ? k * R + lane : (steps + 1) * R + laneG;
           const dpair v = {xn[0], NC == 2 ? xn[NC - 1] : 0.0};
           xNew[at] = v;
         }
@@ -475,10 +481,10 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
         // the next step too, and this row's next cell has i + 1 <= 0 or >= nI.
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-          buf[(k & 1) * NC * R + m * R + lane] = xn[m];
+          buf[(k & 1) * NC * B + m * B + lane] = xn[m];
           xw[m] = xn[m];
         }
-        gsFetch<NC>(S, xOld, R, steps, lane, k + D, ring[q]);
+        gsFetch<NC>(S, xOld, R, steps, laneG, k + D, ring[q]);
         ldsBarrier();
       }
     }

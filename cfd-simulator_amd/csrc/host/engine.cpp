@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -577,10 +578,12 @@ int Engine::initialise(const double* u, const double* v, const double* p, const 
   return CFDGPU_OK;
 }
 
-void Engine::readMonitor(cfdgpu_monitor* out) const {
+void Engine::readMonitor(cfdgpu_monitor* out) const { readMonitorOn(stream_, out); }
+
+void Engine::readMonitorOn(hipStream_t s, cfdgpu_monitor* out) const {
   DevMonitor m{};
-  check(hipMemcpyAsync(&m, h_.monitor, sizeof(DevMonitor), hipMemcpyDeviceToHost, stream_), "read monitor");
-  check(hipStreamSynchronize(stream_), "monitor sync");
+  check(hipMemcpyAsync(&m, h_.monitor, sizeof(DevMonitor), hipMemcpyDeviceToHost, s), "read monitor");
+  check(hipStreamSynchronize(s), "monitor sync");
   out->continuity = m.continuity;
   out->momentum_x = m.momentumX;
   out->momentum_y = m.momentumY;
@@ -803,6 +806,115 @@ int Engine::debugArray(const std::string& name, double* out, int capacity) const
   }
   check(hipStreamSynchronize(stream_), "debug sync");
   return static_cast<int>(count);
+}
+
+}  // namespace cfdgpu
+
+namespace cfdgpu {
+
+Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
+  if (members_.empty()) throw std::invalid_argument("a batch needs at least one solver");
+  if (members_.size() > 65535) throw std::invalid_argument("too many solvers for one batch");
+  const Inst& a = members_.front()->refreshInst();
+  size_t lds = 0;
+  int lanes = 64, blocks = 1, maxNI = 0;
+  for (Engine* e : members_) {
+    if (!e || !e->ready()) throw std::invalid_argument("every solver of a batch must be initialised");
+    const Inst& b = e->refreshInst();
+    if (b.model != a.model || b.momSweeps != a.momSweeps || b.cgMax != a.cgMax ||
+        b.nonOrthPasses != a.nonOrthPasses || b.gsOrder != a.gsOrder || b.precision != a.precision ||
+        b.hasPRef != a.hasPRef)
+      throw std::invalid_argument("batched solvers must share closure, sweeps, CG cap, correctors, ordering and "
+                                  "pressure anchoring");
+    lanes = std::max(lanes, b.gsR);
+    blocks = std::max(blocks, b.nBlocks);
+    maxNI = std::max(maxNI, b.nI);
+  }
+  lds = gsLdsFor(0, maxNI, lanes);
+  if (a.gsOrder == CFDGPU_GS_LEXICOGRAPHIC && lds > 64 * 1024)
+    throw std::invalid_argument("batch too wide for the lexicographic sweep's LDS; use CFDGPU_GS_REDBLACK");
+  cfg_ = LaunchCfg{blocks, 1, static_cast<int>(members_.size()), lanes, lds};
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipMalloc(&dInsts_, sizeof(Inst) * members_.size()), "hipMalloc (batch descriptors)");
+  host_.resize(members_.size());
+}
+
+Batch::~Batch() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (dInsts_) (void)hipFree(dInsts_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Members' own streams drain first; then their current descriptors go up.
+void Batch::gather() {
+  for (size_t q = 0; q < members_.size(); ++q) {
+    check(hipStreamSynchronize(members_[q]->stream()), "member sync");
+    host_[q] = members_[q]->refreshInst();
+  }
+  check(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),
+        "upload batch descriptors");
+}
+
+int Batch::iterate(int count, cfdgpu_monitor* out) {
+  gather();
+  launchResumeAll(dInsts_, size(), stream_);
+  for (int n = 0; n < count; ++n) launchIteration(dInsts_, cfg_, host_.front(), stream_);
+  check(hipGetLastError(), "launch");
+  check(hipStreamSynchronize(stream_), "batch sync");
+  if (out)
+    for (size_t q = 0; q < members_.size(); ++q) members_[q]->readMonitorOn(stream_, &out[q]);
+  return CFDGPU_OK;
+}
+
+// The stop rule of each member, evaluated on the device; finished members
+// return at once from every later launch while the others continue.
+int Batch::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t n = members_.size();
+  std::vector<int> start(n, 0);
+  std::vector<std::array<int, kIFlags>> flags(n);
+  for (size_t q = 0; q < n; ++q) {
+    check(hipMemcpyAsync(&start[q], members_[q]->inst().flags + fIteration, sizeof(int), hipMemcpyDeviceToHost,
+                         members_[q]->stream()),
+          "flags");
+    check(hipStreamSynchronize(members_[q]->stream()), "member sync");
+    Inst& h = members_[q]->hostInst();
+    h.stopTol = rule.tolerance;
+    h.stopMaxIter = (rule.max_iterations > 0) ? start[q] + rule.max_iterations : 0;
+  }
+  // refreshInst() keeps stopTol / stopMaxIter (they are not settings)
+  gather();
+  launchResumeAll(dInsts_, size(), stream_);
+  const int chunk = std::max(1, rule.check_every);
+  int launched = 0;
+  for (;;) {
+    const int m = (rule.max_iterations > 0) ? std::min(chunk, rule.max_iterations - launched) : chunk;
+    for (int q = 0; q < m; ++q) launchIteration(dInsts_, cfg_, host_.front(), stream_);
+    launched += m;
+    check(hipGetLastError(), "launch");
+    for (size_t q = 0; q < n; ++q)
+      check(hipMemcpyAsync(flags[q].data(), host_[q].flags, sizeof(int) * kIFlags, hipMemcpyDeviceToHost, stream_),
+            "flags");
+    check(hipStreamSynchronize(stream_), "batch solve sync");
+    bool all = true;
+    for (size_t q = 0; q < n; ++q) all = all && flags[q][fStopped];
+    if (all) break;
+    if (rule.max_iterations > 0 && launched >= rule.max_iterations) break;
+  }
+  const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (size_t q = 0; q < n; ++q) {
+    if (out) {
+      out[q].iterations = flags[q][fIteration] - start[q];
+      out[q].status = flags[q][fStopped] ? flags[q][fStatus] : 1;
+      members_[q]->readMonitorOn(stream_, &out[q].last);
+      out[q].seconds = seconds;
+    }
+    Inst& h = members_[q]->hostInst();
+    h.stopTol = 0.0;
+    h.stopMaxIter = 0;
+    members_[q]->uploadInst();
+  }
+  return CFDGPU_OK;
 }
 
 }  // namespace cfdgpu

@@ -39,6 +39,8 @@ struct StageRecorder {
 void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hipStream_t s,
                      StageRecorder* rec = nullptr);
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
+// clears the stop flag of `count` instances (a plain iterate never stops by itself)
+void launchResumeAll(const Inst* dInst, int count, hipStream_t s);
 int gsThreadsFor(int nJ);
 size_t gsLdsFor(int nJ, int nI, int threads);
 
@@ -82,8 +84,15 @@ class Engine {
   const Grid& grid() const { return grid_; }
   bool ready() const { return initialised_; }
   hipStream_t stream() const { return stream_; }
-  // for batching: the instance descriptor with device pointers
+  // for batching: the instance descriptor with device pointers, refreshed from
+  // the current settings, and the monitor read on another stream
   const Inst& inst() const { return h_; }
+  const Inst& refreshInst() {
+    fillInstScalars();
+    return h_;
+  }
+  Inst& hostInst() { return h_; }
+  void readMonitorOn(hipStream_t s, cfdgpu_monitor* out) const;
   void uploadInst();
 
  private:
@@ -125,7 +134,34 @@ class Engine {
   std::unique_ptr<StageClock> clock_;
 };
 
+// Several solves advanced together: one launch per stage for all of them
+// (grid.y = instance), each instance's state in its own arena. Members must
+// share the launch-shaping settings (closure, sweeps, CG cap, corrector
+// passes, ordering, pressure anchoring); meshes may differ.
+class Batch {
+ public:
+  explicit Batch(std::vector<Engine*> members);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+  int size() const { return static_cast<int>(members_.size()); }
+  int iterate(int count, cfdgpu_monitor* out);
+  int solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out);
+
+ private:
+  void gather();
+  std::vector<Engine*> members_;
+  std::vector<Inst> host_;
+  Inst* dInsts_{nullptr};
+  hipStream_t stream_{nullptr};
+  LaunchCfg cfg_{};
+};
+
 }  // namespace cfdgpu
+
+struct cfdgpu_batch {
+  std::unique_ptr<cfdgpu::Batch> batch;
+};
 
 struct cfdgpu_solver {
   std::unique_ptr<cfdgpu::Engine> engine;

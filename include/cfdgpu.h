@@ -77,6 +77,7 @@ enum {
 
 typedef struct cfdgpu_mesh cfdgpu_mesh;
 typedef struct cfdgpu_solver cfdgpu_solver;
+typedef struct cfdgpu_batch cfdgpu_batch;
 
 /* mesh::CGridOptions (include/cfd/mesh/CGrid.hpp:53-84) */
 typedef struct {
@@ -155,6 +156,24 @@ typedef struct {
   int upper_separation_face, lower_separation_face; /* wall face index (after), -1 if none */
 } cfdgpu_forces;
 
+/* post::PolarPoint (include/cfd/post/Polar.hpp:32-60) */
+typedef struct {
+  double alpha_deg, cl, cd, cd_pressure, cd_friction, cm, l_over_d;
+  double upper_separation, lower_separation; /* chord fraction, -1 where attached */
+  int status;     /* 0 converged, 1 iteration-limit, 2 diverged (Polar.cpp:79-84) */
+  int iterations;
+  double continuity; /* last continuity residual */
+  int retried_cold;
+} cfdgpu_polar_point;
+
+/* The header fields of post::Polar that toCsv prints (Polar.cpp:126-140) */
+typedef struct {
+  const char* section;
+  const char* mesh;
+  double reynolds, speed, chord, moment_reference;
+  int continued;
+} cfdgpu_polar_header;
+
 /* ---- library ---- */
 const char* cfdgpu_version(void);
 const char* cfdgpu_last_error(void);
@@ -211,6 +230,19 @@ int cfdgpu_iterate_timed(cfdgpu_solver* solver, int count, cfdgpu_monitor* out, 
                          int* stage_calls);
 /* the SolverWorker loop (SolverWorker.cpp:134-277) run device-resident */
 int cfdgpu_solve(cfdgpu_solver* solver, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out);
+/* ---- batches: several solves advanced together, one launch per stage for all
+ *      of them. No reference counterpart (the reference runs one solve per
+ *      thread); it serves polar sweeps and mixed-size batches. The batch refers
+ *      to the solvers, which must be initialised, outlive it, and not be used
+ *      on their own while a batch call runs. Members must share the closure,
+ *      sweep count, CG cap, corrector passes, ordering and pressure anchoring;
+ *      meshes may differ. ---- */
+int cfdgpu_batch_create(cfdgpu_batch** out, cfdgpu_solver* const* solvers, int count);
+void cfdgpu_batch_destroy(cfdgpu_batch* batch);
+/* count outer iterations of every member; out[i] = member i's last monitor (may be NULL) */
+int cfdgpu_batch_iterate(cfdgpu_batch* batch, int count, cfdgpu_monitor* out);
+/* every member's stop rule on the device until all have stopped; out[i] per member (may be NULL) */
+int cfdgpu_batch_solve(cfdgpu_batch* batch, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out);
 int cfdgpu_get_settings(const cfdgpu_solver* solver, cfdgpu_settings* out);
 int cfdgpu_set_settings(cfdgpu_solver* solver, const cfdgpu_settings* settings);
 /* SimpleSolver::field() */
@@ -236,6 +268,22 @@ int cfdgpu_surface(const cfdgpu_mesh* mesh, const double* u, const double* v, co
                    const double* mu, double speed, double alpha_deg, double density,
                    double reference_pressure, double chord, double* rows, int capacity,
                    int* station_count, cfdgpu_forces* forces);
+/* The reference's polar sweep (app/Panels.cpp:1137-1157, 1581-1800 with
+ * post::nextSweepAction, Polar.cpp:86-124): for each angle a new solve from the
+ * uniform field of that freestream, with velocity and pressure carried over from
+ * the previous angle when continue_between_points (the first angle starts from
+ * the uniform field); the closure is attached and initialised cold at every
+ * point; a diverged point is retried once from cold, and if the retry diverges
+ * too the next angle starts cold. Every solve runs on the device under `rule`.
+ * model: CFDGPU_MODEL_*; sst NULL = default constants. out[count]. */
+int cfdgpu_polar_sweep(const cfdgpu_mesh* mesh, const cfdgpu_settings* settings, int model,
+                       const cfdgpu_sst_constants* sst, double speed, double density, double reynolds,
+                       double chord, double reference_pressure, const double* alphas, int count,
+                       int continue_between_points, const cfdgpu_stop_rule* rule, cfdgpu_polar_point* out);
+/* post::toCsv (Polar.cpp:126-163). Returns the text length (without the NUL);
+ * the text is written to buf when capacity exceeds it. Negative: -status. */
+int cfdgpu_polar_csv(const cfdgpu_polar_point* points, int count, const cfdgpu_polar_header* header, char* buf,
+                     int capacity);
 /* post::sweepAngles (Polar.cpp:36-77): returns count, or -status on error */
 int cfdgpu_sweep_angles(double start, double end, double step, double* out, int capacity);
 

@@ -18,8 +18,11 @@ static int fail(const char* what, int nI, int nJ, long v, long lim) {
   return 1;
 }
 
-static int check(int nI, int nJ, int D, int NC) {
+// B: the block's lane count (>= R when the instance is one of a batch whose
+// widest member has more rows); lanes past R address as lane R - 1.
+static int check(int nI, int nJ, int D, int NC, int B) {
   const int R = gsLanes(nJ);
+  if (B < R) return 0;
   if (R < nJ) return fail("lanes < rows", nI, nJ, R, nJ);
   const long len = skewLength(nI, nJ, R);
   const int steps = skewSteps(nI, nJ);
@@ -38,9 +41,10 @@ static int check(int nI, int nJ, int D, int NC) {
   const int rings = (steps + D - 1) / D;
   const int kmax = rings * D - 1 + D;  // last refill
   for (int k = 0; k <= kmax; ++k)
-    for (int lane = 0; lane < R; ++lane) {
-      const long at = static_cast<long>(k < steps - 1 ? k : steps - 1) * R + lane;
-      const long atN = static_cast<long>(k + 1 < steps - 1 ? k + 1 : steps - 1) * R + lane + 1;
+    for (int lane = 0; lane < B; ++lane) {
+      const int laneG = lane < R - 1 ? lane : R - 1;
+      const long at = static_cast<long>(k < steps - 1 ? k : steps - 1) * R + laneG;
+      const long atN = static_cast<long>(k + 1 < steps - 1 ? k + 1 : steps - 1) * R + laneG + 1;
       if (at < 0 || at >= len) return fail("fetch", nI, nJ, at, len);
       if (atN < 0 || atN >= len) return fail("fetch north", nI, nJ, atN, len);
       const long atE0 = atN - 1;
@@ -48,15 +52,15 @@ static int check(int nI, int nJ, int D, int NC) {
       const int i = k - lane;
       const bool on = lane < nJ && i >= 0 && i < nI && k < rings * D;
       // LDS: step buffer, row 0, partner map (partner values lie in [-1, nI))
-      const long bufRead = static_cast<long>(((k - 1) & 1)) * NC * R + (NC - 1) * R + (lane > 0 ? lane - 1 : 0);
-      if (bufRead >= 2L * NC * R) return fail("lds buf read", nI, nJ, bufRead, 2L * NC * R);
-      const long bufWrite = static_cast<long>(k & 1) * NC * R + (NC - 1) * R + lane;
-      if (bufWrite >= 2L * NC * R) return fail("lds buf write", nI, nJ, bufWrite, 2L * NC * R);
+      const long bufRead = static_cast<long>(((k - 1) & 1)) * NC * B + (NC - 1) * B + (lane > 0 ? lane - 1 : 0);
+      if (bufRead >= 2L * NC * B) return fail("lds buf read", nI, nJ, bufRead, 2L * NC * B);
+      const long bufWrite = static_cast<long>(k & 1) * NC * B + (NC - 1) * B + lane;
+      if (bufWrite >= 2L * NC * B) return fail("lds buf write", nI, nJ, bufWrite, 2L * NC * B);
       const int pIdx = i < 0 ? 0 : (i > nI - 1 ? nI - 1 : i);
       if (pIdx < 0 || pIdx >= nI) return fail("lds partner", nI, nJ, pIdx, nI);
       // every lane stores every step: on the wavefront to its cell, off it
       // to the padding row past the last step
-      const long sink = static_cast<long>(steps + 1) * R + lane;
+      const long sink = static_cast<long>(steps + 1) * R + laneG;
       if (sink >= len || sink < static_cast<long>(steps + 1) * R) return fail("sink store", nI, nJ, sink, len);
       if (!on) continue;
       // the fetched slots hold the cell, the cell above and (next step) the cell to the right
@@ -80,8 +84,11 @@ int main() {
                            {256, 64}, {512, 128}, {1024, 256}, {300, 257}, {40, 1000}, {7, 1024}};
   for (const auto& s : shapes) {
     const int nI = s[0], nJ = s[1];
-    const int Dm = gsLanes(nJ) <= 256 ? 6 : 2, Dt = gsLanes(nJ) <= 256 ? 8 : 2;
-    if (check(nI, nJ, Dm, 2) || check(nI, nJ, Dt, 1)) return 1;
+    const int Dm = 4, Dt = 6;
+    for (int B : {64, 128, 256, 512, 1024}) {
+      const int dm = B <= 256 ? Dm : 2, dt = B <= 256 ? Dt : 2;
+      if (check(nI, nJ, dm, 2, B) || check(nI, nJ, dt, 1, B)) return 1;
+    }
   }
   std::printf("ok\n");
   return 0;
