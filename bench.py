@@ -37,9 +37,18 @@ METRIC = "converged solves/s batched-mixed vs plain FP64 at |ΔCl|<1e-3; p50/p99
 UNIT = "converged solves/s"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
-# Outer iterations of the C2 solve until |Cl - Cl_conv| < 1e-3 and
-# |Cd - Cd_conv| < 1e-4 hold for good (python bench.py --converge).
-ITERATIONS_TO_CONVERGE = None
+# Outer iterations of this solve from the uniform field to the reference's
+# stop rule, residuals.worst() < 1e-7 (SolverWorker.cpp:182; the workload's
+# outer-residual target), measured on one MI355X with
+# `python bench.py --solve --tolerance 1e-7` (80.9 s, Cl 0.60811, Cd 0.019724).
+# The plain FP64 path repeats the oracle's iterates, so the oracle stops at
+# the same iteration with the same coefficients (|dCl| at round-off).
+ITERATIONS_TO_CONVERGE = 12068
+
+# HBM bytes per launch of the dominant kernels from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE, scaled by the same counters on a copy of known size
+# (profiles/c2_traffic.md); None until measured for this build.
+TRAFFIC_BYTES = {"k_gs_lex_mom": 83.67e6, "k_gs_lex_turb": 151.62e6, "k_cg_a + k_cg_b": 32.19e6}
 
 WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
                 first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
@@ -54,20 +63,18 @@ def traffic_model():
         "gradients": 15,
         "momentum_assembly": 36,
         "momentum_sweeps": 33,      # 3 sweeps, u and v: 5 coefficients + 2 sources + 2 reads + 2 writes
+        "sst_sweeps": 48,           # 3 sweeps each of k and omega: 5 coefficients + 1 source + 1 read + 1 write
         "rhie_chow_pressure": 34,
         "correction": 23,
         "sst_update": 130,
     }
 
 
-def stage_bytes(stage: str, cells: int, word: int = 8) -> float | None:
-    """Algorithmic bytes of one call of a stage."""
-    tm = traffic_model()
-    key = {"momentum_sweeps": "momentum_sweeps", "gradients": "gradients",
-           "momentum_assembly": "momentum_assembly", "correction": "correction"}.get(stage)
-    if key is None:
-        return None
-    return float(tm[key]) * cells * word
+# The kernels that can dominate an outer iteration: (stage, kernel, unit of
+# the traffic model, launches per stage call and CG iteration).
+DOMINANT_CANDIDATES = [("momentum_sweeps", "k_gs_lex_mom", "momentum_sweeps"),
+                       ("sst_sweeps", "k_gs_lex_turb", "sst_sweeps"),
+                       ("pressure_solve", "k_cg_a + k_cg_b", "cg_iteration")]
 
 
 def build_case(G):
@@ -123,7 +130,7 @@ def cpu_baseline(iterations: int, iters_per_solve):
     t0 = time.perf_counter()
     s.iterate(iterations)
     sec = (time.perf_counter() - t0) / iterations
-    value = (1.0 / (sec * iters_per_solve)) if iters_per_solve else None
+    value = (1.0 / (sec * iters_per_solve)) if iters_per_solve else None  # converged solves/s
     return {"value": value, "unit": UNIT if iters_per_solve else "outer iterations/s",
             "iterations_per_s": 1.0 / sec, "s_per_iteration": sec, "cores": 1, "kind": "port",
             "sample": f"{iterations} outer iterations of the same solve (FP64 oracle, 1 thread, -O2), "
@@ -144,20 +151,26 @@ def run_bench(args):
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
 
-    # dominant stage and its kernel
-    ranked = sorted(stage_ms.items(), key=lambda kv: -kv[1])
-    dom = ranked[0][0]
-    kernels = {"momentum_sweeps": "k_gs_lex_mom", "pressure_solve": "k_cg_a/k_cg_b",
-               "sst_sweeps": "k_gs_lex_turb", "momentum_assembly": "k_mom", "gradients": "k_grad"}
-    calls = max(stage_calls[dom], 1)
-    per_launch_ms = stage_ms[dom] / calls
-    algo = stage_bytes(dom, mesh.cells)
-    roofline = {"bound": "hbm", "kernel": kernels.get(dom, dom), "stage": dom,
-                "achieved": (algo / (per_launch_ms * 1e-3) / 1e9) if algo else None,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
-                "algorithmic_bytes_per_launch": algo, "launch_ms": per_launch_ms}
-    if roofline["achieved"] is not None:
-        roofline["frac"] = roofline["achieved"] / HBM_PEAK_GBS
+    # The dominant kernel: the sweeps are one launch per stage call; the CG is
+    # two launches per CG iteration (the pair is one unit of the traffic model).
+    tm = traffic_model()
+    cg_its = max(mon.pressure_iterations, 1) * args.steps  # CG iterations in the timed region (cap reached each time)
+    # Dominance is per kernel: the pressure stage is two kernels, k_cg_a and
+    # k_cg_b, of about equal time, so each counts for half of it.
+    def entry(stage, kernel, unit):
+        units = cg_its if unit == "cg_iteration" else stage_calls[stage]
+        unit_ms = stage_ms[stage] / units
+        algo = float(tm[unit]) * mesh.cells * 8
+        achieved = algo / (unit_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get(kernel), "kernel": kernel,
+                "stage": stage, "algorithmic_bytes_per_launch": algo, "launch_ms": unit_ms, "words_per_cell": tm[unit]}
+
+    present = [c for c in DOMINANT_CANDIDATES if stage_calls.get(c[0])]
+    weight = {"cg_iteration": 0.5}
+    stage, kernel, unit = max(present, key=lambda c: stage_ms[c[0]] * weight.get(c[2], 1.0))
+    roofline = entry(stage, kernel, unit)
+    cg = entry("pressure_solve", "k_cg_a + k_cg_b", "cg_iteration")
 
     n_conv = ITERATIONS_TO_CONVERGE
     value = (1000.0 / (ms_per_step * n_conv)) if n_conv else None
@@ -174,6 +187,7 @@ def run_bench(args):
                    "cg_iterations_last": mon.pressure_iterations,
                    "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items() if stage_calls[k]}},
         "roofline": roofline,
+        "roofline_pressure_cg": cg,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_iterations, n_conv)
@@ -235,6 +249,20 @@ def run_solve(args):
                       "tolerance": args.tolerance, "cl": f.cl, "cd": f.cd, "cm": f.cm}), flush=True)
 
 
+def run_calibrate_copy(args):
+    """A device-to-device copy of known size, for scaling the HBM traffic
+    counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) to bytes."""
+    import torch
+
+    n = 64 * 1024 * 1024  # doubles: 512 MiB read + 512 MiB written per copy
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,12 +277,15 @@ def main():
     ap.add_argument("--stop-at-tolerance", action="store_true")
     ap.add_argument("--solve", action="store_true", help="one solve to the residual tolerance, timed")
     ap.add_argument("--check-every", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true", help="a copy of known size (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
               "by instance, one solve per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    if args.solve:
+    if args.calibrate_copy:
+        run_calibrate_copy(args)
+    elif args.solve:
         run_solve(args)
     elif args.converge:
         run_converge(args)

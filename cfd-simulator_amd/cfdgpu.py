@@ -81,7 +81,7 @@ class SstConstants(C.Structure):
 # CFDGPU_STAGE_* in order
 STAGES = ("gradients", "momentum_assembly", "momentum_sweeps", "rhie_chow", "pressure_assembly",
           "pressure_solve", "correction", "sst_blending", "sst_assembly", "sst_sweeps",
-          "sst_wall_viscosity", "monitor")
+          "sst_wall_viscosity", "monitor", "sweep_copy")
 
 
 class Monitor(C.Structure):

@@ -96,17 +96,21 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
       hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 0, 0);
       hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 1, sw == h.momSweeps - 1 ? 1 : 0);
     }
+    end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   } else if (lex) {
     if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_mom, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
     else hipLaunchKernelGGL(k_gs_lex_mom_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
+    begin(CFDGPU_STAGE_SWEEP_COPY);
     hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 0, h.momSweeps);
+    end(CFDGPU_STAGE_SWEEP_COPY);
   } else {
     for (int sw = 0; sw < h.momSweeps; ++sw) {
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 0, 0);
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 1, 0);
     }
+    end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   }
-  end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   begin(CFDGPU_STAGE_RHIE_CHOW);
   hipLaunchKernelGGL(k_rhiechow, grid, block, 0, s, d);
   end(CFDGPU_STAGE_RHIE_CHOW);
@@ -153,10 +157,14 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 0, 0);
         hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 1, last);
       }
+      end(CFDGPU_STAGE_SST_SWEEPS);
     } else if (lex) {
       if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
       else hipLaunchKernelGGL(k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      end(CFDGPU_STAGE_SST_SWEEPS);
+      begin(CFDGPU_STAGE_SWEEP_COPY);
       hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 1, 3);
+      end(CFDGPU_STAGE_SWEEP_COPY);
     } else {
       for (int sw = 0; sw < 3; ++sw) {
         const int floor = (sw == 2) ? 1 : 0;
@@ -165,8 +173,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 2, 0, 0);
         hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 2, 1, floor);
       }
+      end(CFDGPU_STAGE_SST_SWEEPS);
     }
-    end(CFDGPU_STAGE_SST_SWEEPS);
     begin(CFDGPU_STAGE_SST_WALL_VISCOSITY);
     hipLaunchKernelGGL(k_sst_wall, dim3(1, cfg.batch), block, 0, s, d);
     hipLaunchKernelGGL(k_sst_mut, grid, block, 0, s, d);

@@ -72,7 +72,8 @@ enum {
   CFDGPU_STAGE_SST_SWEEPS = 9,         /* 3 Gauss-Seidel sweeps of k and of omega */
   CFDGPU_STAGE_SST_WALL_VISCOSITY = 10,/* wall omega + eddy viscosity + effective viscosity */
   CFDGPU_STAGE_MONITOR = 11,           /* reductions, SolverMonitor, stop rule */
-  CFDGPU_STAGE_COUNT = 12
+  CFDGPU_STAGE_SWEEP_COPY = 12,        /* skewed sweep results copied to the fields (+ k, omega floors) */
+  CFDGPU_STAGE_COUNT = 13
 };
 
 typedef struct cfdgpu_mesh cfdgpu_mesh;
