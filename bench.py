@@ -249,6 +249,63 @@ def run_solve(args):
                       "tolerance": args.tolerance, "cl": f.cl, "cd": f.cd, "cm": f.cm}), flush=True)
 
 
+def run_batch_polar(args):
+    """The batched uniform polar: NACA 0012 on a 512 x 128 C-grid (surfacePoints
+    193, wakePoints 64, normalPoints 129, first layer 2.5e-5 chord), k-omega SST,
+    Re 1e6, Tu 0.1 %, mu_t/mu 10, alpha -4..16 deg in 1 deg steps (21 solves),
+    every point started cold from the uniform field and advanced together, one
+    launch per stage for all 21 (cfdgpu_batch_solve). Reports converged
+    solves/s and each point's coefficients."""
+    import cfdgpu as G
+
+    opts = G.cgrid_options("coarse", upstream_chords=15.0, vertical_chords=15.0, downstream_chords=20.0,
+                           surface_points=193, wake_points=64, normal_points=129, first_layer_height=2.5e-5)
+    mesh = G.cgrid(0, 0, 12, options=opts)
+    alphas = list(G.sweep_angles(-4.0, 16.0, 1.0))
+    solvers = []
+    t_setup = time.perf_counter()
+    for a in alphas:
+        kind, vel, pres = G.face_conditions(mesh, 50.0, a)
+        st = G.default_settings(intensity=0.001, viscosity_ratio=10.0)
+        s = G.SimpleSolver(mesh, kind, vel, pres, st)
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(mesh.cells, speed=50.0, alpha_deg=a, density=1.225, reynolds=1e6))
+        solvers.append(s)
+    t_setup = time.perf_counter() - t_setup
+    n = len(solvers)
+    iters, status, done_at = [0] * n, [1] * n, [None] * n
+    active = list(range(n))
+    t0 = time.perf_counter()
+    while active:
+        # members that stopped leave the batch; the rest go on together
+        batch = G.Batch([solvers[i] for i in active])
+        reps = batch.solve(args.tolerance, args.chunk, check_every=50)
+        now = time.perf_counter() - t0
+        for i, r in zip(active, reps):
+            iters[i] += r.iterations
+            if r.status != 1:
+                status[i], done_at[i] = r.status, now
+        active = [i for i in active if status[i] == 1 and iters[i] < args.max_iterations]
+        print(f"{now:8.1f}s active {len(active)} of {n}, iterations {min(iters)}..{max(iters)}", flush=True)
+        del batch
+    sec = time.perf_counter() - t0
+    points = []
+    for a, s, it, stt, t in zip(alphas, solvers, iters, status, done_at):
+        u, v, p, mu = s.field()
+        f, _ = G.surface(mesh, u, v, p, mu, speed=50.0, alpha_deg=a, density=1.225)
+        points.append({"alpha": a, "status": stt, "iterations": it, "seconds_to_stop": t, "cl": f.cl, "cd": f.cd,
+                       "cm": f.cm})
+    converged = sum(1 for x in status if x == 0)
+    lat = sorted(t for t, x in zip(done_at, status) if x == 0)
+    print(json.dumps({"workload": "batched uniform polar, NACA 0012 512x128, SST Re 1e6, alpha -4..16 (21 cold)",
+                      "cells_per_solve": mesh.cells, "solves": len(alphas), "converged": converged,
+                      "seconds": sec, "setup_seconds": t_setup, "converged_solves_per_s": converged / sec,
+                      "time_to_convergence_p50_s": lat[len(lat) // 2] if lat else None,
+                      "time_to_convergence_p99_s": lat[min(len(lat) - 1, int(0.99 * len(lat)))] if lat else None,
+                      "tolerance": args.tolerance, "max_iterations": args.max_iterations, "points": points}),
+          flush=True)
+
+
 def run_calibrate_copy(args):
     """A device-to-device copy of known size, for scaling the HBM traffic
     counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) to bytes."""
@@ -278,6 +335,7 @@ def main():
     ap.add_argument("--solve", action="store_true", help="one solve to the residual tolerance, timed")
     ap.add_argument("--check-every", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true", help="a copy of known size (counter scaling)")
+    ap.add_argument("--batch-polar", action="store_true", help="21-point cold polar as one batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
@@ -285,6 +343,8 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         run_calibrate_copy(args)
+    elif args.batch_polar:
+        run_batch_polar(args)
     elif args.solve:
         run_solve(args)
     elif args.converge:
