@@ -16,6 +16,7 @@ __global__ void k_gs_lex_turb(const Inst*, int);
 __global__ void k_gs_lex_mom_wide(const Inst*, int);
 __global__ void k_gs_lex_turb_wide(const Inst*, int);
 __global__ void k_gs_unskew(const Inst*, int, int);
+__global__ void k_wall_distance(const double2*, int, const double2*, int, double*);
 __global__ void k_gs_color(const Inst*, int, int, int);
 __global__ void k_rhiechow(const Inst*);
 __global__ void k_pressure(const Inst*, int);
@@ -194,6 +195,11 @@ __global__ void k_resume(const Inst* insts, int count) {
 
 void launchResumeAll(const Inst* d, int count, hipStream_t s) {
   hipLaunchKernelGGL(k_resume, dim3((count + 63) / 64), dim3(64), 0, s, d, count);
+}
+
+void launchWallDistance(const double2* centroid, int n, const double2* walls, int nw, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_wall_distance, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, centroid, n, walls, nw,
+                     out);
 }
 
 void launchDivergence(const Inst* d, const LaunchCfg& cfg, double* out, hipStream_t s) {

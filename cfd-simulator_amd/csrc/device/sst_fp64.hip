@@ -386,4 +386,28 @@ __global__ void __launch_bounds__(kBlock) k_laminar_mu(const Inst* insts) {
   if (live) I.mu[c] = I.muMol[c] + 0.0;
 }
 
+// Wall distance for the SST set-up (TurbulenceModel.cpp:28-65): from each
+// cell centroid to the nearest wall-face centre, by brute force over the wall
+// faces, staged through LDS 256 at a time. The minimum is taken over squared
+// distances in wall-face order and the square root last, as on the host, so
+// the result is bit-identical to it.
+__global__ void __launch_bounds__(kBlock) k_wall_distance(const double2* centroid, int n, const double2* walls,
+                                                         int nw, double* out) {
+  __shared__ double2 tile[kBlock];
+  const int c = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+  const double2 p = (c < n) ? centroid[c] : make_double2(0.0, 0.0);
+  double best = 1.7976931348623157e308;  // DBL_MAX
+  for (int base = 0; base < nw; base += kBlock) {
+    __syncthreads();
+    if (base + static_cast<int>(threadIdx.x) < nw) tile[threadIdx.x] = walls[base + threadIdx.x];
+    __syncthreads();
+    const int m = min(kBlock, nw - base);
+    for (int q = 0; q < m; ++q) {
+      const double dx = p.x - tile[q].x, dy = p.y - tile[q].y;
+      best = mn(best, dx * dx + dy * dy);
+    }
+  }
+  if (c < n) out[c] = sqrt(best);
+}
+
 }  // namespace cfdgpu

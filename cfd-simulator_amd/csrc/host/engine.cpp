@@ -404,15 +404,24 @@ bool Engine::sstInitialise() {
     const double far = 1.0e6 * std::sqrt(std::abs(g.totalArea) + 1.0);
     std::fill(wallDist_.begin(), wallDist_.end(), far);
   } else {
-    for (size_t c = 0; c < N; ++c) {
-      const Vec2 cc = g.centroid[c];
-      double best = std::numeric_limits<double>::max();
-      for (const Vec2& w : walls) {
-        const double dx = cc.x - w.x, dy = cc.y - w.y;
-        best = std::min(best, dx * dx + dy * dy);
-      }
-      wallDist_[c] = std::sqrt(best);
-    }
+    // brute force on the device (O(cells x wall faces)); set-up only
+    std::vector<double2> cxy(N), wxy(walls.size());
+    for (size_t c = 0; c < N; ++c) cxy[c] = make_double2(g.centroid[c].x, g.centroid[c].y);
+    for (size_t w = 0; w < walls.size(); ++w) wxy[w] = make_double2(walls[w].x, walls[w].y);
+    double2 *dC = nullptr, *dW = nullptr;
+    double* dOut = nullptr;
+    check(hipMalloc(&dC, N * sizeof(double2)), "hipMalloc (wall distance)");
+    check(hipMalloc(&dW, wxy.size() * sizeof(double2)), "hipMalloc (wall distance)");
+    check(hipMalloc(&dOut, N * sizeof(double)), "hipMalloc (wall distance)");
+    check(hipMemcpyAsync(dC, cxy.data(), N * sizeof(double2), hipMemcpyHostToDevice, stream_), "upload");
+    check(hipMemcpyAsync(dW, wxy.data(), wxy.size() * sizeof(double2), hipMemcpyHostToDevice, stream_), "upload");
+    launchWallDistance(dC, static_cast<int>(N), dW, static_cast<int>(wxy.size()), dOut, stream_);
+    check(hipGetLastError(), "launch (wall distance)");
+    check(hipMemcpyAsync(wallDist_.data(), dOut, N * sizeof(double), hipMemcpyDeviceToHost, stream_), "download");
+    check(hipStreamSynchronize(stream_), "wall distance sync");
+    (void)hipFree(dC);
+    (void)hipFree(dW);
+    (void)hipFree(dOut);
   }
   double refSpeed = 0.0, refRho = 0.0, refMu = 0.0;
   for (size_t c = 0; c < N; ++c) {

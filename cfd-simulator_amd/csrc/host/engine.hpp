@@ -39,6 +39,8 @@ struct StageRecorder {
 void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hipStream_t s,
                      StageRecorder* rec = nullptr);
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
+// distance from each of n centroids to the nearest of nw wall-face centres
+void launchWallDistance(const double2* centroid, int n, const double2* walls, int nw, double* out, hipStream_t s);
 // clears the stop flag of `count` instances (a plain iterate never stops by itself)
 void launchResumeAll(const Inst* dInst, int count, hipStream_t s);
 int gsThreadsFor(int nJ);
