@@ -38,7 +38,9 @@ EXPORTS = [
     "cfdgpu_batch_iterate", "cfdgpu_batch_solve", "cfdgpu_polar_sweep", "cfdgpu_polar_csv", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
     "cfdgpu_set_field", "cfdgpu_get_massflux", "cfdgpu_set_massflux", "cfdgpu_get_turbulence",
     "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_surface",
-    "cfdgpu_sweep_angles", "cfdgpu_debug_array",
+    "cfdgpu_sweep_angles", "cfdgpu_debug_array", "cfdgpu_server_defaults", "cfdgpu_server_create",
+    "cfdgpu_server_destroy", "cfdgpu_server_submit", "cfdgpu_server_step", "cfdgpu_server_run",
+    "cfdgpu_server_collect", "cfdgpu_server_fields",
 ]
 
 
@@ -123,6 +125,33 @@ class Forces(C.Structure):
                 ("upper_separation_face", C.c_int), ("lower_separation_face", C.c_int)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_slots", C.c_int), ("chunk_iterations", C.c_int),
+                ("rule", StopRule), ("settings", Settings), ("model", C.c_int), ("warm_start", C.c_int),
+                ("keep_fields", C.c_int)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("m", C.c_int), ("p", C.c_int), ("t", C.c_int), ("chord", C.c_double), ("mesh", CGridOptions)] + \
+               [(n, C.c_double) for n in ("alpha_deg", "reynolds", "speed", "density", "reference_pressure",
+                                          "arrival_s")] + \
+               [("settings", C.POINTER(Settings))]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ticket", "status", "iterations", "retried_cold", "warm_from", "cells")] + \
+               [(n, C.c_double) for n in ("cl", "cd", "cd_pressure", "cd_friction", "cm", "l_over_d",
+                                          "upper_separation", "lower_separation", "continuity", "arrival_s",
+                                          "admitted_s", "finished_s", "device_bytes")]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("clock_s", C.c_double)] + \
+               [(n, C.c_int) for n in ("queued", "active", "admitted", "finished", "converged", "peak_active",
+                                       "rounds", "meshes_cached")] + \
+               [(n, C.c_double) for n in ("resident_bytes", "peak_resident_bytes", "member_iterations")]
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
@@ -187,6 +216,14 @@ def load(path: str = LIB_PATH):
         "cfdgpu_polar_csv": (C.c_int, [C.POINTER(PolarPoint), C.c_int, C.POINTER(PolarHeader), C.c_char_p,
                                        C.c_int]),
         "cfdgpu_debug_array": (C.c_int, [vp, C.c_char_p, _dp, C.c_int]),
+        "cfdgpu_server_defaults": (None, [C.POINTER(ServerOptions)]),
+        "cfdgpu_server_create": (C.c_int, [C.POINTER(ServerOptions), pp]),
+        "cfdgpu_server_destroy": (None, [vp]),
+        "cfdgpu_server_submit": (C.c_int, [vp, C.POINTER(ServeRequest), C.POINTER(C.c_int)]),
+        "cfdgpu_server_step": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "cfdgpu_server_run": (C.c_int, [vp, C.c_double, C.POINTER(ServerStats)]),
+        "cfdgpu_server_collect": (C.c_int, [vp, C.POINTER(ServeResult), C.c_int, C.POINTER(C.c_int)]),
+        "cfdgpu_server_fields": (C.c_int, [vp, C.c_int, D, D, D, D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -435,6 +472,75 @@ class Batch:
         reps = (SolveReport * len(self.solvers))()
         _check(lib().cfdgpu_batch_solve(self.h, C.byref(rule), reps))
         return list(reps)
+
+
+STATUS_NAMES = {0: "converged", 1: "iteration-limit", 2: "diverged", 3: "rejected"}
+
+
+def server_options(**overrides) -> ServerOptions:
+    o = ServerOptions()
+    lib().cfdgpu_server_defaults(C.byref(o))
+    for k, v in overrides.items():
+        if k in ("tolerance", "max_iterations"):
+            setattr(o.rule, k, v)
+        elif k == "settings":
+            o.settings = v
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def serve_request(m: int, p: int, t: int, alpha_deg: float, reynolds: float, mesh: CGridOptions,
+                  arrival_s: float = 0.0, chord=1.0, speed=1.0, density=1.0, reference_pressure=0.0,
+                  settings: Settings | None = None) -> ServeRequest:
+    r = ServeRequest(m, p, t, chord, mesh, alpha_deg, reynolds, speed, density, reference_pressure, arrival_s)
+    if settings is not None:
+        r._settings = settings  # kept alive with the request
+        r.settings = C.pointer(settings)
+    return r
+
+
+class Server:
+    """The request loop of polar-point solves with continuous batching
+    (cfdgpu_server_*): requests are admitted as they arrive, by memory budget
+    and slot count, and advanced together on the device in chunks."""
+
+    def __init__(self, options: ServerOptions | None = None, **overrides):
+        self.options = options if options is not None else server_options(**overrides)
+        h = C.c_void_p()
+        _check(lib().cfdgpu_server_create(C.byref(self.options), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cfdgpu_server_destroy(self.h)
+            self.h = None
+
+    def submit(self, request: ServeRequest) -> int:
+        t = C.c_int()
+        _check(lib().cfdgpu_server_submit(self.h, C.byref(request), C.byref(t)))
+        return t.value
+
+    def step(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().cfdgpu_server_step(self.h, C.byref(st)))
+        return st
+
+    def run(self, wall_limit_s: float = 0.0) -> ServerStats:
+        st = ServerStats()
+        _check(lib().cfdgpu_server_run(self.h, wall_limit_s, C.byref(st)))
+        return st
+
+    def collect(self, capacity: int = 4096):
+        out = (ServeResult * capacity)()
+        n = C.c_int()
+        _check(lib().cfdgpu_server_collect(self.h, out, capacity, C.byref(n)))
+        return list(out[:n.value])
+
+    def fields(self, ticket: int, cells: int):
+        arrs = [np.zeros(cells) for _ in range(4)]
+        _check(lib().cfdgpu_server_fields(self.h, ticket, *[_opt(a) for a in arrs]))
+        return tuple(arrs)
 
 
 def surface(mesh: Mesh, u, v, p, mu, speed=50.0, alpha_deg=0.0, density=1.225, reference_pressure=0.0, chord=1.0):

@@ -14,6 +14,7 @@
 #include "host/engine.hpp"
 #include "host/polar.hpp"
 #include "host/post.hpp"
+#include "host/serve.hpp"
 
 namespace {
 
@@ -512,6 +513,110 @@ int cfdgpu_polar_sweep(const cfdgpu_mesh* m, const cfdgpu_settings* settings, in
     std::vector<cfdgpu_polar_point> pts;
     cfdgpu::polarSweep(rq, pts);
     for (size_t q = 0; q < pts.size(); ++q) out[q] = pts[q];
+    return CFDGPU_OK;
+  });
+}
+
+void cfdgpu_server_defaults(cfdgpu_server_options* o) {
+  if (!o) return;
+  *o = cfdgpu_server_options{};
+  o->hbm_budget_bytes = 200e9;
+  o->max_slots = 64;
+  o->chunk_iterations = 20;
+  o->rule = cfdgpu_stop_rule{1e-6, 10000, 1};
+  cfdgpu_default_settings(&o->settings);
+  o->model = CFDGPU_MODEL_SST;
+  o->warm_start = 1;
+  o->keep_fields = 0;
+}
+
+int cfdgpu_server_create(const cfdgpu_server_options* o, cfdgpu_server** out) {
+  if (!o || !out) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  if (const int rc = checkPathOptions(o->settings); rc != CFDGPU_OK) return rc;
+  return guarded([&]() -> int {
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
+      return fail(CFDGPU_INITIALIZATION_FAILURE, "no HIP device: the MI355X path needs a GPU");
+    cfdgpu::ServerOptions so;
+    so.budgetBytes = o->hbm_budget_bytes;
+    so.maxSlots = o->max_slots;
+    so.chunk = o->chunk_iterations;
+    so.rule = o->rule;
+    so.settings = o->settings;
+    so.model = o->model;
+    so.warmStart = o->warm_start != 0;
+    so.keepFields = o->keep_fields != 0;
+    auto* s = new cfdgpu_server();
+    try {
+      s->server = std::make_unique<cfdgpu::Server>(so);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+    return CFDGPU_OK;
+  });
+}
+
+void cfdgpu_server_destroy(cfdgpu_server* s) { delete s; }
+
+int cfdgpu_server_submit(cfdgpu_server* s, const cfdgpu_serve_request* r, int* ticket) {
+  if (!s || !r) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  return guarded([&]() -> int {
+    cfdgpu::ServeRequest q;
+    q.m = r->m;
+    q.p = r->p;
+    q.t = r->t;
+    q.chord = r->chord;
+    q.mesh = toOptions(r->mesh);
+    q.alphaDeg = r->alpha_deg;
+    q.reynolds = r->reynolds;
+    q.speed = r->speed;
+    q.density = r->density;
+    q.referencePressure = r->reference_pressure;
+    q.arrival = r->arrival_s;
+    if (r->settings) {
+      if (const int rc = checkPathOptions(*r->settings); rc != CFDGPU_OK) return rc;
+      q.ownSettings = true;
+      q.settings = *r->settings;
+    }
+    const int t = s->server->submit(q);
+    if (ticket) *ticket = t;
+    return CFDGPU_OK;
+  });
+}
+
+int cfdgpu_server_step(cfdgpu_server* s, cfdgpu_server_stats* out) {
+  if (!s) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  return guarded([&]() -> int {
+    s->server->step();
+    if (out) s->server->stats(out);
+    return CFDGPU_OK;
+  });
+}
+
+int cfdgpu_server_run(cfdgpu_server* s, double wall_limit_s, cfdgpu_server_stats* out) {
+  if (!s) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  return guarded([&]() -> int {
+    s->server->run(wall_limit_s);
+    if (out) s->server->stats(out);
+    return CFDGPU_OK;
+  });
+}
+
+int cfdgpu_server_collect(cfdgpu_server* s, cfdgpu_serve_result* out, int capacity, int* count) {
+  if (!s || !count || (capacity > 0 && !out) || capacity < 0) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  return guarded([&]() -> int {
+    *count = s->server->collect(out, capacity);
+    return CFDGPU_OK;
+  });
+}
+
+int cfdgpu_server_fields(const cfdgpu_server* s, int ticket, double* u, double* v, double* p, double* mu) {
+  if (!s) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  return guarded([&]() -> int {
+    if (!s->server->fields(ticket, u, v, p, mu))
+      return fail(CFDGPU_NOT_FOUND, "no kept fields for this ticket (keep_fields off, unfinished or unknown)");
     return CFDGPU_OK;
   });
 }

@@ -215,11 +215,15 @@ void Engine::precomputeGeometry() {
   }
 }
 
-void Engine::allocate() {
-  const size_t N = static_cast<size_t>(grid_.cells());
-  const size_t nJf = static_cast<size_t>(grid_.jFaces()), nIf = static_cast<size_t>(grid_.iFaces());
-  const size_t nI = static_cast<size_t>(grid_.nI), nJ = static_cast<size_t>(grid_.nJ);
-  auto layout = [&](Carver& cv, Inst& I) {
+namespace {
+
+// Every device array of one instance, carved in a fixed order; a sizing pass
+// (base == nullptr) gives the arena's byte count.
+void carveInst(Carver& cv, Inst& I, const Grid& grid) {
+  const size_t N = static_cast<size_t>(grid.cells());
+  const size_t nJf = static_cast<size_t>(grid.jFaces()), nIf = static_cast<size_t>(grid.iFaces());
+  const size_t nI = static_cast<size_t>(grid.nI), nJ = static_cast<size_t>(grid.nJ);
+  {
     I.V = cv.take<double>(N);
     I.jg = cv.take<FaceGeo>(nJf);
     I.ig = cv.take<FaceGeo>(nIf);
@@ -245,7 +249,7 @@ void Engine::allocate() {
                          &I.fkP, &I.fkS, &I.fkN, &I.fkW, &I.fkE, &I.fkR, &I.fkD, &I.fwP, &I.fwS, &I.fwN,
                          &I.fwW, &I.fwE, &I.fwR, &I.fwD, &I.fLinv, &I.fLc};
     for (float** a : fArrays) *a = cv.take<float>(N);
-    const size_t skew = static_cast<size_t>(skewLength(grid_.nI, grid_.nJ, gsThreadsFor(grid_.nJ)));
+    const size_t skew = static_cast<size_t>(skewLength(grid.nI, grid.nJ, gsThreadsFor(grid.nJ)));
     for (double2*& a : I.skMomRow) a = cv.take<double2>(skew);
     I.skMomBy = cv.take<double>(skew);
     for (double2*& a : I.skMomX) a = cv.take<double2>(skew);
@@ -260,16 +264,28 @@ void Engine::allocate() {
     I.monitor = cv.take<DevMonitor>(1);
     I.historyCap = 0;
     I.history = nullptr;
-  };
+  }
+}
+
+}  // namespace
+
+size_t Engine::deviceBytesFor(const Grid& grid) {
   Carver sizing;
   Inst dummy{};
-  layout(sizing, dummy);
+  carveInst(sizing, dummy, grid);
+  return sizing.offset + sizeof(Inst);
+}
+
+void Engine::allocate() {
+  Carver sizing;
+  Inst dummy{};
+  carveInst(sizing, dummy, grid_);
   arena_.bytes = sizing.offset;
   check(hipMalloc(&arena_.base, arena_.bytes), "hipMalloc (solver state)");
   check(hipMemsetAsync(arena_.base, 0, arena_.bytes, stream_), "hipMemset");
   Carver real;
   real.base = static_cast<char*>(arena_.base);
-  layout(real, h_);
+  carveInst(real, h_, grid_);
   h_.nI = grid_.nI;
   h_.nJ = grid_.nJ;
   h_.N = grid_.cells();

@@ -85,6 +85,9 @@ class Engine {
 
   const Grid& grid() const { return grid_; }
   bool ready() const { return initialised_; }
+  // device memory one solver on this mesh holds (state arena + descriptor)
+  static size_t deviceBytesFor(const Grid& grid);
+  size_t deviceBytes() const { return arena_.bytes + sizeof(Inst); }
   hipStream_t stream() const { return stream_; }
   // for batching: the instance descriptor with device pointers, refreshed from
   // the current settings, and the monitor read on another stream

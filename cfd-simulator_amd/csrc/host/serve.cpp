@@ -1,0 +1,411 @@
+// serve.cpp - the request loop: admission by memory budget, device-resident
+// solves advanced together in chunks, retirement with the reference polar's
+// per-point semantics (Polar.cpp:79-124, Panels.cpp:1137-1157, 1723-1730).
+#include "serve.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <thread>
+
+#include "polar.hpp"
+#include "post.hpp"
+
+namespace cfdgpu {
+
+namespace {
+
+constexpr size_t kMeshCache = 32;                 // meshes kept for later requests
+constexpr size_t kDonorsPerKey = 16;              // converged fields kept per section and mesh size
+constexpr size_t kDonorBytes = size_t(2) << 30;   // host bytes of kept warm-start fields
+
+void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+struct Server::Slot {
+  int ticket{-1};
+  std::shared_ptr<const Grid> grid;
+  std::unique_ptr<Engine> engine;
+  bool retried{false};
+  int warmFrom{-1};
+  double admitted{0.0};
+  size_t bytes{0};
+  int lastIteration{0};
+};
+
+Server::Server(const ServerOptions& options) : opt_(options), t0_(std::chrono::steady_clock::now()) {
+  if (opt_.maxSlots < 1 || opt_.maxSlots > 65535) throw std::invalid_argument("max_slots must be in 1..65535");
+  if (opt_.chunk < 1) throw std::invalid_argument("chunk_iterations must be at least 1");
+  if (!(opt_.budgetBytes > 0.0)) throw std::invalid_argument("the memory budget must be positive");
+  if (opt_.model != CFDGPU_MODEL_NONE && opt_.model != CFDGPU_MODEL_LAMINAR && opt_.model != CFDGPU_MODEL_SST)
+    throw std::invalid_argument("unknown turbulence model");
+  hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hipCheck(hipMalloc(&dInsts_, sizeof(Inst) * static_cast<size_t>(opt_.maxSlots)), "hipMalloc (roster)");
+  hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hFlags_), sizeof(int) * kIFlags * static_cast<size_t>(opt_.maxSlots),
+                         hipHostMallocDefault),
+           "hipHostMalloc (flags)");
+}
+
+Server::~Server() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  active_.clear();
+  joining_.clear();
+  if (hFlags_) (void)hipHostFree(hFlags_);
+  if (dInsts_) (void)hipFree(dInsts_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+double Server::clock() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+int Server::submit(const ServeRequest& r) {
+  if (r.m < 0 || r.m > 9 || r.p < 0 || r.p > 9 || r.t < 1 || r.t > 40)
+    throw std::invalid_argument("NACA digits out of range");
+  if (!(r.chord > 0.0) || !(r.reynolds > 0.0) || !(r.speed > 0.0) || !(r.density > 0.0))
+    throw std::invalid_argument("chord, Reynolds number, speed and density must be positive");
+  if (!std::isfinite(r.alphaDeg) || !std::isfinite(r.arrival)) throw std::invalid_argument("non-finite request");
+  if (r.ownSettings) {
+    const cfdgpu_settings& a = r.settings;
+    const cfdgpu_settings& b = opt_.settings;
+    if (a.momentum_sweeps != b.momentum_sweeps || a.pressure_iterations != b.pressure_iterations ||
+        a.non_orthogonal_correctors != b.non_orthogonal_correctors || a.gs_ordering != b.gs_ordering ||
+        a.precision != b.precision || a.pressure_solver != b.pressure_solver)
+      throw std::invalid_argument("a request's sweeps, CG cap, correctors, ordering, precision and pressure solver "
+                                  "must be the server's (they shape the shared launches)");
+  }
+  const int ticket = static_cast<int>(requests_.size());
+  requests_.push_back(r);
+  // arrival order; ties keep submission order
+  auto at = std::upper_bound(pending_.begin(), pending_.end(), r.arrival,
+                             [&](double a, int q) { return a < requests_[static_cast<size_t>(q)].arrival; });
+  pending_.insert(at, ticket);
+  return ticket;
+}
+
+// One mesh per (section, C-grid options), built on first use and kept while
+// any resident solve refers to it or it is among the most recent.
+std::shared_ptr<const Grid> Server::meshFor(const ServeRequest& r) {
+  const CGridOptions& o = r.mesh;
+  const MeshKey key{r.m, r.p, r.t, r.chord, o.upstream, o.downstream, o.vertical, o.surfacePoints,
+                    o.wakePoints, o.normalPoints, o.firstLayer, o.blendLength, o.smoothingPasses};
+  for (auto it = meshes_.begin(); it != meshes_.end(); ++it) {
+    if (it->first == key) {
+      meshes_.splice(meshes_.begin(), meshes_, it);
+      return meshes_.front().second;
+    }
+  }
+  auto g = std::make_shared<Grid>();
+  std::string err;
+  if (!makeCGrid(*g, r.m, r.p, r.t, r.chord, o, err)) throw std::invalid_argument(err);
+  meshes_.emplace_front(key, g);
+  // drop the oldest meshes no solve holds
+  for (auto it = meshes_.end(); meshes_.size() > kMeshCache && it != meshes_.begin();) {
+    --it;
+    if (it->second.use_count() == 1) it = meshes_.erase(it);
+  }
+  return g;
+}
+
+// A new solver for the slot's request: the uniform field of its freestream
+// (FlowField.cpp:21-47), with velocity and pressure taken from the nearest
+// converged alpha of the same section and mesh size unless `cold`; the closure
+// is attached and initialised cold either way (Panels.cpp:1274-1306).
+void Server::startAttempt(Slot& s, bool cold) {
+  const ServeRequest& r = requests_[static_cast<size_t>(s.ticket)];
+  const Grid& g = *s.grid;
+  const int N = g.cells(), F = g.faces();
+  std::vector<int> kind(static_cast<size_t>(F));
+  std::vector<double> vxy(2 * static_cast<size_t>(F)), pres(static_cast<size_t>(F));
+  defaultFaceConditions(g, r.speed, r.alphaDeg, r.referencePressure, kind.data(), vxy.data(), pres.data());
+  std::vector<Vec2> vel(static_cast<size_t>(F));
+  for (int f = 0; f < F; ++f) vel[static_cast<size_t>(f)] = Vec2{vxy[2 * static_cast<size_t>(f)], vxy[2 * static_cast<size_t>(f) + 1]};
+  std::vector<double> u(static_cast<size_t>(N)), v(u.size()), p(u.size()), rho(u.size()), mu(u.size());
+  uniformField(N, r.speed, r.alphaDeg, r.density, r.reynolds, r.chord, r.referencePressure, u.data(), v.data(),
+               p.data(), rho.data(), mu.data());
+  s.warmFrom = -1;
+  if (!cold && opt_.warmStart) {
+    const DonorKey key{r.m, r.p, r.t, r.chord, g.nI, g.nJ, r.speed, r.density};
+    auto it = donors_.find(key);
+    if (it != donors_.end() && !it->second.empty()) {
+      const Donor* best = nullptr;
+      double bestA = 0.0, bestR = 0.0;
+      for (const Donor& d : it->second) {
+        const double da = std::fabs(d.alpha - r.alphaDeg);
+        const double dr = std::fabs(std::log(d.reynolds / r.reynolds));
+        if (!best || da < bestA || (da == bestA && dr < bestR)) {
+          best = &d;
+          bestA = da;
+          bestR = dr;
+        }
+      }
+      u = best->u;
+      v = best->v;
+      p = best->p;
+      s.warmFrom = best->ticket;
+    }
+  }
+  // a retry replaces the diverged solver once the new one is ready
+  auto e = std::make_unique<Engine>(g, std::move(kind), std::move(vel), std::move(pres),
+                                    r.ownSettings ? r.settings : opt_.settings);
+  if (opt_.model != CFDGPU_MODEL_NONE && e->setModel(opt_.model, nullptr) != CFDGPU_OK)
+    throw std::invalid_argument("unknown turbulence model");
+  if (e->initialise(u.data(), v.data(), p.data(), rho.data(), mu.data()) != CFDGPU_OK)
+    throw std::runtime_error("the solver could not start");
+  Inst& h = e->hostInst();
+  h.stopTol = opt_.rule.tolerance;
+  h.stopMaxIter = opt_.rule.max_iterations > 0 ? opt_.rule.max_iterations : 0;
+  hipCheck(hipStreamSynchronize(e->stream()), "solver set-up");
+  s.bytes = e->deviceBytes();
+  s.lastIteration = 0;
+  s.engine = std::move(e);
+}
+
+// Admits the earliest queued request if it has arrived, a slot is free and
+// its device state fits the budget. Requests are admitted in arrival order.
+bool Server::admitOne() {
+  if (pending_.empty()) return false;
+  const int ticket = pending_.front();
+  const ServeRequest& r = requests_[static_cast<size_t>(ticket)];
+  if (r.arrival > clock()) return false;
+  if (static_cast<int>(active_.size() + joining_.size()) >= opt_.maxSlots) return false;
+  auto s = std::make_unique<Slot>();
+  s->ticket = ticket;
+  try {
+    s->grid = meshFor(r);
+    const size_t need = Engine::deviceBytesFor(*s->grid);
+    if (static_cast<double>(need) > opt_.budgetBytes) throw std::invalid_argument("exceeds the memory budget");
+    if (resident_ + static_cast<double>(need) > opt_.budgetBytes) return false;  // waits for memory
+    pending_.pop_front();
+    s->admitted = clock();
+    startAttempt(*s, false);
+  } catch (const std::exception&) {
+    if (!pending_.empty() && pending_.front() == ticket) pending_.pop_front();
+    cfdgpu_serve_result res{};
+    res.ticket = ticket;
+    res.status = 3;
+    res.warm_from = -1;
+    res.upper_separation = res.lower_separation = -1.0;
+    res.arrival_s = r.arrival;
+    res.admitted_s = res.finished_s = clock();
+    res.cells = s->grid ? s->grid->cells() : 0;
+    done_.push_back(res);
+    ++finished_;
+    return true;
+  }
+  resident_ += static_cast<double>(s->bytes);
+  peakResident_ = std::max(peakResident_, resident_);
+  ++admitted_;
+  joining_.push_back(std::move(s));
+  return true;
+}
+
+void Server::finish(Slot& s, int status, int iterations, double continuity) {
+  const ServeRequest& r = requests_[static_cast<size_t>(s.ticket)];
+  const Grid& g = *s.grid;
+  const size_t N = static_cast<size_t>(g.cells());
+  std::vector<double> u(N), v(N), p(N), mu(N);
+  s.engine->getField(u.data(), v.data(), p.data(), mu.data());
+  cfdgpu_serve_result res{};
+  res.ticket = s.ticket;
+  res.status = status;
+  res.iterations = iterations;
+  res.retried_cold = s.retried ? 1 : 0;
+  res.warm_from = s.warmFrom;
+  res.cells = g.cells();
+  res.continuity = continuity;
+  res.upper_separation = res.lower_separation = -1.0;
+  std::vector<double> rows(12 * static_cast<size_t>(g.nI));
+  cfdgpu_forces f{};
+  int count = 0;
+  if (surfaceAndForces(g, u.data(), v.data(), p.data(), mu.data(), r.speed, r.alphaDeg, r.density,
+                       r.referencePressure, r.chord, rows.data(), g.nI, &count, &f) == 0) {
+    res.cl = f.cl;
+    res.cd = f.cd;
+    res.cd_pressure = f.cd_pressure;
+    res.cd_friction = f.cd_friction;
+    res.cm = f.cm;
+    res.l_over_d = (std::isfinite(f.cd) && f.cd > 0.0) ? f.cl / f.cd : 0.0;  // Forces.cpp:50-56
+    res.upper_separation = f.upper_separation;
+    res.lower_separation = f.lower_separation;
+  }
+  res.arrival_s = r.arrival;
+  res.admitted_s = s.admitted;
+  res.finished_s = clock();
+  res.device_bytes = static_cast<double>(s.bytes);
+  done_.push_back(res);
+  ++finished_;
+  if (status == 0) ++converged_;
+  if (opt_.keepFields) {
+    std::vector<double> all;
+    all.reserve(4 * N);
+    for (const auto* a : {&u, &v, &p, &mu}) all.insert(all.end(), a->begin(), a->end());
+    kept_[s.ticket] = std::move(all);
+  }
+  if (status == 0 && opt_.warmStart) {
+    const DonorKey key{r.m, r.p, r.t, r.chord, g.nI, g.nJ, r.speed, r.density};
+    auto& list = donors_[key];
+    list.push_back(Donor{r.alphaDeg, r.reynolds, s.ticket, std::move(u), std::move(v), std::move(p)});
+    donorBytes_ += 3 * N * sizeof(double);
+    donorOrder_.emplace_back(key, s.ticket);
+    if (list.size() > kDonorsPerKey) {
+      donorBytes_ -= 3 * list.front().u.size() * sizeof(double);
+      list.erase(list.begin());
+    }
+    while (donorBytes_ > kDonorBytes && !donorOrder_.empty()) {
+      auto [k, t] = donorOrder_.front();
+      donorOrder_.pop_front();
+      auto it = donors_.find(k);
+      if (it == donors_.end()) continue;
+      auto& l = it->second;
+      auto d = std::find_if(l.begin(), l.end(), [&](const Donor& x) { return x.ticket == t; });
+      if (d == l.end()) continue;
+      donorBytes_ -= 3 * d->u.size() * sizeof(double);
+      l.erase(d);
+    }
+  }
+  resident_ -= static_cast<double>(s.bytes);
+  s.engine.reset();
+}
+
+// Queues `chunk` outer iterations for every resident solve, one launch per
+// stage for all of them, then the read-back of each one's stop flags.
+void Server::launchChunk() {
+  if (active_.empty()) return;
+  if (rosterDirty_) {
+    host_.resize(active_.size());
+    int lanes = 64, blocks = 1, lineBlocks = 1, maxNI = 0;
+    for (size_t q = 0; q < active_.size(); ++q) {
+      host_[q] = active_[q]->engine->refreshInst();
+      lanes = std::max(lanes, host_[q].gsR);
+      blocks = std::max(blocks, host_[q].nBlocks);
+      lineBlocks = std::max(lineBlocks, host_[q].lineBlocks);
+      maxNI = std::max(maxNI, host_[q].nI);
+    }
+    cfg_ = LaunchCfg{blocks, lineBlocks, static_cast<int>(active_.size()), lanes, gsLdsFor(0, maxNI, lanes)};
+    if (opt_.settings.gs_ordering == CFDGPU_GS_LEXICOGRAPHIC && cfg_.gsLds > 64 * 1024)
+      throw std::invalid_argument("resident meshes too wide for the lexicographic sweep's LDS");
+    hipCheck(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),
+             "upload roster");
+    rosterDirty_ = false;
+  }
+  for (int n = 0; n < opt_.chunk; ++n) launchIteration(dInsts_, cfg_, host_.front(), stream_);
+  hipCheck(hipGetLastError(), "launch");
+  for (size_t q = 0; q < host_.size(); ++q)
+    hipCheck(hipMemcpyAsync(hFlags_ + q * kIFlags, host_[q].flags, sizeof(int) * kIFlags, hipMemcpyDeviceToHost,
+                            stream_),
+             "read flags");
+  launched_ = true;
+}
+
+// Waits for the chunk; solves that stopped leave the roster (a first
+// divergence is retried from the undisturbed stream in the same slot).
+void Server::retire() {
+  hipCheck(hipStreamSynchronize(stream_), "chunk");
+  launched_ = false;
+  ++rounds_;
+  std::vector<std::unique_ptr<Slot>> keep;
+  keep.reserve(active_.size());
+  for (size_t q = 0; q < active_.size(); ++q) {
+    Slot& s = *active_[q];
+    const int* fl = hFlags_ + q * kIFlags;
+    memberIterations_ += fl[fIteration] - s.lastIteration;
+    s.lastIteration = fl[fIteration];
+    if (!fl[fStopped]) {
+      keep.push_back(std::move(active_[q]));
+      continue;
+    }
+    rosterDirty_ = true;
+    const int status = fl[fStatus];
+    cfdgpu_monitor mon{};
+    s.engine->readMonitorOn(s.engine->stream(), &mon);
+    if (status == 2 && !s.retried) {
+      // Polar.cpp:112-117: one retry from the undisturbed stream
+      s.retried = true;
+      const size_t before = s.bytes;
+      bool restarted = true;
+      try {
+        startAttempt(s, true);
+      } catch (const std::exception&) {
+        restarted = false;  // reported as diverged below
+      }
+      if (restarted) {
+        resident_ += static_cast<double>(s.bytes) - static_cast<double>(before);
+        peakResident_ = std::max(peakResident_, resident_);
+        keep.push_back(std::move(active_[q]));
+        continue;
+      }
+    }
+    finish(s, status, fl[fIteration], mon.continuity);
+  }
+  active_ = std::move(keep);
+}
+
+void Server::step() {
+  launchChunk();
+  // admission overlaps the chunk running on the device
+  while (admitOne()) {
+  }
+  if (launched_) retire();
+  if (!joining_.empty()) {
+    for (auto& s : joining_) active_.push_back(std::move(s));
+    joining_.clear();
+    rosterDirty_ = true;
+  }
+  peakActive_ = std::max(peakActive_, static_cast<int>(active_.size()));
+  if (active_.empty() && !pending_.empty()) {
+    // idle until the next arrival
+    const double wait = requests_[static_cast<size_t>(pending_.front())].arrival - clock();
+    if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.02)));
+  }
+}
+
+void Server::run(double wallLimit) {
+  while (!pending_.empty() || !active_.empty() || !joining_.empty()) {
+    if (wallLimit > 0.0 && clock() >= wallLimit) break;
+    step();
+  }
+}
+
+int Server::collect(cfdgpu_serve_result* out, int capacity) {
+  const int n = std::min(capacity, static_cast<int>(done_.size()));
+  for (int q = 0; q < n; ++q) out[q] = done_[static_cast<size_t>(q)];
+  done_.erase(done_.begin(), done_.begin() + n);
+  return n;
+}
+
+bool Server::fields(int ticket, double* u, double* v, double* p, double* mu) const {
+  auto it = kept_.find(ticket);
+  if (it == kept_.end()) return false;
+  const size_t N = it->second.size() / 4;
+  double* dst[4] = {u, v, p, mu};
+  for (int a = 0; a < 4; ++a)
+    if (dst[a]) std::copy_n(it->second.begin() + static_cast<std::ptrdiff_t>(a * N), N, dst[a]);
+  return true;
+}
+
+int Server::cellsOf(int ticket) const {
+  auto it = kept_.find(ticket);
+  return it == kept_.end() ? -1 : static_cast<int>(it->second.size() / 4);
+}
+
+void Server::stats(cfdgpu_server_stats* out) const {
+  out->clock_s = clock();
+  out->queued = static_cast<int>(pending_.size());
+  out->active = static_cast<int>(active_.size() + joining_.size());
+  out->admitted = admitted_;
+  out->finished = finished_;
+  out->converged = converged_;
+  out->peak_active = peakActive_;
+  out->rounds = rounds_;
+  out->resident_bytes = resident_;
+  out->peak_resident_bytes = peakResident_;
+  out->member_iterations = static_cast<double>(memberIterations_);
+  out->meshes_cached = static_cast<int>(meshes_.size());
+}
+
+}  // namespace cfdgpu

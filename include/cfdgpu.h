@@ -288,6 +288,82 @@ int cfdgpu_polar_csv(const cfdgpu_polar_point* points, int count, const cfdgpu_p
 /* post::sweepAngles (Polar.cpp:36-77): returns count, or -status on error */
 int cfdgpu_sweep_angles(double start, double end, double step, double* out, int capacity);
 
+/* ---- serving: a request loop for polar-point solves with continuous batching.
+ *      No reference counterpart as a service; the per-solve semantics are the
+ *      reference polar's (app/Panels.cpp:1137-1157, 1581-1800; post/Polar.cpp:79-124).
+ *      A request is admitted once its arrival time has passed on the server
+ *      clock (seconds since cfdgpu_server_create), a slot is free and its device
+ *      state fits the memory budget, in arrival order. Resident solves are
+ *      advanced together, one launch per SIMPLE stage for all of them, in chunks
+ *      of outer iterations; between chunks stopped solves leave and arrivals
+ *      join. Each solve runs the worker's stop rule (SolverWorker.cpp:161-192);
+ *      a first divergence is retried once from the undisturbed stream
+ *      (Polar.cpp:112-117). With warm_start, a new request starts from the
+ *      velocity and pressure of the converged solve of the same section, chord,
+ *      mesh size, speed and density nearest in alpha (then in log Re); the
+ *      closure starts cold. A result is bit-identical to the same request served
+ *      alone when it starts from the same field. One server per host thread. ---- */
+typedef struct cfdgpu_server cfdgpu_server;
+
+typedef struct {
+  double hbm_budget_bytes; /* device bytes the resident solves may hold */
+  int max_slots;           /* most resident solves (1..65535) */
+  int chunk_iterations;    /* outer iterations between two scheduling rounds */
+  cfdgpu_stop_rule rule;   /* tolerance and iteration limit of every solve (check_every unused) */
+  cfdgpu_settings settings;
+  int model;               /* CFDGPU_MODEL_*, default constants */
+  int warm_start;          /* 1: warm start from converged neighbours in alpha */
+  int keep_fields;         /* 1: keep each result's u, v, p, mu for cfdgpu_server_fields */
+} cfdgpu_server_options;
+
+typedef struct {
+  int m, p, t;                 /* NACA four-digit section (closed trailing edge) */
+  double chord;
+  cfdgpu_cgrid_options mesh;   /* its C-grid */
+  double alpha_deg, reynolds, speed, density, reference_pressure;
+  double arrival_s;            /* server-clock arrival time; 0 = at once */
+  const cfdgpu_settings* settings; /* NULL: the server's. Relaxation, scheme, tolerances and inflow
+                                      turbulence may differ; sweeps, CG cap, correctors, ordering,
+                                      precision and pressure solver must be the server's */
+} cfdgpu_serve_request;
+
+typedef struct {
+  int ticket;
+  int status;       /* 0 converged, 1 iteration-limit, 2 diverged (Polar.cpp:79-84),
+                       3 rejected (mesh not buildable or larger than the whole budget) */
+  int iterations;   /* of the last attempt */
+  int retried_cold;
+  int warm_from;    /* ticket whose converged field started the solve, -1 cold */
+  int cells;
+  double cl, cd, cd_pressure, cd_friction, cm, l_over_d;
+  double upper_separation, lower_separation; /* chord fraction, -1 where attached */
+  double continuity;
+  double arrival_s, admitted_s, finished_s;  /* server clock */
+  double device_bytes;
+} cfdgpu_serve_result;
+
+typedef struct {
+  double clock_s;
+  int queued, active, admitted, finished, converged;
+  int peak_active, rounds, meshes_cached;
+  double resident_bytes, peak_resident_bytes;
+  double member_iterations; /* outer iterations advanced, summed over solves */
+} cfdgpu_server_stats;
+
+void cfdgpu_server_defaults(cfdgpu_server_options* out);
+int cfdgpu_server_create(const cfdgpu_server_options* options, cfdgpu_server** out);
+void cfdgpu_server_destroy(cfdgpu_server* server);
+int cfdgpu_server_submit(cfdgpu_server* server, const cfdgpu_serve_request* request, int* ticket);
+/* one scheduling round: a chunk for the resident solves, admission while it runs, retirement */
+int cfdgpu_server_step(cfdgpu_server* server, cfdgpu_server_stats* out);
+/* rounds until nothing is queued or resident, or the server clock reaches wall_limit_s (<= 0: none) */
+int cfdgpu_server_run(cfdgpu_server* server, double wall_limit_s, cfdgpu_server_stats* out);
+/* moves up to capacity finished results into out, oldest first; *count = how many */
+int cfdgpu_server_collect(cfdgpu_server* server, cfdgpu_serve_result* out, int capacity, int* count);
+/* the final fields of a finished ticket (keep_fields); cells from cfdgpu_serve_result */
+int cfdgpu_server_fields(const cfdgpu_server* server, int ticket, double* u, double* v, double* p,
+                         double* mu);
+
 /* ---- staged access for parity tests (one SIMPLE stage on the current device state) ----
  * names: "gradU" "gradV" "gradP" (cells, xy interleaved), "momDiag" "momS" "momN" "momW"
  * "momE" "momBx" "momBy", "pDiag" "pN" "pE" "pWake" "pSource", "correction", "strain",

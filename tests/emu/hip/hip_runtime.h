@@ -178,6 +178,12 @@ inline hipError_t hipMalloc(T** p, size_t bytes) {
 template <class T>
 inline hipError_t hipMallocAsync(T** p, size_t bytes, hipStream_t) { return hipMalloc(p, bytes); }
 inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+constexpr unsigned hipHostMallocDefault = 0;
+inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
+  *p = std::malloc(bytes);
+  return *p ? hipSuccess : 2;
+}
+inline hipError_t hipHostFree(void* p) { std::free(p); return hipSuccess; }
 inline hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
   std::memmove(d, s, n);

@@ -27,3 +27,26 @@ def test_every_declared_symbol_is_exported(G):
 def test_version_and_error_strings(G):
     assert G.lib().cfdgpu_version().startswith(b"cfdgpu")
     assert isinstance(G.lib().cfdgpu_last_error(), bytes)
+
+
+def test_host_side_validation_needs_no_gpu(G):
+    """Argument checks happen before any device call."""
+    lib = G.lib()
+    assert lib.cfdgpu_server_create(None, None) == 1  # InvalidArgument
+    opts = G.server_options()
+    assert opts.max_slots == 64 and opts.chunk_iterations == 20 and opts.warm_start == 1
+    assert opts.hbm_budget_bytes == 200e9 and opts.model == G.MODEL_SST
+    mixed = G.server_options(settings=G.default_settings(precision=G.MIXED))
+    import ctypes as C
+    h = C.c_void_p()
+    assert lib.cfdgpu_server_create(C.byref(mixed), C.byref(h)) == 5  # NotImplemented
+
+
+def test_server_create_fails_loudly_without_a_device(G):
+    if G.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    import ctypes as C
+    h = C.c_void_p()
+    opts = G.server_options()
+    assert G.lib().cfdgpu_server_create(C.byref(opts), C.byref(h)) == 6  # InitializationFailure
+    assert b"GPU" in G.lib().cfdgpu_last_error()
