@@ -306,6 +306,73 @@ def run_batch_polar(args):
           flush=True)
 
 
+def _percentile(sorted_vals, q):
+    if not sorted_vals:
+        return None
+    return sorted_vals[min(len(sorted_vals) - 1, int(q * len(sorted_vals)))]
+
+
+def run_serve(args, mixed_batch=False):
+    """The served request stream through cfdgpu_server_*: the first
+    --requests requests of the committed Poisson manifest (20 requests/s, random
+    NACA four-digit sections, alpha, Re and mesh level; workloads.py), replayed
+    at their arrival times, admitted by an HBM budget of 200 GB into at most
+    --slots resident solves, advanced together in chunks of --serve-chunk outer
+    iterations, warm-started from the nearest converged alpha of the same
+    section and mesh size. With --mixed-batch, the 64-instance mixed-size batch
+    manifest instead, all arriving at once. Reports converged solves/s and the
+    p50/p99 time from arrival to convergence (queueing included)."""
+    import cfdgpu as G
+    import workloads as W
+
+    rows = W.load(W.MIXED_BATCH_FILE if mixed_batch else W.SERVING_FILE)[: args.requests]
+    st = G.default_settings(intensity=0.001, viscosity_ratio=10.0)
+    srv = G.Server(G.server_options(hbm_budget_bytes=200e9, max_slots=args.slots, chunk_iterations=args.serve_chunk,
+                                    tolerance=args.tolerance, max_iterations=args.max_iterations, settings=st,
+                                    warm_start=0 if args.cold else 1))
+    for r in rows:
+        srv.submit(G.serve_request(r["m"], r["p"], r["t"], r["alpha"], r["reynolds"],
+                                   W.mesh_options(G, r["level"], r["reynolds"]),
+                                   arrival_s=0.0 if mixed_batch else r["arrival"]))
+    results, last = [], -1e9
+    while True:
+        stats = srv.step()
+        results += srv.collect()
+        if stats.clock_s - last >= 10.0:
+            last = stats.clock_s
+            print(f"{stats.clock_s:8.1f}s queued {stats.queued} resident {stats.active} finished {stats.finished} "
+                  f"converged {stats.converged} iterations {stats.member_iterations:.0f}", flush=True)
+        if stats.queued == 0 and stats.active == 0:
+            break
+        if args.wall_limit > 0 and stats.clock_s >= args.wall_limit:
+            break
+    wall = stats.clock_s
+    conv = [r for r in results if r.status == 0]
+    ttc = sorted(r.finished_s - r.arrival_s for r in conv)
+    by_level = {}
+    level_of = {q: rows[q]["level"] for q in range(len(rows))}
+    for lv in W.LEVELS:
+        mine = [r for r in results if level_of[r.ticket] == lv]
+        c = sorted(r.finished_s - r.arrival_s for r in mine if r.status == 0)
+        by_level[lv] = {"finished": len(mine), "converged": len(c), "ttc_p50_s": _percentile(c, 0.5),
+                        "iterations_mean": (sum(r.iterations for r in mine) / len(mine)) if mine else None}
+    statuses = {G.STATUS_NAMES[k]: sum(1 for r in results if r.status == k) for k in G.STATUS_NAMES}
+    name = ("mixed-size batch, 64 random NACA sections, coarse/medium/fine C-grids, SST" if mixed_batch else
+            f"served stream, first {len(rows)} requests of the 20 req/s Poisson manifest, SST")
+    print(json.dumps({"workload": name, "requests": len(rows), "finished": len(results),
+                      "unfinished": len(rows) - len(results), "statuses": statuses,
+                      "seconds": wall, "converged_solves_per_s": len(conv) / wall if wall > 0 else None,
+                      "time_to_convergence_p50_s": _percentile(ttc, 0.5),
+                      "time_to_convergence_p99_s": _percentile(ttc, 0.99),
+                      "warm_started": sum(1 for r in results if r.warm_from >= 0),
+                      "retried_cold": sum(1 for r in results if r.retried_cold),
+                      "outer_iterations": stats.member_iterations,
+                      "cell_iterations_per_s": sum(r.cells * r.iterations for r in results) / wall if wall > 0 else None,
+                      "peak_resident": stats.peak_active, "peak_resident_gb": stats.peak_resident_bytes / 1e9,
+                      "slots": args.slots, "chunk": args.serve_chunk, "tolerance": args.tolerance,
+                      "max_iterations": args.max_iterations, "by_level": by_level}), flush=True)
+
+
 def run_calibrate_copy(args):
     """A device-to-device copy of known size, for scaling the HBM traffic
     counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) to bytes."""
@@ -336,6 +403,13 @@ def main():
     ap.add_argument("--check-every", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true", help="a copy of known size (counter scaling)")
     ap.add_argument("--batch-polar", action="store_true", help="21-point cold polar as one batch")
+    ap.add_argument("--serve", action="store_true", help="replay the served request stream")
+    ap.add_argument("--mixed-batch", action="store_true", help="the 64-instance mixed-size batch, served at once")
+    ap.add_argument("--requests", type=int, default=64, help="requests of the manifest replayed by --serve")
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--serve-chunk", type=int, default=20)
+    ap.add_argument("--cold", action="store_true", help="serve without warm starts")
+    ap.add_argument("--wall-limit", type=float, default=900.0, help="seconds before --serve stops waiting")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
@@ -345,6 +419,8 @@ def main():
         run_calibrate_copy(args)
     elif args.batch_polar:
         run_batch_polar(args)
+    elif args.serve or args.mixed_batch:
+        run_serve(args, mixed_batch=args.mixed_batch)
     elif args.solve:
         run_solve(args)
     elif args.converge:

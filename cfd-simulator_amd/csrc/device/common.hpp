@@ -21,6 +21,31 @@ __device__ __forceinline__ bool imposes(const FaceBC& bc, double F) {
   return bc.kind == kWall || bc.kind == kInlet || (bc.kind == kFarField && F < bc.dead);
 }
 
+// Lane l of the wave receives lane l - 1's value (lane 0 receives 0): two
+// 32-bit DPP moves with the whole-wave shift wave_shr:1.
+__device__ __forceinline__ double waveShiftUp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x138, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// Step counters shared by the waves of one workgroup, in LDS.
+__device__ __forceinline__ int ldsPoll(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void ldsPost(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// This wave's earlier LDS accesses are complete before its later ones issue
+// (global loads in flight are not waited for).
+__device__ __forceinline__ void ldsOrder() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt untouched
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void spinPause() { __builtin_amdgcn_s_sleep(1); }
+
 // Blocks are handed to the 8 XCDs round-robin; remap so each XCD works on a
 // contiguous run of tiles and the j-1 / j+1 halo rows it re-reads stay in its
 // own L2.

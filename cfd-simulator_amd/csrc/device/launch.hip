@@ -15,6 +15,10 @@ __global__ void k_gs_lex_mom(const Inst*, int);
 __global__ void k_gs_lex_turb(const Inst*, int);
 __global__ void k_gs_lex_mom_wide(const Inst*, int);
 __global__ void k_gs_lex_turb_wide(const Inst*, int);
+__global__ void k_gs_wave_mom(const Inst*, int);
+__global__ void k_gs_wave_turb(const Inst*, int);
+__global__ void k_gs_wave_mom_wide(const Inst*, int);
+__global__ void k_gs_wave_turb_wide(const Inst*, int);
 __global__ void k_gs_unskew(const Inst*, int, int);
 __global__ void k_wall_distance(const double2*, int, const double2*, int, double*);
 __global__ void k_gs_color(const Inst*, int, int, int);
@@ -43,11 +47,18 @@ template <int PRE> __global__ void k_cgf_b(const Inst*, int);
 
 int gsThreadsFor(int nJ) { return gsLanes(nJ); }
 
+// The lexicographic sweep kernel: decoupled waves (DPP within a wave, LDS
+// step counters between waves) or the barrier-per-step wavefront.
+constexpr bool kDecoupledSweep = false;
+
 size_t gsLdsFor(int nJ, int nI, int threads) {
   (void)nJ;
-  // [2][2][threads] step buffer + [2][nI] row 0 (doubles) + [nI] partner map (ints)
-  return (static_cast<size_t>(2) * 2 * static_cast<size_t>(threads) + 2 * static_cast<size_t>(nI)) * sizeof(double) +
-         static_cast<size_t>(nI) * sizeof(int);
+  const size_t t = static_cast<size_t>(threads), n = static_cast<size_t>(nI), waves = (t + 63) / 64;
+  // barrier wavefront: [2][2][threads] step buffer + [2][nI] row 0 (doubles) + [nI] partner map (ints)
+  const size_t barrier = (2 * 2 * t + 2 * n) * sizeof(double) + n * sizeof(int);
+  // decoupled waves: [2][nI] row 0 + [waves][64][2] hand-off ring (doubles) + partner map and step counters (ints)
+  const size_t decoupled = (2 * n + waves * 64 * 2) * sizeof(double) + (n + waves) * sizeof(int);
+  return barrier > decoupled ? barrier : decoupled;
 }
 
 namespace {
@@ -99,8 +110,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     }
     end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   } else if (lex) {
-    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_mom, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
-    else hipLaunchKernelGGL(k_gs_lex_mom_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom : k_gs_lex_mom, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom_wide : k_gs_lex_mom_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
     end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
     begin(CFDGPU_STAGE_SWEEP_COPY);
     hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 0, h.momSweeps);
@@ -160,8 +171,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
       }
       end(CFDGPU_STAGE_SST_SWEEPS);
     } else if (lex) {
-      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
-      else hipLaunchKernelGGL(k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_turb : k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_turb_wide : k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
       end(CFDGPU_STAGE_SST_SWEEPS);
       begin(CFDGPU_STAGE_SWEEP_COPY);
       hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 1, 3);

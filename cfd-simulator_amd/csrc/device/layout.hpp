@@ -78,6 +78,7 @@ enum IFlag {
   fStopped,        // solve finished: later launches for this instance are no-ops
   fIteration,      // completed outer iterations
   fStatus,         // 0 running/converged, 1 iteration-limit, 2 diverged
+  fSweepStall,     // waits of the decoupled sweep that gave up (0 unless broken)
   kIFlags
 };
 

@@ -492,10 +492,155 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
   }
 }
 
+// The same sweep with the waves decoupled. Inside a wave the value below
+// comes from the lane underneath by a whole-wave DPP shift of the previous
+// step's results, so no LDS round trip and no block barrier separate two
+// steps. Only the row at a wave boundary crosses waves: lane 63 of wave w
+// posts its value for step k in an LDS ring (kSweepHand slots) and then
+// raises the wave's step counter; wave w + 1 reads it on step k + 1 once the
+// counter shows k. A wave runs ahead of the one above it by at most
+// kSweepHand - 16 steps (it checks the counter above every 16 steps before
+// reusing ring slots). Every wait is a bounded spin: if the bound is hit the
+// sweep gives up and counts a stall in flags[fSweepStall], so no wave can
+// hang the device. The arithmetic and its order are those of gsLexSkew.
+constexpr int kSweepHand = 64;
+constexpr int kSpinLimit = 1 << 22;
+
+__device__ __forceinline__ bool waitStep(const int* counter, int target, int* stall) {
+  int spins = 0;
+  while (ldsPoll(counter) < target) {
+    if (++spins > kSpinLimit) {
+      atomicAdd(stall, 1);
+      return false;
+    }
+    spinPause();
+  }
+  return true;
+}
+
+template <int NC, int D>
+__device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsSkew<NC>& S, int sweeps,
+                          int* stall) {
+  HIP_DYNAMIC_SHARED(double, lds)
+  const int B = static_cast<int>(blockDim.x);
+  const int W = B >> 6;
+  double* row0 = lds;                                                // [NC][nI]: row j = 0, updated in place
+  double* hand = row0 + NC * nI;                                     // [W][kSweepHand][NC]: lane 63's values
+  int* partner = reinterpret_cast<int*>(hand + W * kSweepHand * NC);  // [nI]
+  int* posted = partner + nI;                                        // [W]: last step each wave finished
+  const int lane = static_cast<int>(threadIdx.x);
+  const int w = lane >> 6, l = lane & 63;
+  const int laneG = min(lane, R - 1);
+  const bool active = lane < nJ;
+  const int steps = skewSteps(nI, nJ);
+  // the steps on which some row of this wave can have a cell (0 <= k - j < nI)
+  const int kBegin = 64 * w;
+  const int kEnd = min(steps, 64 * w + 63 + nI);
+  const int count = max(kEnd - kBegin, 0);
+  const int rings = (count + D - 1) / D;
+  const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
+  for (int t = lane; t < nI; t += B) partner[t] = partnerSrc[t];
+  bool ok = true;
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    const CFD_GLOBAL dpair* xOld = (sweep & 1) ? S.x[1] : S.x[0];
+    CFD_GLOBAL dpair* xNew = (sweep & 1) ? S.x[0] : S.x[1];
+    for (int t = lane; t < nI; t += B) {
+      const dpair v = xOld[skewIndex(t, 0, R)];
+      row0[t] = v[0];
+      if constexpr (NC == 2) row0[nI + t] = v[1];
+    }
+    if (l == 0) ldsPost(&posted[w], kBegin - 1);
+    __syncthreads();
+    double xw[NC], up[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) xw[q] = up[q] = 0.0;
+    GsCell<NC> ring[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) gsFetch<NC>(S, xOld, R, steps, laneG, kBegin + q, ring[q]);
+    int seenBelow = kBegin - 65;  // the wave underneath has finished this step (its start - 1)
+    for (int r = 0; r < rings; ++r) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        // no per-step condition: steps past kEnd touch no cell (they store into
+        // the padding row), so the body stays branch-free for the read-ahead
+        const int k = kBegin + r * D + q;
+        {
+          const int i = k - lane;
+          const bool on = active && i >= 0 && i < nI;
+          const GsCell<NC>& g = ring[q];
+          // lane 0's value below: the wake partner (wave 0) or lane 63 of the wave underneath
+          double low[NC];
+          int pr = -1;
+          if (w == 0) {
+            pr = partner[min(max(i, 0), nI - 1)];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) low[m] = row0[m * nI + max(pr, 0)];
+          } else {
+            const int i0 = k - 64 * w;  // lane 0's cell
+            if (ok && i0 >= 0 && i0 < nI && seenBelow < k - 1) {
+              ok = waitStep(&posted[w - 1], k - 1, stall);
+              seenBelow = ldsPoll(&posted[w - 1]);
+              ldsOrder();
+            }
+            const double* h = hand + ((w - 1) * kSweepHand + (k - 1) % kSweepHand) * NC;
+#pragma unroll
+            for (int m = 0; m < NC; ++m) low[m] = h[m];
+          }
+          const bool hasBelow = (lane > 0) || (pr >= 0) || (w > 0);
+          double xn[NC];
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            const double below = (l > 0) ? up[m] : low[m];
+            // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
+            // a missing neighbour's term is skipped, not added as zero
+            double off = 0.0;
+            off = hasBelow ? off + g.aS * below : off;
+            off = (i < nI - 1) ? off + g.aE * g.xE[m] : off;
+            off = (lane < nJ - 1) ? off + g.aN * g.xN[m] : off;
+            off = (i > 0) ? off + g.aW * xw[m] : off;
+            // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88)
+            xn[m] = (fabs(g.aP) > 0.0) ? (g.b[m] - off) / g.aP : g.xC[m];
+          }
+          {
+            const int at = on ? k * R + lane : (steps + 1) * R + laneG;
+            const dpair v = {xn[0], NC == 2 ? xn[NC - 1] : 0.0};
+            xNew[at] = v;
+          }
+          if (on && lane == 0) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) row0[m * nI + i] = xn[m];
+          }
+          // post lane 63's value for the wave above; every 16 steps make sure
+          // the ring slots of the next 16 are free (read by the wave above)
+          if (w + 1 < W) {
+            if ((k & 15) == 0 && ok) ok = waitStep(&posted[w + 1], k + 16 - kSweepHand, stall);
+            if (l == 63) {
+              double* h = hand + (w * kSweepHand + k % kSweepHand) * NC;
+#pragma unroll
+              for (int m = 0; m < NC; ++m) h[m] = xn[m];
+            }
+          }
+          ldsOrder();
+          if (l == 63) ldsPost(&posted[w], k);
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            up[m] = waveShiftUp(xn[m]);
+            xw[m] = xn[m];
+          }
+          gsFetch<NC>(S, xOld, R, steps, laneG, k + D, ring[q]);
+        }
+      }
+    }
+    // nobody waits on this wave for the rest of the sweep
+    if (l == 63) ldsPost(&posted[w], steps + kSweepHand);
+    __syncthreads();  // this sweep's writes are the next sweep's old values
+  }
+}
+
 // Momentum: u and v share one matrix, one workgroup. Turbulence: blockIdx.x
 // 0 sweeps k, 1 sweeps omega (independent systems, run side by side). The
 // launch gives the block I.gsR lanes (the layout's stride).
-template <int D>
+template <int D, bool Decoupled>
 __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped] || sweeps <= 0) return;
@@ -503,10 +648,11 @@ __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
                      gpair(static_cast<const double2*>(I.skMomRow[2]))},
                     gmem(static_cast<const double*>(I.skMomBy)),
                     {gpair(I.skMomX[0]), gpair(I.skMomX[1])}};
-  gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
+  if constexpr (Decoupled) gsLexWave<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps, I.flags + fSweepStall);
+  else gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
 }
 
-template <int D>
+template <int D, bool Decoupled>
 __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped] || sweeps <= 0) return;
@@ -516,18 +662,34 @@ __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
                      gpair(static_cast<const double2*>(row[2]))},
                     nullptr,
                     {gpair(I.skTurbX[sys][0]), gpair(I.skTurbX[sys][1])}};
-  gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
+  if constexpr (Decoupled) gsLexWave<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps, I.flags + fSweepStall);
+  else gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
 }
 
 // Up to 256 rows: one wave per 64 rows, each lane holding a ring of cells
 // fetched ahead (4 for u and v, 6 for k or omega).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_lex_mom(const Inst* insts, int sweeps) {
-  gsLexMomentum<4>(insts, sweeps);
+  gsLexMomentum<4, false>(insts, sweeps);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_lex_turb(const Inst* insts, int sweeps) {
-  gsLexTurbulence<6>(insts, sweeps);
+  gsLexTurbulence<6, false>(insts, sweeps);
+}
+// The decoupled-wave sweep (gsLexWave), same shapes.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_gs_wave_mom(const Inst* insts, int sweeps) {
+  gsLexMomentum<6, true>(insts, sweeps);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_gs_wave_turb(const Inst* insts, int sweeps) {
+  gsLexTurbulence<8, true>(insts, sweeps);
+}
+__global__ void __launch_bounds__(1024) k_gs_wave_mom_wide(const Inst* insts, int sweeps) {
+  gsLexMomentum<2, true>(insts, sweeps);
+}
+__global__ void __launch_bounds__(1024) k_gs_wave_turb_wide(const Inst* insts, int sweeps) {
+  gsLexTurbulence<2, true>(insts, sweeps);
 }
 
 // After `sweeps` sweeps the result is in the skewed copy x[sweeps & 1]:
@@ -552,10 +714,10 @@ __global__ void __launch_bounds__(kBlock) k_gs_unskew(const Inst* insts, int sys
 
 // More than 256 rows (up to 1024): a ring of two cells.
 __global__ void __launch_bounds__(1024) k_gs_lex_mom_wide(const Inst* insts, int sweeps) {
-  gsLexMomentum<2>(insts, sweeps);
+  gsLexMomentum<2, false>(insts, sweeps);
 }
 __global__ void __launch_bounds__(1024) k_gs_lex_turb_wide(const Inst* insts, int sweeps) {
-  gsLexTurbulence<2>(insts, sweeps);
+  gsLexTurbulence<2, false>(insts, sweeps);
 }
 
 // ---------------------------------------------------------------------------

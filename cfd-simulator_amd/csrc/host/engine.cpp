@@ -626,6 +626,20 @@ void Engine::readMonitorOn(hipStream_t s, cfdgpu_monitor* out) const {
   out->max_eddy_ratio = m.maxEddyRatio;
 }
 
+// The lexicographic sweep's waves wait on each other with bounded spins; a
+// wait that gave up left wrong values behind, so the call fails.
+void checkSweepStall(const int* flags) {
+  if (flags[fSweepStall] != 0)
+    throw std::runtime_error("the lexicographic sweep stalled (a wave's wait for its neighbour timed out)");
+}
+
+void Engine::checkStall() const {
+  int flags[kIFlags] = {};
+  check(hipMemcpyAsync(flags, h_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream_), "flags");
+  check(hipStreamSynchronize(stream_), "flags sync");
+  checkSweepStall(flags);
+}
+
 int Engine::iterate(int count, cfdgpu_monitor* out) {
   if (!initialised_) return CFDGPU_INITIALIZATION_FAILURE;
   // a plain iterate() never stops by itself (SimpleSolver::iterate never fails)
@@ -635,6 +649,7 @@ int Engine::iterate(int count, cfdgpu_monitor* out) {
   check(hipGetLastError(), "launch");
   if (out) readMonitor(out);
   else check(hipStreamSynchronize(stream_), "iterate sync");
+  checkStall();
   return CFDGPU_OK;
 }
 
@@ -650,6 +665,7 @@ int Engine::iterateTimed(int count, cfdgpu_monitor* out, double* stageMs, int* s
   check(hipStreamSynchronize(stream_), "iterate sync");
   if (out) readMonitor(out);
   clock.collect(stageMs, stageCalls);
+  checkStall();
   return CFDGPU_OK;
 }
 
@@ -676,6 +692,7 @@ int Engine::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
     check(hipGetLastError(), "launch");
     check(hipMemcpyAsync(flags, h_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream_), "flags");
     check(hipStreamSynchronize(stream_), "solve sync");
+    checkSweepStall(flags);
     if (flags[fStopped]) break;
     if (rule.max_iterations > 0 && launched >= rule.max_iterations) break;
   }
@@ -886,6 +903,7 @@ int Batch::iterate(int count, cfdgpu_monitor* out) {
   for (int n = 0; n < count; ++n) launchIteration(dInsts_, cfg_, host_.front(), stream_);
   check(hipGetLastError(), "launch");
   check(hipStreamSynchronize(stream_), "batch sync");
+  for (Engine* e : members_) e->checkStall();
   if (out)
     for (size_t q = 0; q < members_.size(); ++q) members_[q]->readMonitorOn(stream_, &out[q]);
   return CFDGPU_OK;
@@ -922,7 +940,10 @@ int Batch::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
             "flags");
     check(hipStreamSynchronize(stream_), "batch solve sync");
     bool all = true;
-    for (size_t q = 0; q < n; ++q) all = all && flags[q][fStopped];
+    for (size_t q = 0; q < n; ++q) {
+      checkSweepStall(flags[q].data());
+      all = all && flags[q][fStopped];
+    }
     if (all) break;
     if (rule.max_iterations > 0 && launched >= rule.max_iterations) break;
   }

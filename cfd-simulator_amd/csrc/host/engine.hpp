@@ -47,6 +47,8 @@ int gsThreadsFor(int nJ);
 size_t gsLdsFor(int nJ, int nI, int threads);
 
 class StageClock;
+// throws when flags (kIFlags ints of one instance) record a stalled sweep
+void checkSweepStall(const int* flags);
 
 // Device arena for one instance: every array of Inst carved out of one allocation.
 struct DeviceArena {
@@ -99,6 +101,8 @@ class Engine {
   Inst& hostInst() { return h_; }
   void readMonitorOn(hipStream_t s, cfdgpu_monitor* out) const;
   void uploadInst();
+  // throws when a lexicographic sweep of this solver stalled (see checkSweepStall)
+  void checkStall() const;
 
  private:
   void precomputeGeometry();

@@ -313,6 +313,7 @@ void Server::retire() {
   for (size_t q = 0; q < active_.size(); ++q) {
     Slot& s = *active_[q];
     const int* fl = hFlags_ + q * kIFlags;
+    checkSweepStall(fl);
     memberIterations_ += fl[fIteration] - s.lastIteration;
     s.lastIteration = fl[fIteration];
     if (!fl[fStopped]) {

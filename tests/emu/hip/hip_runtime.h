@@ -75,6 +75,19 @@ inline double shflDown(double v, int offset, int width) {
   wb.arrive_and_wait();
   return r;
 }
+// DPP move with wave_shr:1 (the only control the kernels use): lane l gets
+// lane l - 1's value, lane 0 keeps `old`.
+inline int dppWaveShr1(int old, int v, int ctrl) {
+  if (ctrl != 0x138) std::abort();
+  const unsigned lane = tIdx.x % 64u;
+  const unsigned wave = tIdx.x / 64u;
+  auto& wb = *ctx->waves[wave];
+  ctx->xchg[tIdx.x] = static_cast<double>(v);
+  wb.arrive_and_wait();
+  const int r = (lane > 0) ? static_cast<int>(ctx->xchg[tIdx.x - 1]) : old;
+  wb.arrive_and_wait();
+  return r;
+}
 // A persistent pool of lane threads (created once, up to 1024), so long runs
 // do not create threads per launch.
 class LanePool {
@@ -159,6 +172,20 @@ void runBlock(dim3 b, dim3 t, size_t shmem, F&& body) {
 #define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(cfdemu::ctx->lds.data());
 inline double __shfl_down(double v, int offset, int width = 64) { return cfdemu::shflDown(v, offset, width); }
+#define __builtin_amdgcn_update_dpp(old, v, ctrl, rm, bm, bc) cfdemu::dppWaveShr1((old), (v), (ctrl))
+#define __builtin_amdgcn_s_sleep(x) std::this_thread::yield()
+#define __HIP_MEMORY_SCOPE_WORKGROUP 2
+template <class T>
+inline T __hip_atomic_load(const T* p, int order, int) {
+  (void)order;
+  return std::atomic_ref<T>(*const_cast<T*>(p)).load(std::memory_order_seq_cst);
+}
+template <class T>
+inline void __hip_atomic_store(T* p, T v, int order, int) {
+  (void)order;
+  std::atomic_ref<T>(*p).store(v, std::memory_order_seq_cst);
+}
+inline int atomicAdd(int* p, int v) { return std::atomic_ref<int>(*p).fetch_add(v); }
 
 // runtime API
 using hipError_t = int;
