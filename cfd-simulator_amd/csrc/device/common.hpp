@@ -45,6 +45,17 @@ __device__ __forceinline__ void ldsOrder() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 __device__ __forceinline__ void spinPause() { __builtin_amdgcn_s_sleep(1); }
+// Compiler-only barrier: the values in v are computed before it and no memory
+// access is moved across it (emits no instruction).
+__device__ __forceinline__ void holdUntilHere(double v) { asm volatile("" : : "v"(v) : "memory"); }
+// Compiler-only: v is computed on every lane at this point (the compiler may
+// not sink its computation into a branch of the select that uses it).
+__device__ __forceinline__ void computedHere(double v) { asm volatile("" : : "v"(v)); }
+template <int N>
+__device__ __forceinline__ void holdUntilHere(const double (&v)[N]) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) asm volatile("" : : "v"(v[q]) : "memory");
+}
 
 // Blocks are handed to the 8 XCDs round-robin; remap so each XCD works on a
 // contiguous run of tiles and the j-1 / j+1 halo rows it re-reads stay in its

@@ -49,15 +49,16 @@ int gsThreadsFor(int nJ) { return gsLanes(nJ); }
 
 // The lexicographic sweep kernel: decoupled waves (DPP within a wave, LDS
 // step counters between waves) or the barrier-per-step wavefront.
-constexpr bool kDecoupledSweep = false;
+constexpr bool kDecoupledSweep = true;
 
 size_t gsLdsFor(int nJ, int nI, int threads) {
   (void)nJ;
   const size_t t = static_cast<size_t>(threads), n = static_cast<size_t>(nI), waves = (t + 63) / 64;
   // barrier wavefront: [2][2][threads] step buffer + [2][nI] row 0 (doubles) + [nI] partner map (ints)
   const size_t barrier = (2 * 2 * t + 2 * n) * sizeof(double) + n * sizeof(int);
-  // decoupled waves: [2][nI] row 0 + [waves][64][2] hand-off ring (doubles) + partner map and step counters (ints)
-  const size_t decoupled = (2 * n + waves * 64 * 2) * sizeof(double) + (n + waves) * sizeof(int);
+  // decoupled waves: row 0, the [waves][64] hand-off ring and a dummy per lane (doubles); partner map, step
+  // counters and a dummy per lane (ints)
+  const size_t decoupled = (n + waves * 64 + t) * sizeof(double) + (n + waves + t) * sizeof(int);
   return barrier > decoupled ? barrier : decoupled;
 }
 
@@ -110,8 +111,10 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     }
     end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   } else if (lex) {
-    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom : k_gs_lex_mom, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
-    else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom_wide : k_gs_lex_mom_wide, dim3(1, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    // decoupled: u and v in two workgroups; barrier wavefront: both in one
+    const dim3 momGrid(kDecoupledSweep ? 2 : 1, cfg.batch);
+    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom : k_gs_lex_mom, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom_wide : k_gs_lex_mom_wide, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
     end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
     begin(CFDGPU_STAGE_SWEEP_COPY);
     hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 0, h.momSweeps);

@@ -372,16 +372,25 @@ __device__ __forceinline__ void gsFetch(const GsSkew<NC>& S, const CFD_GLOBAL dp
   g.aE = r2[0];
   g.b[0] = r2[1];
   if constexpr (NC == 2) g.b[1] = S.b1[at];
-  const dpair c = xOld[at];
-  const dpair e = xOld[atE];
-  const dpair n = xOld[atN];
-  g.xC[0] = c[0];
-  g.xE[0] = e[0];
-  g.xN[0] = n[0];
   if constexpr (NC == 2) {
+    const dpair c = xOld[at];
+    const dpair e = xOld[atE];
+    const dpair n = xOld[atN];
+    g.xC[0] = c[0];
+    g.xE[0] = e[0];
+    g.xN[0] = n[0];
     g.xC[1] = c[1];
     g.xE[1] = e[1];
     g.xN[1] = n[1];
+  } else {
+    // one component: 8-byte loads of the pair's first half. A 16-byte load
+    // with an unused half leaves registers the compiler reuses at once, and
+    // reusing a register a load is still writing forces a wait for every
+    // load in flight.
+    const CFD_GLOBAL double* x1 = reinterpret_cast<const CFD_GLOBAL double*>(xOld);
+    g.xC[0] = x1[2 * at];
+    g.xE[0] = x1[2 * atE];
+    g.xN[0] = x1[2 * atN];
   }
 }
 
@@ -492,25 +501,32 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
   }
 }
 
-// The same sweep with the waves decoupled. Inside a wave the value below
-// comes from the lane underneath by a whole-wave DPP shift of the previous
-// step's results, so no LDS round trip and no block barrier separate two
-// steps. Only the row at a wave boundary crosses waves: lane 63 of wave w
-// posts its value for step k in an LDS ring (kSweepHand slots) and then
-// raises the wave's step counter; wave w + 1 reads it on step k + 1 once the
-// counter shows k. A wave runs ahead of the one above it by at most
-// kSweepHand - 16 steps (it checks the counter above every 16 steps before
-// reusing ring slots). Every wait is a bounded spin: if the bound is hit the
-// sweep gives up and counts a stall in flags[fSweepStall], so no wave can
-// hang the device. The arithmetic and its order are those of gsLexSkew.
-constexpr int kSweepHand = 64;
+// The same sweep with the waves decoupled, one component per workgroup.
+// Inside a wave the value below comes from the lane underneath by a
+// whole-wave DPP shift of the previous step's results, so no LDS round trip
+// and no block barrier separate two steps. Only the row at a wave boundary
+// crosses waves: lane 63 of wave w posts its value for step k in an LDS ring
+// (kSweepHand slots) and then raises the wave's step counter; wave w + 1
+// reads it on step k + 1 once the counter shows k. A wave runs ahead of the
+// one above it by at most kSweepHand - 16 steps (it checks the counter above
+// every 16 steps before reusing ring slots). Every wait is a bounded spin: if
+// the bound is hit the sweep gives up and counts a stall in
+// flags[fSweepStall], so no wave can hang the device. The step has no
+// divergent branch: writes that only one lane makes go, on the other lanes,
+// to a per-lane dummy slot. The arithmetic and its order are those of
+// gsLexSkew.
+constexpr int kSweepHand = 64;  // a power of two
 constexpr int kSpinLimit = 1 << 22;
 
+// Waits until *counter >= target; the counter is read as a wave-uniform value.
 __device__ __forceinline__ bool waitStep(const int* counter, int target, int* stall) {
   int spins = 0;
-  while (ldsPoll(counter) < target) {
+#pragma unroll 1
+  while (__builtin_amdgcn_readfirstlane(ldsPoll(counter)) < target) {
     if (++spins > kSpinLimit) {
-      atomicAdd(stall, 1);
+      // global address space, not FLAT: a FLAT access in flight would make
+      // every later LDS wait also wait for all outstanding global loads
+      __hip_atomic_fetch_add(gmem(stall), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     spinPause();
@@ -518,117 +534,146 @@ __device__ __forceinline__ bool waitStep(const int* counter, int target, int* st
   return true;
 }
 
-template <int NC, int D>
-__device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsSkew<NC>& S, int sweeps,
-                          int* stall) {
+// One component of a skewed system: the matrix rows (aP, aS), (aN, aW),
+// (aE, b) as pairs, and the unknowns as one half of the pair arrays. With
+// SepB the right-hand side comes from its own array (v of momentum) and only
+// aE is read from the third pair.
+struct GsOne {
+  const CFD_GLOBAL dpair* row[3];
+  const CFD_GLOBAL double* b;
+  CFD_GLOBAL double* x[2];  // the pair arrays as doubles: unknown q at 2 q + half
+  int half;
+};
+struct GsCell1 {
+  double aP, aS, aN, aW, aE, b, xC, xN, xE;
+};
+
+// Every value is loaded at its own width, none with an unused half: a load
+// whose result is partly unused leaves registers the compiler reuses at once,
+// and reusing a register a load is still writing forces a wait for every load
+// in flight. Steps are clamped into [0, steps - 1] as in gsFetch.
+template <bool SepB>
+__device__ __forceinline__ void gsFetch1(const GsOne& S, const CFD_GLOBAL double* xOld, int R, int steps, int lane,
+                                         int k, GsCell1& g) {
+  const int at = min(k, steps - 1) * R + lane;
+  const int atE = min(k + 1, steps - 1) * R + lane;
+  const int atN = atE + 1;
+  const dpair r0 = S.row[0][at];
+  const dpair r1 = S.row[1][at];
+  g.aP = r0[0];
+  g.aS = r0[1];
+  g.aN = r1[0];
+  g.aW = r1[1];
+  if constexpr (SepB) {
+    g.aE = reinterpret_cast<const CFD_GLOBAL double*>(S.row[2])[2 * at];
+    g.b = S.b[at];
+  } else {
+    const dpair r2 = S.row[2][at];
+    g.aE = r2[0];
+    g.b = r2[1];
+  }
+  g.xC = xOld[2 * at + S.half];
+  g.xE = xOld[2 * atE + S.half];
+  g.xN = xOld[2 * atN + S.half];
+}
+
+template <int D, bool SepB>
+__device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOne& S, int sweeps, int* stall) {
   HIP_DYNAMIC_SHARED(double, lds)
   const int B = static_cast<int>(blockDim.x);
   const int W = B >> 6;
-  double* row0 = lds;                                                // [NC][nI]: row j = 0, updated in place
-  double* hand = row0 + NC * nI;                                     // [W][kSweepHand][NC]: lane 63's values
-  int* partner = reinterpret_cast<int*>(hand + W * kSweepHand * NC);  // [nI]
-  int* posted = partner + nI;                                        // [W]: last step each wave finished
+  double* row0 = lds;                                       // [nI]: row j = 0, updated in place
+  double* hand = row0 + nI;                                 // [W][kSweepHand]: lane 63's values
+  double* dummy = hand + W * kSweepHand;                    // [B]: writes of the lanes that make none
+  int* partner = reinterpret_cast<int*>(dummy + B);         // [nI]
+  int* posted = partner + nI;                               // [W]: last step each wave finished
+  int* dummyInt = posted + W;                               // [B]
   const int lane = static_cast<int>(threadIdx.x);
-  const int w = lane >> 6, l = lane & 63;
+  // the wave index as a scalar, so that branches on it are uniform
+  const int w = __builtin_amdgcn_readfirstlane(lane >> 6), l = lane & 63;
   const int laneG = min(lane, R - 1);
   const bool active = lane < nJ;
   const int steps = skewSteps(nI, nJ);
   // the steps on which some row of this wave can have a cell (0 <= k - j < nI)
   const int kBegin = 64 * w;
   const int kEnd = min(steps, 64 * w + 63 + nI);
-  const int count = max(kEnd - kBegin, 0);
+  // the loop starts D steps early: those steps only fill the read-ahead ring
+  // (every lane is off the wavefront there), so no value is loaded before the
+  // loop and the compiler's waits inside it stay partial
+  const int kFirst = kBegin - D;
+  const int count = max(kEnd - kFirst, 0);
   const int rings = (count + D - 1) / D;
   const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
   for (int t = lane; t < nI; t += B) partner[t] = partnerSrc[t];
   bool ok = true;
   for (int sweep = 0; sweep < sweeps; ++sweep) {
-    const CFD_GLOBAL dpair* xOld = (sweep & 1) ? S.x[1] : S.x[0];
-    CFD_GLOBAL dpair* xNew = (sweep & 1) ? S.x[0] : S.x[1];
-    for (int t = lane; t < nI; t += B) {
-      const dpair v = xOld[skewIndex(t, 0, R)];
-      row0[t] = v[0];
-      if constexpr (NC == 2) row0[nI + t] = v[1];
-    }
+    const CFD_GLOBAL double* xOld = (sweep & 1) ? S.x[1] : S.x[0];
+    CFD_GLOBAL double* xNew = (sweep & 1) ? S.x[0] : S.x[1];
+    for (int t = lane; t < nI; t += B) row0[t] = xOld[2 * skewIndex(t, 0, R) + S.half];
     if (l == 0) ldsPost(&posted[w], kBegin - 1);
     __syncthreads();
-    double xw[NC], up[NC];
+    double xw = 0.0, up = 0.0;
+    GsCell1 ring[D];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) xw[q] = up[q] = 0.0;
-    GsCell<NC> ring[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) gsFetch<NC>(S, xOld, R, steps, laneG, kBegin + q, ring[q]);
+    for (int q = 0; q < D; ++q) ring[q] = GsCell1{};
     int seenBelow = kBegin - 65;  // the wave underneath has finished this step (its start - 1)
     for (int r = 0; r < rings; ++r) {
 #pragma unroll
       for (int q = 0; q < D; ++q) {
         // no per-step condition: steps past kEnd touch no cell (they store into
         // the padding row), so the body stays branch-free for the read-ahead
-        const int k = kBegin + r * D + q;
-        {
-          const int i = k - lane;
-          const bool on = active && i >= 0 && i < nI;
-          const GsCell<NC>& g = ring[q];
-          // lane 0's value below: the wake partner (wave 0) or lane 63 of the wave underneath
-          double low[NC];
-          int pr = -1;
-          if (w == 0) {
-            pr = partner[min(max(i, 0), nI - 1)];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) low[m] = row0[m * nI + max(pr, 0)];
-          } else {
-            const int i0 = k - 64 * w;  // lane 0's cell
-            if (ok && i0 >= 0 && i0 < nI && seenBelow < k - 1) {
-              ok = waitStep(&posted[w - 1], k - 1, stall);
-              seenBelow = ldsPoll(&posted[w - 1]);
-              ldsOrder();
-            }
-            const double* h = hand + ((w - 1) * kSweepHand + (k - 1) % kSweepHand) * NC;
-#pragma unroll
-            for (int m = 0; m < NC; ++m) low[m] = h[m];
+        const int k = kFirst + r * D + q;
+        const int i = k - lane;
+        const bool on = active && i >= 0 && i < nI;
+        const GsCell1& g = ring[q];
+        // lane 0's value below: the wake partner (wave 0) or lane 63 of the wave underneath
+        double low;
+        int pr = -1;
+        if (w == 0) {
+          pr = partner[min(max(i, 0), nI - 1)];
+          low = row0[max(pr, 0)];
+        } else {
+          const int i0 = k - 64 * w;  // lane 0's cell
+          if (ok && i0 >= 0 && i0 < nI && seenBelow < k - 1) {
+            ok = waitStep(&posted[w - 1], k - 1, stall);
+            seenBelow = __builtin_amdgcn_readfirstlane(ldsPoll(&posted[w - 1]));
+            ldsOrder();
           }
-          const bool hasBelow = (lane > 0) || (pr >= 0) || (w > 0);
-          double xn[NC];
-#pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            const double below = (l > 0) ? up[m] : low[m];
-            // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
-            // a missing neighbour's term is skipped, not added as zero
-            double off = 0.0;
-            off = hasBelow ? off + g.aS * below : off;
-            off = (i < nI - 1) ? off + g.aE * g.xE[m] : off;
-            off = (lane < nJ - 1) ? off + g.aN * g.xN[m] : off;
-            off = (i > 0) ? off + g.aW * xw[m] : off;
-            // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88)
-            xn[m] = (fabs(g.aP) > 0.0) ? (g.b[m] - off) / g.aP : g.xC[m];
-          }
-          {
-            const int at = on ? k * R + lane : (steps + 1) * R + laneG;
-            const dpair v = {xn[0], NC == 2 ? xn[NC - 1] : 0.0};
-            xNew[at] = v;
-          }
-          if (on && lane == 0) {
-#pragma unroll
-            for (int m = 0; m < NC; ++m) row0[m * nI + i] = xn[m];
-          }
-          // post lane 63's value for the wave above; every 16 steps make sure
-          // the ring slots of the next 16 are free (read by the wave above)
-          if (w + 1 < W) {
-            if ((k & 15) == 0 && ok) ok = waitStep(&posted[w + 1], k + 16 - kSweepHand, stall);
-            if (l == 63) {
-              double* h = hand + (w * kSweepHand + k % kSweepHand) * NC;
-#pragma unroll
-              for (int m = 0; m < NC; ++m) h[m] = xn[m];
-            }
-          }
-          ldsOrder();
-          if (l == 63) ldsPost(&posted[w], k);
-#pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            up[m] = waveShiftUp(xn[m]);
-            xw[m] = xn[m];
-          }
-          gsFetch<NC>(S, xOld, R, steps, laneG, k + D, ring[q]);
+          low = hand[(w - 1) * kSweepHand + ((k - 1) & (kSweepHand - 1))];
         }
+        computedHere(low);  // read by every lane: no branch around the LDS read
+        const bool hasBelow = (lane > 0) || (pr >= 0);
+        const double below = (l > 0) ? up : low;
+        // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
+        // a missing neighbour's term is skipped, not added as zero
+        double off = 0.0;
+        off = hasBelow ? off + g.aS * below : off;
+        off = (i < nI - 1) ? off + g.aE * g.xE : off;
+        off = (lane < nJ - 1) ? off + g.aN * g.xN : off;
+        off = (i > 0) ? off + g.aW * xw : off;
+        // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88);
+        // the quotient is formed on every lane so the choice is a select
+        const double quotient = (g.b - off) / g.aP;
+        computedHere(quotient);
+        const double xn = (fabs(g.aP) > 0.0) ? quotient : g.xC;
+        xNew[2 * (on ? k * R + lane : (steps + 1) * R + laneG) + S.half] = xn;
+        *((on && lane == 0) ? &row0[max(i, 0)] : &dummy[lane]) = xn;
+        // post lane 63's value for the wave above; every 16 steps make sure
+        // the ring slots of the next 16 are free (read by the wave above)
+        const bool started = k >= kBegin;  // the ring-filling steps post nothing
+        const bool feeds = (w + 1 < W) && started;
+        if (feeds && (k & 15) == 0 && ok) ok = waitStep(&posted[w + 1], k + 16 - kSweepHand, stall);
+        *((feeds && l == 63) ? &hand[w * kSweepHand + (k & (kSweepHand - 1))] : &dummy[lane]) = xn;
+        ldsOrder();
+        ldsPost((started && l == 63) ? &posted[w] : &dummyInt[lane], k);
+        up = waveShiftUp(xn);
+        xw = xn;
+        // the refill may not be hoisted above the shift: the slot's old
+        // values and this step's results are dead by here, so the loads land
+        // in the slot's own registers (no copies of loads in flight, which
+        // would force a full wait at the loop latch)
+        holdUntilHere(up);
+        gsFetch1<SepB>(S, xOld, R, steps, laneG, k + D, ring[q]);
       }
     }
     // nobody waits on this wave for the rest of the sweep
@@ -648,8 +693,18 @@ __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
                      gpair(static_cast<const double2*>(I.skMomRow[2]))},
                     gmem(static_cast<const double*>(I.skMomBy)),
                     {gpair(I.skMomX[0]), gpair(I.skMomX[1])}};
-  if constexpr (Decoupled) gsLexWave<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps, I.flags + fSweepStall);
-  else gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
+  if constexpr (Decoupled) {
+    // u (blockIdx.x 0) and v (1) in two workgroups, side by side
+    const int half = (blockIdx.x == 0) ? 0 : 1;
+    const GsOne C{{S.row[0], S.row[1], S.row[2]},
+                  S.b1,
+                  {reinterpret_cast<CFD_GLOBAL double*>(S.x[0]), reinterpret_cast<CFD_GLOBAL double*>(S.x[1])},
+                  half};
+    if (half == 0) gsLexWave<D, false>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
+    else gsLexWave<D, true>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
+  } else {
+    gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
+  }
 }
 
 template <int D, bool Decoupled>
@@ -662,8 +717,15 @@ __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
                      gpair(static_cast<const double2*>(row[2]))},
                     nullptr,
                     {gpair(I.skTurbX[sys][0]), gpair(I.skTurbX[sys][1])}};
-  if constexpr (Decoupled) gsLexWave<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps, I.flags + fSweepStall);
-  else gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
+  if constexpr (Decoupled) {
+    const GsOne C{{S.row[0], S.row[1], S.row[2]},
+                  nullptr,
+                  {reinterpret_cast<CFD_GLOBAL double*>(S.x[0]), reinterpret_cast<CFD_GLOBAL double*>(S.x[1])},
+                  0};
+    gsLexWave<D, false>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
+  } else {
+    gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
+  }
 }
 
 // Up to 256 rows: one wave per 64 rows, each lane holding a ring of cells
@@ -676,20 +738,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
 k_gs_lex_turb(const Inst* insts, int sweeps) {
   gsLexTurbulence<6, false>(insts, sweeps);
 }
-// The decoupled-wave sweep (gsLexWave), same shapes.
+// The decoupled-wave sweep (gsLexWave): one workgroup per component (u and v
+// side by side for momentum, k and omega for turbulence).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_wave_mom(const Inst* insts, int sweeps) {
-  gsLexMomentum<6, true>(insts, sweeps);
+  gsLexMomentum<8, true>(insts, sweeps);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_wave_turb(const Inst* insts, int sweeps) {
   gsLexTurbulence<8, true>(insts, sweeps);
 }
 __global__ void __launch_bounds__(1024) k_gs_wave_mom_wide(const Inst* insts, int sweeps) {
-  gsLexMomentum<2, true>(insts, sweeps);
+  gsLexMomentum<3, true>(insts, sweeps);
 }
 __global__ void __launch_bounds__(1024) k_gs_wave_turb_wide(const Inst* insts, int sweeps) {
-  gsLexTurbulence<2, true>(insts, sweeps);
+  gsLexTurbulence<3, true>(insts, sweeps);
 }
 
 // After `sweeps` sweeps the result is in the skewed copy x[sweeps & 1]:

@@ -174,6 +174,8 @@ void runBlock(dim3 b, dim3 t, size_t shmem, F&& body) {
 inline double __shfl_down(double v, int offset, int width = 64) { return cfdemu::shflDown(v, offset, width); }
 #define __builtin_amdgcn_update_dpp(old, v, ctrl, rm, bm, bc) cfdemu::dppWaveShr1((old), (v), (ctrl))
 #define __builtin_amdgcn_s_sleep(x) std::this_thread::yield()
+// lanes are threads here: each keeps its own copy of a wave-uniform value
+#define __builtin_amdgcn_readfirstlane(x) (x)
 #define __HIP_MEMORY_SCOPE_WORKGROUP 2
 template <class T>
 inline T __hip_atomic_load(const T* p, int order, int) {
@@ -186,6 +188,12 @@ inline void __hip_atomic_store(T* p, T v, int order, int) {
   std::atomic_ref<T>(*p).store(v, std::memory_order_seq_cst);
 }
 inline int atomicAdd(int* p, int v) { return std::atomic_ref<int>(*p).fetch_add(v); }
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <class T>
+inline T __hip_atomic_fetch_add(T* p, T v, int order, int) {
+  (void)order;
+  return std::atomic_ref<T>(*p).fetch_add(v);
+}
 
 // runtime API
 using hipError_t = int;
