@@ -47,8 +47,8 @@ ITERATIONS_TO_CONVERGE = 12068
 
 # HBM bytes per launch of the dominant kernels from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE, scaled by the same counters on a copy of known size
-# (profiles/c2_traffic.md); None until measured for this build.
-TRAFFIC_BYTES = {"k_gs_lex_mom": 83.67e6, "k_gs_lex_turb": 151.62e6, "k_cg_a + k_cg_b": 32.19e6}
+# (profiles/naca2412_1024x256_sst_traffic.md); None until measured for this build.
+TRAFFIC_BYTES = {"k_gs_wave_mom": 140.38e6, "k_gs_wave_turb": 133.66e6, "k_cg_a + k_cg_b": 32.19e6}
 
 WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
                 first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
@@ -72,8 +72,8 @@ def traffic_model():
 
 # The kernels that can dominate an outer iteration: (stage, kernel, unit of
 # the traffic model, launches per stage call and CG iteration).
-DOMINANT_CANDIDATES = [("momentum_sweeps", "k_gs_lex_mom", "momentum_sweeps"),
-                       ("sst_sweeps", "k_gs_lex_turb", "sst_sweeps"),
+DOMINANT_CANDIDATES = [("momentum_sweeps", "k_gs_wave_mom", "momentum_sweeps"),
+                       ("sst_sweeps", "k_gs_wave_turb", "sst_sweeps"),
                        ("pressure_solve", "k_cg_a + k_cg_b", "cg_iteration")]
 
 

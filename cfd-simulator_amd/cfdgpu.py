@@ -40,7 +40,7 @@ EXPORTS = [
     "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_surface",
     "cfdgpu_sweep_angles", "cfdgpu_debug_array", "cfdgpu_server_defaults", "cfdgpu_server_create",
     "cfdgpu_server_destroy", "cfdgpu_server_submit", "cfdgpu_server_step", "cfdgpu_server_run",
-    "cfdgpu_server_collect", "cfdgpu_server_fields",
+    "cfdgpu_server_collect", "cfdgpu_server_fields", "cfdgpu_solver_surface",
 ]
 
 
@@ -224,6 +224,8 @@ def load(path: str = LIB_PATH):
         "cfdgpu_server_run": (C.c_int, [vp, C.c_double, C.POINTER(ServerStats)]),
         "cfdgpu_server_collect": (C.c_int, [vp, C.POINTER(ServeResult), C.c_int, C.POINTER(C.c_int)]),
         "cfdgpu_server_fields": (C.c_int, [vp, C.c_int, D, D, D, D]),
+        "cfdgpu_solver_surface": (C.c_int, [vp] + [C.c_double] * 5 + [_dp, C.c_int, C.POINTER(C.c_int),
+                                                                      C.POINTER(Forces)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -436,6 +438,17 @@ class SimpleSolver:
         out = np.zeros(self.mesh.cells)
         _check(lib().cfdgpu_divergence(self.h, out))
         return out
+
+    def surface(self, speed=50.0, alpha_deg=0.0, density=1.225, reference_pressure=0.0, chord=1.0):
+        """The surface distribution and forces of the current field, on the
+        device (cfdgpu_solver_surface). Returns (Forces, rows[n,12])."""
+        cap = 12 * self.mesh.ni
+        rows = np.zeros(cap)
+        f = Forces()
+        n = C.c_int()
+        _check(lib().cfdgpu_solver_surface(self.h, speed, alpha_deg, density, reference_pressure, chord, rows, cap,
+                                           C.byref(n), C.byref(f)))
+        return f, rows[:12 * n.value].reshape(n.value, 12)
 
     def debug_array(self, name: str):
         cap = 2 * self.mesh.cells

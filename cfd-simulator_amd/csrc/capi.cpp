@@ -484,6 +484,18 @@ int cfdgpu_surface(const cfdgpu_mesh* m, const double* u, const double* v, const
   });
 }
 
+int cfdgpu_solver_surface(cfdgpu_solver* s, double speed, double alpha_deg, double density, double pref, double chord,
+                          double* rows, int capacity, int* count, cfdgpu_forces* forces) {
+  if (!s) return fail(CFDGPU_INVALID_ARGUMENT, "null solver");
+  if (capacity < 0 || (capacity > 0 && !rows)) return fail(CFDGPU_INVALID_ARGUMENT, "bad row buffer");
+  return guarded([&]() -> int {
+    const int rc = s->engine->surface(speed, alpha_deg, density, pref, chord, rows, capacity, count, forces);
+    if (rc == CFDGPU_NOT_FOUND) return fail(rc, "the mesh has fewer than four wall faces");
+    if (rc == CFDGPU_INITIALIZATION_FAILURE) return fail(rc, "the solver is not initialised");
+    return rc;
+  });
+}
+
 int cfdgpu_polar_sweep(const cfdgpu_mesh* m, const cfdgpu_settings* settings, int model,
                        const cfdgpu_sst_constants* sst, double speed, double density, double reynolds, double chord,
                        double pref, const double* alphas, int count, int continued, const cfdgpu_stop_rule* rule,

@@ -140,10 +140,11 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
       if (h.psolve == 1) launchMixedPressure<1>(d, cfg, h, s);
       else launchMixedPressure<0>(d, cfg, h, s);
     } else {
-      hipLaunchKernelGGL(k_cg_init, grid, block, 0, s, d);
+      const dim3 cgGrid(cfg.maxCgBlocks, cfg.batch);
+      hipLaunchKernelGGL(k_cg_init, cgGrid, block, 0, s, d);
       for (int it = 0; it < h.cgMax; ++it) {
-        hipLaunchKernelGGL(k_cg_a, grid, block, 0, s, d, it);
-        hipLaunchKernelGGL(k_cg_b, grid, block, 0, s, d, it);
+        hipLaunchKernelGGL(k_cg_a, cgGrid, block, 0, s, d, it);
+        hipLaunchKernelGGL(k_cg_b, cgGrid, block, 0, s, d, it);
       }
     }
     end(CFDGPU_STAGE_PRESSURE_SOLVE);

@@ -108,6 +108,7 @@ __host__ __device__ constexpr int skewIndex(int i, int j, int R) { return (i + j
 __host__ __device__ constexpr int skewLength(int nI, int nJ, int R) { return (skewSteps(nI, nJ) + 1) * R + R; }
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
+constexpr int kCgCells = 4;          // cells per thread in the CG launches
 
 struct Inst {
   int nI, nJ, N;
@@ -125,6 +126,7 @@ struct Inst {
   double stopTol;  // device-side stop rule (worst() < stopTol); <= 0 disables
   int stopMaxIter;
   int nBlocks;     // blocks of a cell-parallel launch for this instance
+  int cgBlocks;    // blocks of a CG launch (kBlock x kCgCells cells each)
 
   // geometry (read-only)
   const double* V;

@@ -617,6 +617,16 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
 #pragma unroll
     for (int q = 0; q < D; ++q) ring[q] = GsCell1{};
     int seenBelow = kBegin - 65;  // the wave underneath has finished this step (its start - 1)
+    // lane 0's value below for the coming step and, in wave 0, the wake
+    // partners of this lane's next two cells: read one (two) steps ahead so
+    // the step does not wait for LDS
+    int prCur = -1, prNext = -1;
+    double lowNext = 0.0;
+    if (w == 0) {
+      prCur = partner[min(max(kFirst - lane, 0), nI - 1)];
+      prNext = partner[min(max(kFirst + 1 - lane, 0), nI - 1)];
+      lowNext = row0[max(prCur, 0)];
+    }
     for (int r = 0; r < rings; ++r) {
 #pragma unroll
       for (int q = 0; q < D; ++q) {
@@ -626,22 +636,26 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         const int i = k - lane;
         const bool on = active && i >= 0 && i < nI;
         const GsCell1& g = ring[q];
-        // lane 0's value below: the wake partner (wave 0) or lane 63 of the wave underneath
-        double low;
-        int pr = -1;
+        // the reads issued on the last step are complete, and so are its
+        // writes: post its hand-off value (counters run one step behind)
+        ldsOrder();
+        ldsPost((k - 1 >= kBegin && l == 63) ? &posted[w] : &dummyInt[lane], k - 1);
+        const double low = lowNext;
+        const int pr = prCur;
+        computedHere(low);  // every lane: no branch around the value
+        // read ahead for step k + 1
         if (w == 0) {
-          pr = partner[min(max(i, 0), nI - 1)];
-          low = row0[max(pr, 0)];
+          prCur = prNext;
+          lowNext = row0[max(prCur, 0)];  // forwarded below if this step writes that cell
+          prNext = partner[min(max(i + 2, 0), nI - 1)];
         } else {
-          const int i0 = k - 64 * w;  // lane 0's cell
-          if (ok && i0 >= 0 && i0 < nI && seenBelow < k - 1) {
-            ok = waitStep(&posted[w - 1], k - 1, stall);
+          const int i1 = k + 1 - 64 * w;  // lane 0's cell on the next step
+          if (ok && i1 >= 0 && i1 < nI && seenBelow < k) {
+            ok = waitStep(&posted[w - 1], k, stall);
             seenBelow = __builtin_amdgcn_readfirstlane(ldsPoll(&posted[w - 1]));
-            ldsOrder();
           }
-          low = hand[(w - 1) * kSweepHand + ((k - 1) & (kSweepHand - 1))];
+          lowNext = hand[(w - 1) * kSweepHand + (k & (kSweepHand - 1))];
         }
-        computedHere(low);  // read by every lane: no branch around the LDS read
         const bool hasBelow = (lane > 0) || (pr >= 0);
         const double below = (l > 0) ? up : low;
         // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
@@ -658,14 +672,14 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         const double xn = (fabs(g.aP) > 0.0) ? quotient : g.xC;
         xNew[2 * (on ? k * R + lane : (steps + 1) * R + laneG) + S.half] = xn;
         *((on && lane == 0) ? &row0[max(i, 0)] : &dummy[lane]) = xn;
-        // post lane 63's value for the wave above; every 16 steps make sure
-        // the ring slots of the next 16 are free (read by the wave above)
-        const bool started = k >= kBegin;  // the ring-filling steps post nothing
-        const bool feeds = (w + 1 < W) && started;
+        // the value read ahead is this cell's old one when the next cell's
+        // partner is this cell (the write above comes after that read)
+        if (w == 0) lowNext = (on && prCur == i) ? xn : lowNext;
+        // lane 63's value for the wave above; every 16 steps make sure the
+        // ring slots of the next 16 are free (read by the wave above)
+        const bool feeds = (w + 1 < W) && k >= kBegin;
         if (feeds && (k & 15) == 0 && ok) ok = waitStep(&posted[w + 1], k + 16 - kSweepHand, stall);
         *((feeds && l == 63) ? &hand[w * kSweepHand + (k & (kSweepHand - 1))] : &dummy[lane]) = xn;
-        ldsOrder();
-        ldsPost((started && l == 63) ? &posted[w] : &dummyInt[lane], k);
         up = waveShiftUp(xn);
         xw = xn;
         // the refill may not be hoisted above the shift: the slot's old
@@ -676,6 +690,7 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         gsFetch1<SepB>(S, xOld, R, steps, laneG, k + D, ring[q]);
       }
     }
+    ldsOrder();  // the last step's hand-off value is written
     // nobody waits on this wave for the rest of the sweep
     if (l == 63) ldsPost(&posted[w], steps + kSweepHand);
     __syncthreads();  // this sweep's writes are the next sweep's old values
@@ -1019,9 +1034,10 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
 // (LinearSystem.cpp:118-210), two launches per iteration:
 //   A: p_k = z + beta p_{k-1} (halo recomputed locally), Ap, p.Ap
 //   B: x += alpha p, r -= alpha Ap, |r|_1, z = r / diag, r.z
-// The scalars live in device memory and the last block of each launch
-// finalises them, so the host never waits inside the solve; once the test
-// passes (or the iteration breaks down) every later launch is a no-op.
+// The scalars live in device memory: every block of a launch sums the
+// previous launch's per-block partials, so the host never waits inside the
+// solve; once the test passes (or the iteration breaks down) every later
+// launch is a no-op.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c, int i, int j) {
@@ -1038,28 +1054,48 @@ __device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c
   return y;
 }
 
+// The CG launches give each thread kCgCells cells (c = tile * kBlock *
+// kCgCells + q * kBlock + threadIdx.x, q in order), so a launch has a quarter
+// of the blocks of the cell-parallel kernels and every block of the next
+// launch sums a quarter of the partials. The per-thread sums run over q in
+// order before the block sum: the result is fixed run to run.
+#define CFD_CG_PROLOGUE                                                 \
+  const Inst& I = insts[blockIdx.y];                                    \
+  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;               \
+  if (I.flags[fStopped]) return;                                        \
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);   \
+  const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x); \
+  const int nI = I.nI;                                                  \
+  const int nJ = I.nJ;                                                  \
+  (void)nJ;
+
 __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
-  CFD_PROLOGUE
+  CFD_CG_PROLOGUE
   __shared__ double sm[16];
   double mean = 0.0;
   if (!I.hasPRef) mean = blockSumPartials(partialSlot(I, sSourceSum), I.nBlocks, sm) / static_cast<double>(I.N);
   double part[2] = {0.0, 0.0};
-  if (live) {
-    double b = I.pb[c];
-    if (!I.hasPRef) {
-      b -= mean;
-      I.pb[c] = b;
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    const int c = base + q * kBlock;
+    if (c < I.N) {
+      double b = I.pb[c];
+      if (!I.hasPRef) {
+        b -= mean;
+        I.pb[c] = b;
+      }
+      const double r = b - 0.0;  // x0 = 0, so A x0 = 0
+      const double d = I.pP[c];
+      const double z = (fabs(d) > 0.0) ? r / d : r;
+      I.r[c] = r;
+      I.z[c] = z;
+      I.pd0[c] = z;
+      I.x[c] = 0.0;
+      part[0] += fabs(r);
+      part[1] += r * z;
     }
-    const double r = b - 0.0;  // x0 = 0, so A x0 = 0
-    const double d = I.pP[c];
-    const double z = (fabs(d) > 0.0) ? r / d : r;
-    I.r[c] = r;
-    I.z[c] = z;
-    I.pd0[c] = z;
-    I.x[c] = 0.0;
-    part[0] = fabs(r);
-    part[1] = r * z;
   }
+  (void)nI;
   blockSum<2>(part, sm);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
@@ -1074,45 +1110,54 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
 // Iteration `it`, launch A. Finishes the test of iteration it-1 from the
 // partials of launch B (|r|_1 <= tol |r0|_1, then the r.z guard), forms beta,
 // then p = z + beta p_old (neighbours' p recomputed locally, so no second
-// launch is needed before the stencil), Ap and p.Ap. The cell's operands do
+// launch is needed before the stencil), Ap and p.Ap. The cells' operands do
 // not depend on beta, so they are loaded before the partials are summed and
 // the two memory round trips overlap.
 __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
-  CFD_PROLOGUE
+  CFD_CG_PROLOGUE
   if (I.flags[fCgDone]) return;
   __shared__ double sm[16];
-  // operands: z and p_old at the cell and its four neighbours, the matrix row
+  // operands: z and p_old at each cell and its four neighbours, the matrix row
   const CFD_GLOBAL double* z = gmem(static_cast<const double*>(I.z));
   const CFD_GLOBAL double* pOld = gmem(static_cast<const double*>((it & 1) ? I.pd0 : I.pd1));
   const CFD_GLOBAL double* pFirst = gmem(static_cast<const double*>(I.pd0));
   const CFD_GLOBAL double* src = (it == 0) ? pFirst : pOld;  // iteration 0 uses p0 = z (in pd0)
-  double zc = 0.0, zS = 0.0, zN = 0.0, zW = 0.0, zE = 0.0;
-  double oc = 0.0, oS = 0.0, oN = 0.0, oW = 0.0, oE = 0.0;
-  double aP = 0.0, aS = 0.0, aN = 0.0, aW = 0.0, aE = 0.0;
-  bool hasS = false;
-  if (live) {
+  const CFD_GLOBAL double* pP = gmem(static_cast<const double*>(I.pP));
+  const CFD_GLOBAL double* pN = gmem(static_cast<const double*>(I.pN));
+  const CFD_GLOBAL double* pE = gmem(static_cast<const double*>(I.pE));
+  const CFD_GLOBAL double* pWk = gmem(static_cast<const double*>(I.pWk));
+  double zc[kCgCells], zS[kCgCells], zN[kCgCells], zW[kCgCells], zE[kCgCells];
+  double oc[kCgCells], oS[kCgCells], oN[kCgCells], oW[kCgCells], oE[kCgCells];
+  double aP[kCgCells], aS[kCgCells], aN[kCgCells], aW[kCgCells], aE[kCgCells];
+  bool hasS[kCgCells], hasN[kCgCells], hasW[kCgCells], hasE[kCgCells], live[kCgCells];
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    const int c = base + q * kBlock;
+    live[q] = c < I.N;
+    const int cc = live[q] ? c : 0;
+    const int i = cc % nI, j = cc / nI;
     const int pr = (j == 0) ? I.partner[i] : -1;
-    hasS = (j > 0) || (pr >= 0);
-    const int cS = (j > 0) ? c - nI : (pr >= 0 ? pr : c);
-    const int cN = (j < nJ - 1) ? c + nI : c;
-    const int cW = (i > 0) ? c - 1 : c;
-    const int cE = (i < nI - 1) ? c + 1 : c;
-    const CFD_GLOBAL double* pP = gmem(static_cast<const double*>(I.pP));
-    const CFD_GLOBAL double* pN = gmem(static_cast<const double*>(I.pN));
-    const CFD_GLOBAL double* pE = gmem(static_cast<const double*>(I.pE));
-    const CFD_GLOBAL double* pWk = gmem(static_cast<const double*>(I.pWk));
-    aP = pP[c];
-    aS = (j > 0) ? pN[c - nI] : pWk[c];
-    aN = pN[c];
-    aW = pE[cW == c ? c : c - 1];
-    aE = pE[c];
-    oc = src[c]; oS = src[cS]; oN = src[cN]; oW = src[cW]; oE = src[cE];
-    if (it > 0) { zc = z[c]; zS = z[cS]; zN = z[cN]; zW = z[cW]; zE = z[cE]; }
+    hasS[q] = (j > 0) || (pr >= 0);
+    hasN[q] = j < nJ - 1;
+    hasW[q] = i > 0;
+    hasE[q] = i < nI - 1;
+    const int cS = (j > 0) ? cc - nI : (pr >= 0 ? pr : cc);
+    const int cN = hasN[q] ? cc + nI : cc;
+    const int cW = hasW[q] ? cc - 1 : cc;
+    const int cE = hasE[q] ? cc + 1 : cc;
+    aP[q] = pP[cc];
+    aS[q] = (j > 0) ? pN[cc - nI] : pWk[cc];
+    aN[q] = pN[cc];
+    aW[q] = pE[cW == cc ? cc : cc - 1];
+    aE[q] = pE[cc];
+    oc[q] = src[cc]; oS[q] = src[cS]; oN[q] = src[cN]; oW[q] = src[cW]; oE[q] = src[cE];
+    zc[q] = zS[q] = zN[q] = zW[q] = zE[q] = 0.0;
+    if (it > 0) { zc[q] = z[cc]; zS[q] = z[cS]; zN[q] = z[cN]; zW[q] = z[cW]; zE[q] = z[cE]; }
   }
   double sums[2];
   {
     const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
-    blockSumPartials<2>(slots, I.nBlocks, sm, sums);
+    blockSumPartials<2>(slots, I.cgBlocks, sm, sums);
   }
   const double nrm = sums[0];
   const double rzNew = sums[1];
@@ -1136,57 +1181,66 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
     return;
   }
   double part[1] = {0.0};
-  if (live) {
-    // p = z + beta p_old (LinearSystem.cpp:199-201); iteration 0 has p = z
-    auto pv = [&](double zq, double oq) { return (it == 0) ? oq : zq + beta * oq; };
-    const double pc = pv(zc, oc);
+  // p = z + beta p_old (LinearSystem.cpp:199-201); iteration 0 has p = z
+  auto pv = [&](double zq, double oq) { return (it == 0) ? oq : zq + beta * oq; };
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    if (!live[q]) continue;
+    const int c = base + q * kBlock;
+    const double pc = pv(zc[q], oc[q]);
     // LinearSystem::multiply order: diagonal, then the faces of the cell
     // (below/wake, above, left, right as the structured sweep visits them)
-    double y = aP * pc;
-    if (hasS) y += aS * pv(zS, oS);
-    if (j < nJ - 1) y += aN * pv(zN, oN);
-    if (i > 0) y += aW * pv(zW, oW);
-    if (i < nI - 1) y += aE * pv(zE, oE);
+    double y = aP[q] * pc;
+    if (hasS[q]) y += aS[q] * pv(zS[q], oS[q]);
+    if (hasN[q]) y += aN[q] * pv(zN[q], oN[q]);
+    if (hasW[q]) y += aW[q] * pv(zW[q], oW[q]);
+    if (hasE[q]) y += aE[q] * pv(zE[q], oE[q]);
     if (it > 0) gmem((it & 1) ? I.pd1 : I.pd0)[c] = pc;
     gmem(I.Ap)[c] = y;
-    part[0] = pc * y;
+    part[0] += pc * y;
   }
   blockSum<1>(part, sm);
   if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
 }
 
 // Iteration `it`, launch B: alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap,
-// z = r / diag, and the partials of |r|_1 and r.z for the next launch A. The
-// cell's operands are loaded before the p.Ap partials are summed.
+// z = r / a_P, and the partials of |r|_1 and r.z (LinearSystem.cpp:160-198).
 __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
-  CFD_PROLOGUE
+  CFD_CG_PROLOGUE
   if (I.flags[fCgDone]) return;
   __shared__ double sm[16];
-  double xc = 0.0, pc = 0.0, rc = 0.0, apc = 0.0, d = 0.0;
-  if (live) {
-    xc = gmem(static_cast<const double*>(I.x))[c];
-    pc = gmem(static_cast<const double*>((it & 1) ? I.pd1 : I.pd0))[c];
-    rc = gmem(static_cast<const double*>(I.r))[c];
-    apc = gmem(static_cast<const double*>(I.Ap))[c];
-    d = gmem(static_cast<const double*>(I.pP))[c];
+  double xc[kCgCells], pc[kCgCells], rc[kCgCells], apc[kCgCells], d[kCgCells];
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    const int c = base + q * kBlock;
+    const int cc = (c < I.N) ? c : 0;
+    xc[q] = gmem(static_cast<const double*>(I.x))[cc];
+    pc[q] = gmem(static_cast<const double*>((it & 1) ? I.pd1 : I.pd0))[cc];
+    rc[q] = gmem(static_cast<const double*>(I.r))[cc];
+    apc[q] = gmem(static_cast<const double*>(I.Ap))[cc];
+    d[q] = gmem(static_cast<const double*>(I.pP))[cc];
   }
-  const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.nBlocks, sm);
+  const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.cgBlocks, sm);
   if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
     if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
     return;
   }
   const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
   double part[2] = {0.0, 0.0};
-  if (live) {
-    const double x = xc + alpha * pc;
-    const double r = rc - alpha * apc;
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    const int c = base + q * kBlock;
+    if (c >= I.N) continue;
+    const double x = xc[q] + alpha * pc[q];
+    const double r = rc[q] - alpha * apc[q];
     gmem(I.x)[c] = x;
     gmem(I.r)[c] = r;
-    const double z = (fabs(d) > 0.0) ? r / d : r;
-    gmem(I.z)[c] = z;
-    part[0] = fabs(r);
-    part[1] = r * z;
+    const double zq = (fabs(d[q]) > 0.0) ? r / d[q] : r;
+    gmem(I.z)[c] = zq;
+    part[0] += fabs(r);
+    part[1] += r * zq;
   }
+  (void)nI;
   blockSum<2>(part, sm);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = part[0];

@@ -1,5 +1,6 @@
 // engine.cpp - host setup and device residency of one SIMPLE instance.
 #include "engine.hpp"
+#include "post.hpp"
 
 #include <algorithm>
 #include <array>
@@ -114,6 +115,8 @@ Engine::Engine(const Grid& grid, std::vector<int> kind, std::vector<Vec2> bcVelo
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (dWall_) (void)hipFree(dWall_);
+  if (dSurf_) (void)hipFree(dSurf_);
   if (dInst_) (void)hipFree(dInst_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -290,11 +293,13 @@ void Engine::allocate() {
   h_.nJ = grid_.nJ;
   h_.N = grid_.cells();
   h_.nBlocks = (h_.N + kBlock - 1) / kBlock;
+  h_.cgBlocks = (h_.N + kBlock * kCgCells - 1) / (kBlock * kCgCells);
   h_.lineBlocks = (h_.nI + 63) / 64;
   h_.gsR = gsThreadsFor(grid_.nJ);
   h_.gsSkewLen = skewLength(grid_.nI, grid_.nJ, h_.gsR);
   check(hipMalloc(&dInst_, sizeof(Inst)), "hipMalloc (descriptor)");
   cfg_.maxBlocks = h_.nBlocks;
+  cfg_.maxCgBlocks = h_.cgBlocks;
   cfg_.maxLineBlocks = h_.lineBlocks;
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
@@ -708,6 +713,73 @@ int Engine::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
   return CFDGPU_OK;
 }
 
+int Engine::surface(double speed, double alphaDeg, double density, double pref, double chord, double* rows,
+                    int capacity, int* count, cfdgpu_forces* out) {
+  if (!initialised_) return CFDGPU_INITIALIZATION_FAILURE;
+  if (!wallBuilt_) {
+    std::vector<WallStation> table;
+    hasWall_ = wallTable(grid_, table, le_);
+    wallBuilt_ = true;
+    if (hasWall_) {
+      nWall_ = static_cast<int>(table.size());
+      check(hipMalloc(&dWall_, sizeof(WallStation) * table.size()), "hipMalloc (wall table)");
+      // rows (12 per station), force terms (6 per station), the forces record
+      check(hipMalloc(&dSurf_, sizeof(double) * (18 * table.size() + kForceDoubles) + sizeof(int) * kForceInts),
+            "hipMalloc (surface)");
+      check(hipMemcpyAsync(dWall_, table.data(), sizeof(WallStation) * table.size(), hipMemcpyHostToDevice, stream_),
+            "upload wall table");
+      check(hipStreamSynchronize(stream_), "wall table sync");
+    }
+  }
+  if (!hasWall_) return CFDGPU_NOT_FOUND;  // fewer than four wall faces (SurfaceData.cpp:27-59)
+  const size_t n = static_cast<size_t>(nWall_);
+  double* dRows = dSurf_;
+  double* dTerms = dRows + 12 * n;
+  double* dFd = dTerms + 6 * n;
+  int* dFi = reinterpret_cast<int*>(dFd + kForceDoubles);
+  const double q = 0.5 * density * speed * speed;
+  const double a = alphaDeg * 3.141592653589793238462643383279502884 / 180.0;
+  launchSurface(h_.u, h_.v, h_.p, h_.mu, static_cast<const WallStation*>(dWall_), nWall_, le_, q, pref, chord,
+                std::cos(a), std::sin(a), dRows, dTerms, dFd, dFi, stream_);
+  check(hipGetLastError(), "launch surface");
+  std::vector<double> hostRows(12 * n);
+  double fd[kForceDoubles];
+  int fi[kForceInts];
+  check(hipMemcpyAsync(hostRows.data(), dRows, sizeof(double) * hostRows.size(), hipMemcpyDeviceToHost, stream_),
+        "download stations");
+  check(hipMemcpyAsync(fd, dFd, sizeof(fd), hipMemcpyDeviceToHost, stream_), "download forces");
+  check(hipMemcpyAsync(fi, dFi, sizeof(fi), hipMemcpyDeviceToHost, stream_), "download forces");
+  check(hipStreamSynchronize(stream_), "surface sync");
+  for (size_t r = 0; r < n; ++r)
+    if (rows && static_cast<long long>(r + 1) * 12 <= capacity) std::copy_n(hostRows.data() + 12 * r, 12, rows + 12 * r);
+  if (count) *count = nWall_;
+  if (out) {
+    cfdgpu_forces f{};
+    f.cl = fd[0];
+    f.cd = fd[1];
+    f.cm = fd[2];
+    f.cd_pressure = fd[3];
+    f.cd_friction = fd[4];
+    f.lift = fd[5];
+    f.drag = fd[6];
+    f.pitching_moment = fd[7];
+    f.fx_pressure = fd[8];
+    f.fy_pressure = fd[9];
+    f.fx_friction = fd[10];
+    f.fy_friction = fd[11];
+    f.upper_separation = fd[12];
+    f.lower_separation = fd[13];
+    f.stagnation_xc = fd[14];
+    f.n_upper = fi[0];
+    f.n_lower = fi[1];
+    f.stations = fi[2];
+    f.upper_separation_face = fi[3];
+    f.lower_separation_face = fi[4];
+    *out = f;
+  }
+  return CFDGPU_OK;
+}
+
 int Engine::getField(double* u, double* v, double* p, double* mu) const {
   const size_t N = static_cast<size_t>(grid_.cells());
   auto down = [&](double* dst, const double* src) {
@@ -859,7 +931,7 @@ Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
   if (members_.size() > 65535) throw std::invalid_argument("too many solvers for one batch");
   const Inst& a = members_.front()->refreshInst();
   size_t lds = 0;
-  int lanes = 64, blocks = 1, maxNI = 0;
+  int lanes = 64, blocks = 1, cgBlocks = 1, maxNI = 0;
   for (Engine* e : members_) {
     if (!e || !e->ready()) throw std::invalid_argument("every solver of a batch must be initialised");
     const Inst& b = e->refreshInst();
@@ -870,12 +942,13 @@ Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
                                   "pressure anchoring");
     lanes = std::max(lanes, b.gsR);
     blocks = std::max(blocks, b.nBlocks);
+    cgBlocks = std::max(cgBlocks, b.cgBlocks);
     maxNI = std::max(maxNI, b.nI);
   }
   lds = gsLdsFor(0, maxNI, lanes);
   if (a.gsOrder == CFDGPU_GS_LEXICOGRAPHIC && lds > 64 * 1024)
     throw std::invalid_argument("batch too wide for the lexicographic sweep's LDS; use CFDGPU_GS_REDBLACK");
-  cfg_ = LaunchCfg{blocks, 1, static_cast<int>(members_.size()), lanes, lds};
+  cfg_ = LaunchCfg{blocks, cgBlocks, 1, static_cast<int>(members_.size()), lanes, lds};
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   check(hipMalloc(&dInsts_, sizeof(Inst) * members_.size()), "hipMalloc (batch descriptors)");
   host_.resize(members_.size());

@@ -23,6 +23,7 @@ namespace cfdgpu {
 // Kernel launchers (defined in launch.hip).
 struct LaunchCfg {
   int maxBlocks;   // grid.x for cell-parallel kernels
+  int maxCgBlocks; // grid.x for the CG kernels
   int maxLineBlocks;  // grid.x for the column (line-preconditioner) kernels
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
@@ -41,6 +42,11 @@ void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hip
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
 // distance from each of n centroids to the nearest of nw wall-face centres
 void launchWallDistance(const double2* centroid, int n, const double2* walls, int nw, double* out, hipStream_t s);
+struct WallStation;
+// surface stations and forces of one field (surface.hip)
+void launchSurface(const double* u, const double* v, const double* p, const double* mu, const WallStation* st,
+                   int nWall, int le, double q, double pref, double chord, double ca, double sa, double* rows,
+                   double* terms, double* fd, int* fi, hipStream_t s);
 // clears the stop flag of `count` instances (a plain iterate never stops by itself)
 void launchResumeAll(const Inst* dInst, int count, hipStream_t s);
 int gsThreadsFor(int nJ);
@@ -83,6 +89,10 @@ class Engine {
   int setTurbulence(const double* k, const double* omega, const double* mut);
   int turbulenceInfo(double* kIn, double* wIn, double* yplus) const;
   int divergence(double* out) const;
+  // surface distribution and forces of the current field, on the device; the
+  // same numbers as surfaceAndForces on the downloaded field
+  int surface(double speed, double alphaDeg, double density, double pref, double chord, double* rows, int capacity,
+              int* count, cfdgpu_forces* out);
   int debugArray(const std::string& name, double* out, int capacity) const;
 
   const Grid& grid() const { return grid_; }
@@ -141,6 +151,11 @@ class Engine {
   hipStream_t stream_{nullptr};
   LaunchCfg cfg_{};
   std::unique_ptr<StageClock> clock_;
+  // device surface integration: the wall table and its scratch, made on first use
+  bool wallBuilt_{false}, hasWall_{false};
+  int nWall_{0}, le_{0};
+  void* dWall_{nullptr};
+  double* dSurf_{nullptr};
 };
 
 // Several solves advanced together: one launch per stage for all of them

@@ -105,10 +105,10 @@ double separation(const std::vector<Station>& s, int* face) {
 
 }  // namespace
 
-int surfaceAndForces(const Grid& g, const double* u, const double* v, const double* p, const double* mu, double speed,
-                     double alphaDeg, double density, double pref, double chord, double* rows, int capacity,
-                     int* count, cfdgpu_forces* out) {
-  // collectWallFaces (SurfaceData.cpp:27-59): the j = 0 wall faces in i order
+namespace {
+
+// collectWallFaces (SurfaceData.cpp:27-59): the j = 0 wall faces in i order
+std::vector<WallFace> collectWall(const Grid& g) {
   std::vector<WallFace> wall;
   for (int i = 0; i < g.nI; ++i) {
     if (g.bottomType[static_cast<size_t>(i)] != kWallB) continue;
@@ -125,7 +125,11 @@ int surfaceAndForces(const Grid& g, const double* u, const double* v, const doub
     w.len = l;
     wall.push_back(w);
   }
-  if (wall.size() < 4) return 3;  // NotFound
+  return wall;
+}
+
+// the wall face whose first node lies furthest forward (SurfaceData.cpp:231-249)
+size_t leadingEdge(const Grid& g, const std::vector<WallFace>& wall) {
   size_t le = 0;
   double fwd = std::numeric_limits<double>::max();
   for (size_t k = 0; k < wall.size(); ++k) {
@@ -135,6 +139,68 @@ int surfaceAndForces(const Grid& g, const double* u, const double* v, const doub
       le = k;
     }
   }
+  return le;
+}
+
+}  // namespace
+
+bool wallTable(const Grid& g, std::vector<WallStation>& out, int& leOut) {
+  const std::vector<WallFace> wall = collectWall(g);
+  out.clear();
+  if (wall.size() < 4) return false;
+  const size_t le = leadingEdge(g, wall);
+  out.resize(wall.size());
+  for (size_t k = 0; k < wall.size(); ++k) {
+    const WallFace& w = wall[k];
+    WallStation& s = out[k];
+    s.cx = w.centre.x;
+    s.cy = w.centre.y;
+    s.nx = w.normal.x;
+    s.ny = w.normal.y;
+    s.tx = w.tangent.x;
+    s.ty = w.tangent.y;
+    s.len = w.len;
+    s.face = w.i;
+    s.owner = w.owner;
+    s.pad = 0;
+    // the static part of evaluate(): wall distance and the second cell
+    const Vec2 off = g.centroid[static_cast<size_t>(w.owner)] - w.centre;
+    s.wd = std::abs(dot(off, w.normal));
+    s.second = -1;
+    s.od = 0.0;
+    if (s.wd > 0.0) {
+      const int second = secondCell(g, w);
+      if (second >= 0) {
+        const Vec2 oo = g.centroid[static_cast<size_t>(second)] - w.centre;
+        const double od = std::abs(dot(oo, w.normal));
+        if (od > s.wd * 1.05) {
+          s.second = second;
+          s.od = od;
+        }
+      }
+    }
+  }
+  // arc lengths in station order, as the arc lambda of surfaceAndForces sums them
+  double t = 0.0;
+  for (size_t k = le; k < wall.size(); ++k) {
+    if (k > le) t += distance(wall[k - 1].centre, wall[k].centre);
+    out[k].arc = t;
+  }
+  t = 0.0;
+  for (size_t k = le; k-- > 0;) {
+    if (k + 1 < le) t += distance(wall[k + 1].centre, wall[k].centre);
+    out[k].arc = t;
+  }
+  leOut = static_cast<int>(le);
+  return true;
+}
+
+int surfaceAndForces(const Grid& g, const double* u, const double* v, const double* p, const double* mu, double speed,
+                     double alphaDeg, double density, double pref, double chord, double* rows, int capacity,
+                     int* count, cfdgpu_forces* out) {
+  const std::vector<WallFace> wall = collectWall(g);
+  if (wall.size() < 4) return 3;  // NotFound
+  const size_t le = leadingEdge(g, wall);
   const double q = 0.5 * density * speed * speed;
   size_t stag = 0;
   double hp = -std::numeric_limits<double>::max();

@@ -208,8 +208,16 @@ void Server::finish(Slot& s, int status, int iterations, double continuity) {
   const ServeRequest& r = requests_[static_cast<size_t>(s.ticket)];
   const Grid& g = *s.grid;
   const size_t N = static_cast<size_t>(g.cells());
-  std::vector<double> u(N), v(N), p(N), mu(N);
-  s.engine->getField(u.data(), v.data(), p.data(), mu.data());
+  // the fields come to the host only when they are kept or donated
+  const bool donate = status == 0 && opt_.warmStart;
+  std::vector<double> u, v, p, mu;
+  if (opt_.keepFields || donate) {
+    u.resize(N);
+    v.resize(N);
+    p.resize(N);
+    mu.resize(N);
+    s.engine->getField(u.data(), v.data(), p.data(), mu.data());
+  }
   cfdgpu_serve_result res{};
   res.ticket = s.ticket;
   res.status = status;
@@ -219,11 +227,11 @@ void Server::finish(Slot& s, int status, int iterations, double continuity) {
   res.cells = g.cells();
   res.continuity = continuity;
   res.upper_separation = res.lower_separation = -1.0;
-  std::vector<double> rows(12 * static_cast<size_t>(g.nI));
+  // surface and forces on the device (the host routine's numbers, without the fields)
   cfdgpu_forces f{};
   int count = 0;
-  if (surfaceAndForces(g, u.data(), v.data(), p.data(), mu.data(), r.speed, r.alphaDeg, r.density,
-                       r.referencePressure, r.chord, rows.data(), g.nI, &count, &f) == 0) {
+  if (s.engine->surface(r.speed, r.alphaDeg, r.density, r.referencePressure, r.chord, nullptr, 0, &count, &f) ==
+      CFDGPU_OK) {
     res.cl = f.cl;
     res.cd = f.cd;
     res.cd_pressure = f.cd_pressure;
@@ -246,7 +254,7 @@ void Server::finish(Slot& s, int status, int iterations, double continuity) {
     for (const auto* a : {&u, &v, &p, &mu}) all.insert(all.end(), a->begin(), a->end());
     kept_[s.ticket] = std::move(all);
   }
-  if (status == 0 && opt_.warmStart) {
+  if (donate) {
     const DonorKey key{r.m, r.p, r.t, r.chord, g.nI, g.nJ, r.speed, r.density};
     auto& list = donors_[key];
     list.push_back(Donor{r.alphaDeg, r.reynolds, s.ticket, std::move(u), std::move(v), std::move(p)});
@@ -278,15 +286,16 @@ void Server::launchChunk() {
   if (active_.empty()) return;
   if (rosterDirty_) {
     host_.resize(active_.size());
-    int lanes = 64, blocks = 1, lineBlocks = 1, maxNI = 0;
+    int lanes = 64, blocks = 1, cgBlocks = 1, lineBlocks = 1, maxNI = 0;
     for (size_t q = 0; q < active_.size(); ++q) {
       host_[q] = active_[q]->engine->refreshInst();
       lanes = std::max(lanes, host_[q].gsR);
       blocks = std::max(blocks, host_[q].nBlocks);
+      cgBlocks = std::max(cgBlocks, host_[q].cgBlocks);
       lineBlocks = std::max(lineBlocks, host_[q].lineBlocks);
       maxNI = std::max(maxNI, host_[q].nI);
     }
-    cfg_ = LaunchCfg{blocks, lineBlocks, static_cast<int>(active_.size()), lanes, gsLdsFor(0, maxNI, lanes)};
+    cfg_ = LaunchCfg{blocks, cgBlocks, lineBlocks, static_cast<int>(active_.size()), lanes, gsLdsFor(0, maxNI, lanes)};
     if (opt_.settings.gs_ordering == CFDGPU_GS_LEXICOGRAPHIC && cfg_.gsLds > 64 * 1024)
       throw std::invalid_argument("resident meshes too wide for the lexicographic sweep's LDS");
     hipCheck(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),

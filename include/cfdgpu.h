@@ -269,6 +269,13 @@ int cfdgpu_surface(const cfdgpu_mesh* mesh, const double* u, const double* v, co
                    const double* mu, double speed, double alpha_deg, double density,
                    double reference_pressure, double chord, double* rows, int capacity,
                    int* station_count, cfdgpu_forces* forces);
+/* cfdgpu_surface of the solver's current field, computed on the device (the wall
+ * stations evaluated in parallel, the stagnation point, separation scans and force sums in
+ * the reference's order): the same rows and forces as cfdgpu_surface on the downloaded
+ * field, without downloading it. NOT_FOUND when the mesh has fewer than four wall faces. */
+int cfdgpu_solver_surface(cfdgpu_solver* solver, double speed, double alpha_deg, double density,
+                          double reference_pressure, double chord, double* rows, int capacity,
+                          int* station_count, cfdgpu_forces* forces);
 /* The reference's polar sweep (app/Panels.cpp:1137-1157, 1581-1800 with
  * post::nextSweepAction, Polar.cpp:86-124): for each angle a new solve from the
  * uniform field of that freestream, with velocity and pressure carried over from
