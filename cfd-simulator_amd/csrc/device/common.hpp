@@ -172,8 +172,12 @@ __device__ __forceinline__ void wavePartials(const double* const (&p)[NS], int n
       const CFD_GLOBAL double* g = gmem(p[q]);
 #pragma unroll
       for (int u = 0; u < kPartialUnroll; ++u) {
+        // every lane loads (clamped to the last partial, n >= 1) and the
+        // value is then chosen: no branch around the load, so all loads of
+        // the wave are in flight together
         const int b = b0 + u * 64 + lane;
-        t[q][u] = (b < n) ? g[b] : identity;
+        const double val = g[min(b, n - 1)];
+        t[q][u] = (b < n) ? val : identity;
       }
     }
 #pragma unroll
@@ -209,7 +213,10 @@ __device__ __forceinline__ double waveMinPartials(const double* p, int n, double
   return r[0];
 }
 
-// Sums of NS slots' partials, broadcast to every thread of the block.
+// Sums of NS slots' partials, broadcast to every thread of the block. Wave 0
+// sums them (up to kPartialUnroll loads per lane in flight per slot) while
+// the other waves wait at the barrier; splitting the sum over all four waves
+// measured slower.
 template <int NS>
 __device__ __forceinline__ void blockSumPartials(const double* const (&p)[NS], int n, double* sm,
                                                  double (&out)[NS]) {

@@ -108,7 +108,7 @@ __host__ __device__ constexpr int skewIndex(int i, int j, int R) { return (i + j
 __host__ __device__ constexpr int skewLength(int nI, int nJ, int R) { return (skewSteps(nI, nJ) + 1) * R + R; }
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
-constexpr int kCgCells = 4;          // cells per thread in the CG launches
+constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
 
 struct Inst {
   int nI, nJ, N;

@@ -1055,14 +1055,15 @@ __device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c
 }
 
 // The CG launches give each thread kCgCells cells (c = tile * kBlock *
-// kCgCells + q * kBlock + threadIdx.x, q in order), so a launch has a quarter
-// of the blocks of the cell-parallel kernels and every block of the next
-// launch sums a quarter of the partials. The per-thread sums run over q in
-// order before the block sum: the result is fixed run to run.
+// kCgCells + q * kBlock + threadIdx.x, q in order), so a launch has
+// 1 / kCgCells of the blocks of the cell-parallel kernels and every block of
+// the next launch sums that many fewer partials. The per-thread sums run over
+// q in order before the block sum: the result is fixed run to run.
 #define CFD_CG_PROLOGUE                                                 \
   const Inst& I = insts[blockIdx.y];                                    \
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;               \
-  if (I.flags[fStopped]) return;                                        \
+  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags)); \
+  if (flagsG[fStopped]) return;                                         \
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);   \
   const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x); \
   const int nI = I.nI;                                                  \
@@ -1075,22 +1076,29 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
   double mean = 0.0;
   if (!I.hasPRef) mean = blockSumPartials(partialSlot(I, sSourceSum), I.nBlocks, sm) / static_cast<double>(I.N);
   double part[2] = {0.0, 0.0};
+  double bq[kCgCells], dq[kCgCells];
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    const int cc = min(base + q * kBlock, I.N - 1);
+    bq[q] = gmem(static_cast<const double*>(I.pb))[cc];
+    dq[q] = gmem(static_cast<const double*>(I.pP))[cc];
+  }
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
     const int c = base + q * kBlock;
     if (c < I.N) {
-      double b = I.pb[c];
+      double b = bq[q];
       if (!I.hasPRef) {
         b -= mean;
-        I.pb[c] = b;
+        gmem(I.pb)[c] = b;
       }
       const double r = b - 0.0;  // x0 = 0, so A x0 = 0
-      const double d = I.pP[c];
+      const double d = dq[q];
       const double z = (fabs(d) > 0.0) ? r / d : r;
-      I.r[c] = r;
-      I.z[c] = z;
-      I.pd0[c] = z;
-      I.x[c] = 0.0;
+      gmem(I.r)[c] = r;
+      gmem(I.z)[c] = z;
+      gmem(I.pd0)[c] = z;
+      gmem(I.x)[c] = 0.0;
       part[0] += fabs(r);
       part[1] += r * z;
     }
@@ -1101,8 +1109,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
     partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
     partialSlot(I, sCgRz)[blockIdx.x] = part[1];
     if (blockIdx.x == 0) {
-      I.flags[fCgDone] = 0;
-      I.flags[fCgIters] = 0;
+      gmem(I.flags)[fCgDone] = 0;
+      gmem(I.flags)[fCgIters] = 0;
     }
   }
 }
@@ -1114,9 +1122,37 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
 // not depend on beta, so they are loaded before the partials are summed and
 // the two memory round trips overlap.
 __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
-  CFD_CG_PROLOGUE
-  if (I.flags[fCgDone]) return;
+  const Inst& I = insts[blockIdx.y];
+  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
+  const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
+  const int nI = I.nI;
+  const int nJ = I.nJ;
   __shared__ double sm[16];
+  // Every operand is read through the global address space, without a branch
+  // around its load: the flags, the CG scalars and the cells' wake partners
+  // are one memory round trip, the cells' operands and the partials a second
+  // (a FLAT or branch-guarded load would make a wait drain every load in
+  // flight).
+  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
+  const CFD_GLOBAL double* red = gmem(static_cast<const double*>(I.red));
+  const CFD_GLOBAL int* partnerG = gmem(static_cast<const int*>(I.partner));
+  int cq[kCgCells], iq[kCgCells], jq[kCgCells], prq[kCgCells];
+  bool live[kCgCells];
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    const int c = base + q * kBlock;
+    live[q] = c < I.N;
+    cq[q] = live[q] ? c : 0;
+    iq[q] = cq[q] % nI;
+    jq[q] = cq[q] / nI;
+    prq[q] = partnerG[iq[q]];
+  }
+  const int stopped = flagsG[fStopped];
+  const int cgDone = flagsG[fCgDone];
+  const double norm0 = red[sCgNorm0];
+  const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
+  if (stopped || cgDone) return;
   // operands: z and p_old at each cell and its four neighbours, the matrix row
   const CFD_GLOBAL double* z = gmem(static_cast<const double*>(I.z));
   const CFD_GLOBAL double* pOld = gmem(static_cast<const double*>((it & 1) ? I.pd0 : I.pd1));
@@ -1129,14 +1165,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
   double zc[kCgCells], zS[kCgCells], zN[kCgCells], zW[kCgCells], zE[kCgCells];
   double oc[kCgCells], oS[kCgCells], oN[kCgCells], oW[kCgCells], oE[kCgCells];
   double aP[kCgCells], aS[kCgCells], aN[kCgCells], aW[kCgCells], aE[kCgCells];
-  bool hasS[kCgCells], hasN[kCgCells], hasW[kCgCells], hasE[kCgCells], live[kCgCells];
+  bool hasS[kCgCells], hasN[kCgCells], hasW[kCgCells], hasE[kCgCells];
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
-    const int c = base + q * kBlock;
-    live[q] = c < I.N;
-    const int cc = live[q] ? c : 0;
-    const int i = cc % nI, j = cc / nI;
-    const int pr = (j == 0) ? I.partner[i] : -1;
+    const int cc = cq[q], i = iq[q], j = jq[q];
+    const int pr = (j == 0) ? prq[q] : -1;
     hasS[q] = (j > 0) || (pr >= 0);
     hasN[q] = j < nJ - 1;
     hasW[q] = i > 0;
@@ -1146,9 +1179,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
     const int cW = hasW[q] ? cc - 1 : cc;
     const int cE = hasE[q] ? cc + 1 : cc;
     aP[q] = pP[cc];
-    aS[q] = (j > 0) ? pN[cc - nI] : pWk[cc];
+    const double aBelow = pN[(j > 0) ? cc - nI : cc];
+    const double aWake = pWk[cc];
+    aS[q] = (j > 0) ? aBelow : aWake;
     aN[q] = pN[cc];
-    aW[q] = pE[cW == cc ? cc : cc - 1];
+    aW[q] = pE[cW];  // the left face's coefficient is stored with the cell to its left
     aE[q] = pE[cc];
     oc[q] = src[cc]; oS[q] = src[cS]; oN[q] = src[cN]; oW[q] = src[cW]; oE[q] = src[cE];
     zc[q] = zS[q] = zN[q] = zW[q] = zE[q] = 0.0;
@@ -1166,18 +1201,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
   if (it == 0) {
     if (!(nrm > 0.0)) done = true;  // already solved (LinearSystem.cpp:144-149)
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-      I.red[sCgNorm0] = nrm;
-      I.red[sCgRz0] = rzNew;
+      gmem(I.red)[sCgNorm0] = nrm;
+      gmem(I.red)[sCgRz0] = rzNew;
     }
   } else {
-    const double rzOld = I.red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
-    if (nrm <= I.cgTol * I.red[sCgNorm0]) done = true;
+    if (nrm <= I.cgTol * norm0) done = true;
     else if (!(fabs(rzOld) > 0.0)) done = true;
     else beta = rzNew / rzOld;
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) I.red[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) gmem(I.red)[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
   }
   if (done) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    if (threadIdx.x == 0 && blockIdx.x == 0) gmem(I.flags)[fCgDone] = 1;
     return;
   }
   double part[1] = {0.0};
@@ -1206,9 +1240,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
 // Iteration `it`, launch B: alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap,
 // z = r / a_P, and the partials of |r|_1 and r.z (LinearSystem.cpp:160-198).
 __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
-  CFD_CG_PROLOGUE
-  if (I.flags[fCgDone]) return;
+  const Inst& I = insts[blockIdx.y];
+  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
+  const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
   __shared__ double sm[16];
+  // the flags, r.z of this iteration, the cells' operands and the partials in
+  // one memory round trip; a stopped or finished solve returns after the sum
+  // (the whole block together)
+  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
+  const int stopped = flagsG[fStopped];
+  const int cgDone = flagsG[fCgDone];
+  const double rz = gmem(static_cast<const double*>(I.red))[(it & 1) ? sCgRz1 : sCgRz0];
   double xc[kCgCells], pc[kCgCells], rc[kCgCells], apc[kCgCells], d[kCgCells];
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
@@ -1221,11 +1264,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
     d[q] = gmem(static_cast<const double*>(I.pP))[cc];
   }
   const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.cgBlocks, sm);
+  if (stopped || cgDone) return;
   if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
-    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    if (threadIdx.x == 0 && blockIdx.x == 0) gmem(I.flags)[fCgDone] = 1;
     return;
   }
-  const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
+  const double alpha = rz / pap;
   double part[2] = {0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
@@ -1240,14 +1284,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
     part[0] += fabs(r);
     part[1] += r * zq;
   }
-  (void)nI;
   blockSum<2>(part, sm);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
     partialSlot(I, sCgRz)[blockIdx.x] = part[1];
     if (blockIdx.x == 0) {
-      I.flags[fCgIters] = it + 1;
-      I.flags[fCgTotal] += 1;
+      gmem(I.flags)[fCgIters] = it + 1;
+      gmem(I.flags)[fCgTotal] += 1;
     }
   }
 }
