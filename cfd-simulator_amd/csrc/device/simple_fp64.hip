@@ -1121,7 +1121,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
 // launch is needed before the stencil), Ap and p.Ap. The cells' operands do
 // not depend on beta, so they are loaded before the partials are summed and
 // the two memory round trips overlap.
-__global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
+// At most 128 VGPRs (4 waves per SIMD; 150 unconstrained): measured faster in
+// batches, the same for one solve.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_cg_a(const Inst* insts, int it) {
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
@@ -1169,25 +1171,33 @@ __global__ void __launch_bounds__(kBlock) k_cg_a(const Inst* insts, int it) {
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
     const int cc = cq[q], i = iq[q], j = jq[q];
-    const int pr = (j == 0) ? prq[q] : -1;
-    hasS[q] = (j > 0) || (pr >= 0);
     hasN[q] = j < nJ - 1;
     hasW[q] = i > 0;
     hasE[q] = i < nI - 1;
-    const int cS = (j > 0) ? cc - nI : (pr >= 0 ? pr : cc);
+    // the cell below; row 0 takes its wake partner's values further down,
+    // so only the lanes of row 0 wait for the partner index
+    const int cS = (j > 0) ? cc - nI : cc;
     const int cN = hasN[q] ? cc + nI : cc;
     const int cW = hasW[q] ? cc - 1 : cc;
     const int cE = hasE[q] ? cc + 1 : cc;
     aP[q] = pP[cc];
-    const double aBelow = pN[(j > 0) ? cc - nI : cc];
-    const double aWake = pWk[cc];
-    aS[q] = (j > 0) ? aBelow : aWake;
+    aS[q] = pN[(j > 0) ? cc - nI : cc];  // row 0: the wake coefficient, below
     aN[q] = pN[cc];
     aW[q] = pE[cW];  // the left face's coefficient is stored with the cell to its left
     aE[q] = pE[cc];
     oc[q] = src[cc]; oS[q] = src[cS]; oN[q] = src[cN]; oW[q] = src[cW]; oE[q] = src[cE];
     zc[q] = zS[q] = zN[q] = zW[q] = zE[q] = 0.0;
     if (it > 0) { zc[q] = z[cc]; zS[q] = z[cS]; zN[q] = z[cN]; zW[q] = z[cW]; zE[q] = z[cE]; }
+    hasS[q] = j > 0;
+  }
+#pragma unroll
+  for (int q = 0; q < kCgCells; ++q) {
+    if (jq[q] == 0 && prq[q] >= 0) {
+      hasS[q] = true;
+      aS[q] = pWk[cq[q]];
+      oS[q] = src[prq[q]];
+      if (it > 0) zS[q] = z[prq[q]];
+    }
   }
   double sums[2];
   {
