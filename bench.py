@@ -370,7 +370,38 @@ def run_serve(args, mixed_batch=False):
                       "cell_iterations_per_s": sum(r.cells * r.iterations for r in results) / wall if wall > 0 else None,
                       "peak_resident": stats.peak_active, "peak_resident_gb": stats.peak_resident_bytes / 1e9,
                       "slots": args.slots, "chunk": args.serve_chunk, "tolerance": args.tolerance,
-                      "max_iterations": args.max_iterations, "by_level": by_level}), flush=True)
+                      "max_iterations": args.max_iterations, "by_level": by_level,
+                      "one_at_a_time": alone_comparison(G, W, rows, results, wall) if args.alone_rate else None}),
+          flush=True)
+
+
+def alone_comparison(G, W, rows, results, batched_seconds):
+    """The same solves one at a time on the plain path: one outer iteration of
+    a solve alone, timed per mesh level (the first instance of each level in
+    the manifest, 100 iterations after 20), times every result's outer
+    iterations (its last attempt only, so the estimate is a lower bound)."""
+    per_level = {}
+    for lv in W.LEVELS:
+        r = next((x for x in rows if x["level"] == lv), None)
+        if r is None:
+            continue
+        mesh = G.cgrid(r["m"], r["p"], r["t"], options=W.mesh_options(G, lv, r["reynolds"]))
+        kind, vel, pres = G.face_conditions(mesh, 1.0, r["alpha"])
+        s = G.SimpleSolver(mesh, kind, vel, pres, G.default_settings(intensity=0.001, viscosity_ratio=10.0))
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(mesh.cells, speed=1.0, alpha_deg=r["alpha"], density=1.0,
+                                      reynolds=r["reynolds"]))
+        s.iterate(20)
+        t0 = time.perf_counter()
+        s.iterate(100)
+        per_level[lv] = (time.perf_counter() - t0) / 100
+        print(f"alone: {lv} {mesh.cells} cells {per_level[lv] * 1e3:.3f} ms per outer iteration", flush=True)
+    level_of = {q: rows[q]["level"] for q in range(len(rows))}
+    seconds = sum(r.iterations * per_level[level_of[r.ticket]] for r in results)
+    converged = sum(1 for r in results if r.status == 0)
+    return {"ms_per_iteration": {k: v * 1e3 for k, v in per_level.items()}, "seconds_estimate": seconds,
+            "converged_solves_per_s": converged / seconds if seconds > 0 else None,
+            "batched_speedup": seconds / batched_seconds if batched_seconds > 0 else None}
 
 
 def run_calibrate_copy(args):
@@ -410,6 +441,8 @@ def main():
     ap.add_argument("--serve-chunk", type=int, default=20)
     ap.add_argument("--cold", action="store_true", help="serve without warm starts")
     ap.add_argument("--wall-limit", type=float, default=900.0, help="seconds before --serve stops waiting")
+    ap.add_argument("--alone-rate", action="store_true",
+                    help="with --serve / --mixed-batch: also price the same solves run one at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
