@@ -10,17 +10,18 @@ path reproduces the reference iteration for iteration.
 
 A step is one outer iteration (SimpleSolver::iterate, SimpleSolver.cpp:662-794)
 with the state resident in HBM. The headline value is converged solves/s:
-steps/s divided by the number of outer iterations this solve takes until Cl
-and Cd stay within 1e-3 and 1e-4 of their converged values. That count comes
-from `python bench.py --converge`, which runs the solve to its residual
-tolerance and prints the history it was read from.
+steps/s divided by the number of outer iterations this solve takes from the
+uniform field to the reference's stop rule, residuals.worst() < 1e-7, as
+measured by `python bench.py --solve --tolerance 1e-7`.
 
 Beside the timed loop the line carries
-  roofline      the dominant stage's kernel: algorithmic HBM bytes per launch
-                (traffic_model()) over its mean launch time from HIP events
-                recorded on the solver's stream inside the timed region;
-  cpu_baseline  the FP64 oracle (a restatement of the reference solver, one
-                thread) timed on a few outer iterations of the same solve.
+  roofline         the dominant unit of work (by time per step): algorithmic
+                   HBM bytes per launch (traffic_model()) over its mean launch
+                   time from HIP events recorded on the solver's stream inside
+                   the timed region; roofline_others the same for the rest;
+  mixed_precision  the same workload with FP32 storage in the pressure CG;
+  cpu_baseline     the FP64 oracle (a restatement of the reference solver, one
+                   thread) timed on a few outer iterations of the same solve.
 """
 from __future__ import annotations
 
@@ -91,9 +92,13 @@ def build_case(G):
     return mesh, kind, vel, pres, field
 
 
-def gpu_solver(G, mesh, kind, vel, pres, field):
+def precision_of(G, args):
+    return G.MIXED if getattr(args, "precision", "fp64") == "mixed" else G.FP64
+
+
+def gpu_solver(G, mesh, kind, vel, pres, field, precision=0):
     w = WORKLOAD
-    st = G.default_settings(intensity=w["intensity"], viscosity_ratio=w["viscosity_ratio"])
+    st = G.default_settings(intensity=w["intensity"], viscosity_ratio=w["viscosity_ratio"], precision=precision)
     s = G.SimpleSolver(mesh, kind, vel, pres, st)
     s.set_turbulence_model(G.MODEL_SST)
     s.initialise(*field)
@@ -143,7 +148,7 @@ def run_bench(args):
     if G.device_count() < 1:
         raise SystemExit("bench.py needs a GPU: no HIP device visible")
     mesh, kind, vel, pres, field = build_case(G)
-    s = gpu_solver(G, mesh, kind, vel, pres, field)
+    s = gpu_solver(G, mesh, kind, vel, pres, field, precision_of(G, args))
     if args.warmup:
         s.iterate(args.warmup)
     t0 = time.perf_counter()
@@ -152,11 +157,12 @@ def run_bench(args):
     ms_per_step = wall * 1000.0 / args.steps
 
     # The dominant kernel: the sweeps are one launch per stage call; the CG is
-    # two launches per CG iteration (the pair is one unit of the traffic model).
+    # two launches per CG iteration, and the pair is one unit of the traffic
+    # model and of the timing (events around every launch would add their own
+    # gaps to 400 launches per step). By the kernel trace k_cg_a alone is
+    # already the largest kernel, so the CG iteration is the dominant unit.
     tm = traffic_model()
     cg_its = max(mon.pressure_iterations, 1) * args.steps  # CG iterations in the timed region (cap reached each time)
-    # Dominance is per kernel: the pressure stage is two kernels, k_cg_a and
-    # k_cg_b, of about equal time, so each counts for half of it.
     def entry(stage, kernel, unit):
         units = cg_its if unit == "cg_iteration" else stage_calls[stage]
         unit_ms = stage_ms[stage] / units
@@ -167,10 +173,9 @@ def run_bench(args):
                 "stage": stage, "algorithmic_bytes_per_launch": algo, "launch_ms": unit_ms, "words_per_cell": tm[unit]}
 
     present = [c for c in DOMINANT_CANDIDATES if stage_calls.get(c[0])]
-    weight = {"cg_iteration": 0.5}
-    stage, kernel, unit = max(present, key=lambda c: stage_ms[c[0]] * weight.get(c[2], 1.0))
-    roofline = entry(stage, kernel, unit)
-    cg = entry("pressure_solve", "k_cg_a + k_cg_b", "cg_iteration")
+    entries = {c[1]: entry(*c) for c in present}
+    kernel = max(present, key=lambda c: stage_ms[c[0]])[1]
+    roofline = entries[kernel]
 
     n_conv = ITERATIONS_TO_CONVERGE
     value = (1000.0 / (ms_per_step * n_conv)) if n_conv else None
@@ -179,19 +184,44 @@ def run_bench(args):
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "NACA 2412 C-grid 1024x256 (262,144 cells, y1 1e-6 c), k-omega SST, Re 1e6, "
-                               "alpha 4 deg, Tu 0.1%, mut/mu 10; plain FP64, lexicographic GS; "
-                               "one step = one SIMPLE outer iteration",
+                               "alpha 4 deg, Tu 0.1%, mut/mu 10; "
+                               + ("plain FP64" if args.precision == "fp64" else "mixed (FP32 CG storage)")
+                               + ", lexicographic GS; one step = one SIMPLE outer iteration",
                    "cells": mesh.cells, "iterations_per_solve": n_conv,
                    "time_to_convergence_s": (ms_per_step * n_conv / 1000.0) if n_conv else None,
                    "outer_iterations_per_s": 1000.0 / ms_per_step,
                    "cg_iterations_last": mon.pressure_iterations,
                    "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items() if stage_calls[k]}},
         "roofline": roofline,
-        "roofline_pressure_cg": cg,
+        "roofline_others": {k: v for k, v in entries.items() if k != kernel},
     }
+    line["config"]["precision"] = args.precision
+    if args.precision == "fp64" and not args.no_mixed:
+        line["mixed_precision"] = mixed_comparison(G, mesh, kind, vel, pres, field, args, ms_per_step, stage_ms)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_iterations, n_conv)
     print(json.dumps(line), flush=True)
+
+
+def mixed_comparison(G, mesh, kind, vel, pres, field, args, fp64_ms, fp64_stage_ms):
+    """The same workload on the mixed path (FP32 storage in the pressure
+    solve), timed over a quarter of the steps after the same warm-up: outer
+    iteration and pressure-solve times beside the FP64 path's."""
+    del field
+    s = gpu_solver(G, mesh, kind, vel, pres, G.uniform_field(
+        mesh.cells, speed=WORKLOAD["speed"], alpha_deg=WORKLOAD["alpha"], density=WORKLOAD["density"],
+        reynolds=WORKLOAD["reynolds"]), G.MIXED)
+    steps = max(args.steps // 4, 1)
+    if args.warmup:
+        s.iterate(args.warmup)
+    t0 = time.perf_counter()
+    _, stage_ms, _ = s.iterate_timed(steps)
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    p32 = stage_ms["pressure_solve"] / steps
+    p64 = fp64_stage_ms["pressure_solve"] / args.steps
+    return {"ms_per_step": ms, "steps": steps, "speedup_per_step": fp64_ms / ms,
+            "pressure_solve_ms": p32, "pressure_solve_ms_fp64": p64, "pressure_solve_speedup": p64 / p32,
+            "storage": "FP32 matrix and r, z, p, Ap in the pressure CG; FP64 arithmetic, reductions, x, sweeps"}
 
 
 def run_converge(args):
@@ -239,14 +269,15 @@ def run_solve(args):
     import cfdgpu as G
 
     mesh, kind, vel, pres, field = build_case(G)
-    s = gpu_solver(G, mesh, kind, vel, pres, field)
+    s = gpu_solver(G, mesh, kind, vel, pres, field, precision_of(G, args))
     t0 = time.perf_counter()
     rep = s.solve(args.tolerance, args.max_iterations, check_every=args.check_every)
     sec = time.perf_counter() - t0
     f = coefficients(G, mesh, s)
     print(json.dumps({"iterations": rep.iterations, "status": rep.status, "seconds": sec,
                       "ms_per_iteration": 1000.0 * sec / max(rep.iterations, 1), "worst": rep.last.worst(),
-                      "tolerance": args.tolerance, "cl": f.cl, "cd": f.cd, "cm": f.cm}), flush=True)
+                      "tolerance": args.tolerance, "cl": f.cl, "cd": f.cd, "cm": f.cm,
+                      "precision": args.precision}), flush=True)
 
 
 def run_batch_polar(args):
@@ -266,7 +297,7 @@ def run_batch_polar(args):
     t_setup = time.perf_counter()
     for a in alphas:
         kind, vel, pres = G.face_conditions(mesh, 50.0, a)
-        st = G.default_settings(intensity=0.001, viscosity_ratio=10.0)
+        st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=precision_of(G, args))
         s = G.SimpleSolver(mesh, kind, vel, pres, st)
         s.set_turbulence_model(G.MODEL_SST)
         s.initialise(*G.uniform_field(mesh.cells, speed=50.0, alpha_deg=a, density=1.225, reynolds=1e6))
@@ -298,6 +329,7 @@ def run_batch_polar(args):
     converged = sum(1 for x in status if x == 0)
     lat = sorted(t for t, x in zip(done_at, status) if x == 0)
     print(json.dumps({"workload": "batched uniform polar, NACA 0012 512x128, SST Re 1e6, alpha -4..16 (21 cold)",
+                      "precision": args.precision,
                       "cells_per_solve": mesh.cells, "solves": len(alphas), "converged": converged,
                       "seconds": sec, "setup_seconds": t_setup, "converged_solves_per_s": converged / sec,
                       "time_to_convergence_p50_s": lat[len(lat) // 2] if lat else None,
@@ -326,7 +358,7 @@ def run_serve(args, mixed_batch=False):
     import workloads as W
 
     rows = W.load(W.MIXED_BATCH_FILE if mixed_batch else W.SERVING_FILE)[: args.requests]
-    st = G.default_settings(intensity=0.001, viscosity_ratio=10.0)
+    st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=precision_of(G, args))
     srv = G.Server(G.server_options(hbm_budget_bytes=200e9, max_slots=args.slots, chunk_iterations=args.serve_chunk,
                                     tolerance=args.tolerance, max_iterations=args.max_iterations, settings=st,
                                     warm_start=0 if args.cold else 1))
@@ -349,6 +381,12 @@ def run_serve(args, mixed_batch=False):
     wall = stats.clock_s
     conv = [r for r in results if r.status == 0]
     ttc = sorted(r.finished_s - r.arrival_s for r in conv)
+    per_request = {r.ticket: {"level": rows[r.ticket]["level"], "status": r.status, "iterations": r.iterations,
+                              "cl": r.cl, "cd": r.cd, "cm": r.cm} for r in results}
+    if args.results_file:
+        with open(args.results_file, "w") as fh:
+            json.dump({"precision": args.precision, "tolerance": args.tolerance, "results": per_request}, fh)
+    versus = compare_results(per_request, args.compare_to) if args.compare_to else None
     by_level = {}
     level_of = {q: rows[q]["level"] for q in range(len(rows))}
     for lv in W.LEVELS:
@@ -359,7 +397,7 @@ def run_serve(args, mixed_batch=False):
     statuses = {G.STATUS_NAMES[k]: sum(1 for r in results if r.status == k) for k in G.STATUS_NAMES}
     name = ("mixed-size batch, 64 random NACA sections, coarse/medium/fine C-grids, SST" if mixed_batch else
             f"served stream, first {len(rows)} requests of the 20 req/s Poisson manifest, SST")
-    print(json.dumps({"workload": name, "requests": len(rows), "finished": len(results),
+    print(json.dumps({"workload": name, "precision": args.precision, "requests": len(rows), "finished": len(results),
                       "unfinished": len(rows) - len(results), "statuses": statuses,
                       "seconds": wall, "converged_solves_per_s": len(conv) / wall if wall > 0 else None,
                       "time_to_convergence_p50_s": _percentile(ttc, 0.5),
@@ -371,8 +409,24 @@ def run_serve(args, mixed_batch=False):
                       "peak_resident": stats.peak_active, "peak_resident_gb": stats.peak_resident_bytes / 1e9,
                       "slots": args.slots, "chunk": args.serve_chunk, "tolerance": args.tolerance,
                       "max_iterations": args.max_iterations, "by_level": by_level,
-                      "one_at_a_time": alone_comparison(G, W, rows, results, wall) if args.alone_rate else None}),
+                      "one_at_a_time": alone_comparison(G, W, rows, results, wall) if args.alone_rate else None,
+                      "versus": versus}),
           flush=True)
+
+
+def compare_results(mine, path):
+    """Per-request coefficients against another run of the same manifest (the
+    --results-file of a plain FP64 run): the largest |dCl|, |dCd|, |dCm| over
+    the requests both runs converged, and how many statuses differ."""
+    with open(path) as fh:
+        other = json.load(fh)
+    theirs = {int(k): v for k, v in other["results"].items()}
+    both = [t for t in mine if t in theirs and mine[t]["status"] == 0 and theirs[t]["status"] == 0]
+    d = {q: max((abs(mine[t][q] - theirs[t][q]) for t in both), default=None) for q in ("cl", "cd", "cm")}
+    return {"against": other["precision"], "converged_in_both": len(both),
+            "status_differs": sum(1 for t in mine if t in theirs and mine[t]["status"] != theirs[t]["status"]),
+            "max_abs_dcl": d["cl"], "max_abs_dcd": d["cd"], "max_abs_dcm": d["cm"],
+            "accepted": (d["cl"] is not None and d["cl"] < 1e-3 and d["cd"] < 1e-4)}
 
 
 def alone_comparison(G, W, rows, results, batched_seconds):
@@ -443,6 +497,11 @@ def main():
     ap.add_argument("--wall-limit", type=float, default=900.0, help="seconds before --serve stops waiting")
     ap.add_argument("--alone-rate", action="store_true",
                     help="with --serve / --mixed-batch: also price the same solves run one at a time")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="fp64: the plain path; mixed: FP32 storage in the pressure CG")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-path comparison of the default run")
+    ap.add_argument("--results-file", help="with --serve / --mixed-batch: write every request's result here")
+    ap.add_argument("--compare-to", help="with --serve / --mixed-batch: a --results-file to compare coefficients with")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "

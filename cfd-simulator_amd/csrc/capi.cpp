@@ -40,17 +40,15 @@ int guarded(F&& f) {
   }
 }
 
-// Device-path options the library runs. The mixed-precision path is switched
-// off: it made an illegal memory access on the GPU whose cause has not been
-// found, so it stays unreachable until one is.
+// Device-path options the library runs.
 int checkPathOptions(const cfdgpu_settings& s) {
-  if (s.precision == CFDGPU_PRECISION_MIXED)
-    return fail(CFDGPU_NOT_IMPLEMENTED, "mixed precision is disabled in this build; use CFDGPU_PRECISION_FP64");
-  if (s.precision != CFDGPU_PRECISION_FP64) return fail(CFDGPU_INVALID_ARGUMENT, "unknown precision");
+  if (s.precision != CFDGPU_PRECISION_FP64 && s.precision != CFDGPU_PRECISION_MIXED)
+    return fail(CFDGPU_INVALID_ARGUMENT, "unknown precision");
   if (s.gs_ordering != CFDGPU_GS_LEXICOGRAPHIC && s.gs_ordering != CFDGPU_GS_REDBLACK)
     return fail(CFDGPU_INVALID_ARGUMENT, "unknown Gauss-Seidel ordering");
   if (s.pressure_solver != CFDGPU_PSOLVE_JACOBI_CG)
-    return fail(CFDGPU_NOT_IMPLEMENTED, "only the Jacobi-PCG pressure solver runs in FP64");
+    return fail(CFDGPU_NOT_IMPLEMENTED, "only the Jacobi-PCG pressure solver is implemented");
+  if (s.refinement_passes != 1) return fail(CFDGPU_NOT_IMPLEMENTED, "refinement_passes is reserved and must be 1");
   return CFDGPU_OK;
 }
 

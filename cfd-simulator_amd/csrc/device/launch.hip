@@ -36,14 +36,9 @@ __global__ void k_sst_blend(const Inst*);
 __global__ void k_sst_asm(const Inst*);
 __global__ void k_sst_wall(const Inst*);
 __global__ void k_sst_mut(const Inst*);
-__global__ void k_fsweep(const Inst*, int, int, int);
-__global__ void k_cgf_rhs(const Inst*, int);
-__global__ void k_line_factor(const Inst*);
-__global__ void k_line_solve(const Inst*, int);
-__global__ void k_cgf_accumulate(const Inst*);
-template <int PRE> __global__ void k_cgf_init(const Inst*);
-template <int PRE> __global__ void k_cgf_a(const Inst*, int);
-template <int PRE> __global__ void k_cgf_b(const Inst*, int);
+__global__ void k_cg_init_f32(const Inst*);
+__global__ void k_cg_a_f32(const Inst*, int);
+__global__ void k_cg_b_f32(const Inst*, int);
 
 int gsThreadsFor(int nJ) { return gsLanes(nJ); }
 
@@ -62,36 +57,12 @@ size_t gsLdsFor(int nJ, int nI, int threads) {
   return barrier > decoupled ? barrier : decoupled;
 }
 
-namespace {
-
-// FP32 pressure correction with FP64 reductions, refined against the FP64
-// matrix (one pass = one FP32 CG solve of the FP64 residual).
-template <int PRE>
-void launchMixedPressure(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s) {
-  const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
-  const dim3 block(kBlock);
-  const dim3 lines(static_cast<unsigned>(cfg.maxLineBlocks), static_cast<unsigned>(cfg.batch));
-  if (PRE == 1) hipLaunchKernelGGL(k_line_factor, lines, dim3(64), 0, s, d);
-  for (int pass = 0; pass < h.refinePasses; ++pass) {
-    hipLaunchKernelGGL(k_cgf_rhs, grid, block, 0, s, d, pass);
-    hipLaunchKernelGGL(k_cgf_init<PRE>, grid, block, 0, s, d);
-    if (PRE == 1) hipLaunchKernelGGL(k_line_solve, lines, dim3(64), 0, s, d, -1);
-    for (int it = 0; it < h.cgMax; ++it) {
-      hipLaunchKernelGGL(k_cgf_a<PRE>, grid, block, 0, s, d, it);
-      hipLaunchKernelGGL(k_cgf_b<PRE>, grid, block, 0, s, d, it);
-      if (PRE == 1) hipLaunchKernelGGL(k_line_solve, lines, dim3(64), 0, s, d, it);
-    }
-    hipLaunchKernelGGL(k_cgf_accumulate, grid, block, 0, s, d);
-  }
-}
-
-}  // namespace
-
 void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s, StageRecorder* rec) {
   const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
   const dim3 block(kBlock);
   const dim3 one((cfg.batch + 63) / 64);
   const bool lex = h.gsOrder == 0;
+  // the mixed path differs only in the pressure solve's storage (FP32)
   const bool mixed = h.precision == 1;
   auto begin = [&](int stage) { if (rec) rec->begin(stage, s); };
   auto end = [&](int stage) { if (rec) rec->end(stage, s); };
@@ -103,14 +74,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   hipLaunchKernelGGL(k_mom, grid, block, 0, s, d);
   end(CFDGPU_STAGE_MOMENTUM_ASSEMBLY);
   begin(CFDGPU_STAGE_MOMENTUM_SWEEPS);
-  if (mixed) {
-    // two-colour sweeps on the FP32 defect; the last adds it to u, v
-    for (int sw = 0; sw < h.momSweeps; ++sw) {
-      hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 0, 0);
-      hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 0, 1, sw == h.momSweeps - 1 ? 1 : 0);
-    }
-    end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
-  } else if (lex) {
+  if (lex) {
     // decoupled: u and v in two workgroups; barrier wavefront: both in one
     const dim3 momGrid(kDecoupledSweep ? 2 : 1, cfg.batch);
     if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom : k_gs_lex_mom, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
@@ -136,11 +100,14 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     hipLaunchKernelGGL(k_pressure, grid, block, 0, s, d, pass);
     end(CFDGPU_STAGE_PRESSURE_ASSEMBLY);
     begin(CFDGPU_STAGE_PRESSURE_SOLVE);
+    const dim3 cgGrid(cfg.maxCgBlocks, cfg.batch);
     if (mixed) {
-      if (h.psolve == 1) launchMixedPressure<1>(d, cfg, h, s);
-      else launchMixedPressure<0>(d, cfg, h, s);
+      hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d);
+      for (int it = 0; it < h.cgMax; ++it) {
+        hipLaunchKernelGGL(k_cg_a_f32, cgGrid, block, 0, s, d, it);
+        hipLaunchKernelGGL(k_cg_b_f32, cgGrid, block, 0, s, d, it);
+      }
     } else {
-      const dim3 cgGrid(cfg.maxCgBlocks, cfg.batch);
       hipLaunchKernelGGL(k_cg_init, cgGrid, block, 0, s, d);
       for (int it = 0; it < h.cgMax; ++it) {
         hipLaunchKernelGGL(k_cg_a, cgGrid, block, 0, s, d, it);
@@ -165,16 +132,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     hipLaunchKernelGGL(k_sst_asm, grid, block, 0, s, d);
     end(CFDGPU_STAGE_SST_ASSEMBLY);
     begin(CFDGPU_STAGE_SST_SWEEPS);
-    if (mixed) {
-      for (int sw = 0; sw < 3; ++sw) {
-        const int last = (sw == 2) ? 1 : 0;
-        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 1, 0, 0);
-        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 1, 1, last);
-        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 0, 0);
-        hipLaunchKernelGGL(k_fsweep, grid, block, 0, s, d, 2, 1, last);
-      }
-      end(CFDGPU_STAGE_SST_SWEEPS);
-    } else if (lex) {
+    if (lex) {
       if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_turb : k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
       else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_turb_wide : k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
       end(CFDGPU_STAGE_SST_SWEEPS);

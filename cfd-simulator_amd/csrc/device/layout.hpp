@@ -120,8 +120,7 @@ struct Inst {
   double cgTol;
   int nonOrthPasses;  // max(1, nonOrthogonalCorrectors)
   int gsOrder;        // 0 lexicographic (exact), 1 two-colour
-  int precision;      // 0 FP64, 1 mixed
-  int psolve;         // 0 Jacobi-PCG, 1 line-preconditioned CG
+  int precision;      // 0 FP64, 1 mixed (FP32 storage in the pressure solve)
   double refMassFlow;
   double stopTol;  // device-side stop rule (worst() < stopTol); <= 0 disables
   int stopMaxIter;
@@ -181,13 +180,10 @@ struct Inst {
   double sigK1, sigW1, beta1, sigK2, sigW2, beta2, betaStar, kappa, a1, prodLimit;
   double gamma1, gamma2, betaStarQ;  // gamma1(), gamma2(), betaStar^(1/4) (host-computed)
 
-  // mixed-precision mirrors (FP32 bulk)
-  float *faP, *faS, *faN, *faW, *faE, *fbx, *fby, *fdx, *fdy;
-  float *fpP, *fpN, *fpE, *fpWk, *fx, *fr, *fz, *fpd0, *fpd1, *fAp, *fb;
-  float *fkP, *fkS, *fkN, *fkW, *fkE, *fkR, *fkD, *fwP, *fwS, *fwN, *fwW, *fwE, *fwR, *fwD;
-  float *fLinv, *fLc;  // line preconditioner: 1/pivot and eliminated upper coefficient
-  int lineBlocks;      // 64-column blocks of the line preconditioner
-  int refinePasses;    // FP32 CG passes refined against the FP64 matrix
+  // mixed precision: the pressure system and the CG vectors stored in FP32
+  // (the CG's arithmetic, its reductions and x stay FP64)
+  float *fpP, *fpN, *fpE, *fpWk;
+  float *fr, *fz, *fpd0, *fpd1, *fAp;
 
   // reductions
   double* red;        // kSlots scalars

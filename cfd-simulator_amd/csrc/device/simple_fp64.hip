@@ -280,18 +280,6 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     part[0] = fabs(row.bx - axu);
     part[1] = fabs(row.by - axv);
     part[2] = fabs(d);
-    if (I.precision == 1) {
-      // FP32 matrix and FP64 defect for the defect-correction sweeps
-      I.faP[c] = static_cast<float>(d);
-      I.faS[c] = static_cast<float>(aS);
-      I.faN[c] = static_cast<float>(aN);
-      I.faW[c] = static_cast<float>(aW);
-      I.faE[c] = static_cast<float>(aE);
-      I.fbx[c] = static_cast<float>(row.bx - axu);
-      I.fby[c] = static_cast<float>(row.by - axv);
-      I.fdx[c] = 0.0f;
-      I.fdy[c] = 0.0f;
-    }
   }
   blockSum<3>(part, sm);
   if (threadIdx.x == 0) {
@@ -1038,20 +1026,38 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
 // previous launch's per-block partials, so the host never waits inside the
 // solve; once the test passes (or the iteration breaks down) every later
 // launch is a no-op.
+//
+// The bodies are written once for the storage type T of the matrix and of
+// r, z, p and Ap: double (the plain path, the reference's arithmetic) or
+// float (the mixed path, half the bytes per iteration). Either way every
+// value is widened on load, the arithmetic and the reductions are FP64, x is
+// FP64, and a value is rounded to T once, where it is stored; the rounded
+// value is the one used afterwards (p in the stencil, r in |r|_1 and r.z), so
+// the FP32 solve is a consistent CG on the FP32-rounded system. With T =
+// double every conversion is the identity.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c, int i, int j) {
-  const int nI = I.nI, nJ = I.nJ;
-  double y = I.pP[c] * x[c];
-  if (j > 0) y += I.pN[c - nI] * x[c - nI];
-  else {
-    const int pr = I.partner[i];
-    if (pr >= 0) y += I.pWk[c] * x[pr];
-  }
-  if (j < nJ - 1) y += I.pN[c] * x[c + nI];
-  if (i > 0) y += I.pE[c - 1] * x[c - 1];
-  if (i < nI - 1) y += I.pE[c] * x[c + 1];
-  return y;
+template <typename T>
+struct CgStore {
+  const T *pP, *pN, *pE, *pWk;
+  T *r, *z, *pd0, *pd1, *Ap;
+};
+
+template <typename T>
+__device__ __forceinline__ CgStore<T> cgStore(const Inst& I);
+template <>
+__device__ __forceinline__ CgStore<double> cgStore<double>(const Inst& I) {
+  return CgStore<double>{I.pP, I.pN, I.pE, I.pWk, I.r, I.z, I.pd0, I.pd1, I.Ap};
+}
+template <>
+__device__ __forceinline__ CgStore<float> cgStore<float>(const Inst& I) {
+  return CgStore<float>{I.fpP, I.fpN, I.fpE, I.fpWk, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp};
+}
+
+// a value as it is after a store and a load of type T
+template <typename T>
+__device__ __forceinline__ double asStored(double v) {
+  return static_cast<double>(static_cast<T>(v));
 }
 
 // The CG launches give each thread kCgCells cells (c = tile * kBlock *
@@ -1059,20 +1065,15 @@ __device__ __forceinline__ double spmvPres(const Inst& I, const double* x, int c
 // 1 / kCgCells of the blocks of the cell-parallel kernels and every block of
 // the next launch sums that many fewer partials. The per-thread sums run over
 // q in order before the block sum: the result is fixed run to run.
-#define CFD_CG_PROLOGUE                                                 \
-  const Inst& I = insts[blockIdx.y];                                    \
-  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;               \
-  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags)); \
-  if (flagsG[fStopped]) return;                                         \
-  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);   \
-  const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x); \
-  const int nI = I.nI;                                                  \
-  const int nJ = I.nJ;                                                  \
-  (void)nJ;
-
-__global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
-  CFD_CG_PROLOGUE
-  __shared__ double sm[16];
+template <typename T>
+__device__ __forceinline__ void cgInit(const Inst* insts, double* sm) {
+  const Inst& I = insts[blockIdx.y];
+  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
+  if (flagsG[fStopped]) return;
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
+  const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
+  const CgStore<T> S = cgStore<T>(I);
   double mean = 0.0;
   if (!I.hasPRef) mean = blockSumPartials(partialSlot(I, sSourceSum), I.nBlocks, sm) / static_cast<double>(I.N);
   double part[2] = {0.0, 0.0};
@@ -1081,7 +1082,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
   for (int q = 0; q < kCgCells; ++q) {
     const int cc = min(base + q * kBlock, I.N - 1);
     bq[q] = gmem(static_cast<const double*>(I.pb))[cc];
-    dq[q] = gmem(static_cast<const double*>(I.pP))[cc];
+    dq[q] = static_cast<double>(gmem(S.pP)[cc]);
   }
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
@@ -1092,18 +1093,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
         b -= mean;
         gmem(I.pb)[c] = b;
       }
-      const double r = b - 0.0;  // x0 = 0, so A x0 = 0
+      const double r = asStored<T>(b - 0.0);  // x0 = 0, so A x0 = 0
       const double d = dq[q];
-      const double z = (fabs(d) > 0.0) ? r / d : r;
-      gmem(I.r)[c] = r;
-      gmem(I.z)[c] = z;
-      gmem(I.pd0)[c] = z;
+      const double z = asStored<T>((fabs(d) > 0.0) ? r / d : r);
+      gmem(S.r)[c] = static_cast<T>(r);
+      gmem(S.z)[c] = static_cast<T>(z);
+      gmem(S.pd0)[c] = static_cast<T>(z);
       gmem(I.x)[c] = 0.0;
       part[0] += fabs(r);
       part[1] += r * z;
     }
   }
-  (void)nI;
   blockSum<2>(part, sm);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = part[0];
@@ -1121,16 +1121,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
 // launch is needed before the stencil), Ap and p.Ap. The cells' operands do
 // not depend on beta, so they are loaded before the partials are summed and
 // the two memory round trips overlap.
-// At most 128 VGPRs (4 waves per SIMD; 150 unconstrained): measured faster in
-// batches, the same for one solve.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_cg_a(const Inst* insts, int it) {
+template <typename T>
+__device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
   const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
   const int nI = I.nI;
   const int nJ = I.nJ;
-  __shared__ double sm[16];
+  const CgStore<T> S = cgStore<T>(I);
   // Every operand is read through the global address space, without a branch
   // around its load: the flags, the CG scalars and the cells' wake partners
   // are one memory round trip, the cells' operands and the partials a second
@@ -1156,14 +1155,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
   if (stopped || cgDone) return;
   // operands: z and p_old at each cell and its four neighbours, the matrix row
-  const CFD_GLOBAL double* z = gmem(static_cast<const double*>(I.z));
-  const CFD_GLOBAL double* pOld = gmem(static_cast<const double*>((it & 1) ? I.pd0 : I.pd1));
-  const CFD_GLOBAL double* pFirst = gmem(static_cast<const double*>(I.pd0));
-  const CFD_GLOBAL double* src = (it == 0) ? pFirst : pOld;  // iteration 0 uses p0 = z (in pd0)
-  const CFD_GLOBAL double* pP = gmem(static_cast<const double*>(I.pP));
-  const CFD_GLOBAL double* pN = gmem(static_cast<const double*>(I.pN));
-  const CFD_GLOBAL double* pE = gmem(static_cast<const double*>(I.pE));
-  const CFD_GLOBAL double* pWk = gmem(static_cast<const double*>(I.pWk));
+  const CFD_GLOBAL T* z = gmem(static_cast<const T*>(S.z));
+  const CFD_GLOBAL T* pOld = gmem(static_cast<const T*>((it & 1) ? S.pd0 : S.pd1));
+  const CFD_GLOBAL T* pFirst = gmem(static_cast<const T*>(S.pd0));
+  const CFD_GLOBAL T* src = (it == 0) ? pFirst : pOld;  // iteration 0 uses p0 = z (in pd0)
+  const CFD_GLOBAL T* pP = gmem(S.pP);
+  const CFD_GLOBAL T* pN = gmem(S.pN);
+  const CFD_GLOBAL T* pE = gmem(S.pE);
+  const CFD_GLOBAL T* pWk = gmem(S.pWk);
   double zc[kCgCells], zS[kCgCells], zN[kCgCells], zW[kCgCells], zE[kCgCells];
   double oc[kCgCells], oS[kCgCells], oN[kCgCells], oW[kCgCells], oE[kCgCells];
   double aP[kCgCells], aS[kCgCells], aN[kCgCells], aW[kCgCells], aE[kCgCells];
@@ -1225,8 +1224,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     return;
   }
   double part[1] = {0.0};
-  // p = z + beta p_old (LinearSystem.cpp:199-201); iteration 0 has p = z
-  auto pv = [&](double zq, double oq) { return (it == 0) ? oq : zq + beta * oq; };
+  // p = z + beta p_old (LinearSystem.cpp:199-201), as stored; iteration 0 has p = z
+  auto pv = [&](double zq, double oq) { return (it == 0) ? oq : asStored<T>(zq + beta * oq); };
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
     if (!live[q]) continue;
@@ -1239,8 +1238,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     if (hasN[q]) y += aN[q] * pv(zN[q], oN[q]);
     if (hasW[q]) y += aW[q] * pv(zW[q], oW[q]);
     if (hasE[q]) y += aE[q] * pv(zE[q], oE[q]);
-    if (it > 0) gmem((it & 1) ? I.pd1 : I.pd0)[c] = pc;
-    gmem(I.Ap)[c] = y;
+    y = asStored<T>(y);
+    if (it > 0) gmem((it & 1) ? S.pd1 : S.pd0)[c] = static_cast<T>(pc);
+    gmem(S.Ap)[c] = static_cast<T>(y);
     part[0] += pc * y;
   }
   blockSum<1>(part, sm);
@@ -1249,12 +1249,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
 
 // Iteration `it`, launch B: alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap,
 // z = r / a_P, and the partials of |r|_1 and r.z (LinearSystem.cpp:160-198).
-__global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
+template <typename T>
+__device__ __forceinline__ void cgB(const Inst* insts, int it, double* sm) {
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
   const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
-  __shared__ double sm[16];
+  const CgStore<T> S = cgStore<T>(I);
   // the flags, r.z of this iteration, the cells' operands and the partials in
   // one memory round trip; a stopped or finished solve returns after the sum
   // (the whole block together)
@@ -1268,10 +1269,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
     const int c = base + q * kBlock;
     const int cc = (c < I.N) ? c : 0;
     xc[q] = gmem(static_cast<const double*>(I.x))[cc];
-    pc[q] = gmem(static_cast<const double*>((it & 1) ? I.pd1 : I.pd0))[cc];
-    rc[q] = gmem(static_cast<const double*>(I.r))[cc];
-    apc[q] = gmem(static_cast<const double*>(I.Ap))[cc];
-    d[q] = gmem(static_cast<const double*>(I.pP))[cc];
+    pc[q] = gmem(static_cast<const T*>((it & 1) ? S.pd1 : S.pd0))[cc];
+    rc[q] = gmem(static_cast<const T*>(S.r))[cc];
+    apc[q] = gmem(static_cast<const T*>(S.Ap))[cc];
+    d[q] = gmem(S.pP)[cc];
   }
   const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.cgBlocks, sm);
   if (stopped || cgDone) return;
@@ -1286,11 +1287,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
     const int c = base + q * kBlock;
     if (c >= I.N) continue;
     const double x = xc[q] + alpha * pc[q];
-    const double r = rc[q] - alpha * apc[q];
+    const double r = asStored<T>(rc[q] - alpha * apc[q]);
     gmem(I.x)[c] = x;
-    gmem(I.r)[c] = r;
-    const double zq = (fabs(d[q]) > 0.0) ? r / d[q] : r;
-    gmem(I.z)[c] = zq;
+    gmem(S.r)[c] = static_cast<T>(r);
+    const double zq = asStored<T>((fabs(d[q]) > 0.0) ? r / d[q] : r);
+    gmem(S.z)[c] = static_cast<T>(zq);
     part[0] += fabs(r);
     part[1] += r * zq;
   }
@@ -1303,6 +1304,36 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
       gmem(I.flags)[fCgTotal] += 1;
     }
   }
+}
+
+// FP64 storage: the plain path
+__global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
+  __shared__ double sm[16];
+  cgInit<double>(insts, sm);
+}
+// At most 128 VGPRs (4 waves per SIMD; 150 unconstrained): measured faster in
+// batches, the same for one solve.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_cg_a(const Inst* insts, int it) {
+  __shared__ double sm[16];
+  cgA<double>(insts, it, sm);
+}
+__global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
+  __shared__ double sm[16];
+  cgB<double>(insts, it, sm);
+}
+
+// FP32 storage: the mixed path
+__global__ void __launch_bounds__(kBlock) k_cg_init_f32(const Inst* insts) {
+  __shared__ double sm[16];
+  cgInit<float>(insts, sm);
+}
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_cg_a_f32(const Inst* insts, int it) {
+  __shared__ double sm[16];
+  cgA<float>(insts, it, sm);
+}
+__global__ void __launch_bounds__(kBlock) k_cg_b_f32(const Inst* insts, int it) {
+  __shared__ double sm[16];
+  cgB<float>(insts, it, sm);
 }
 
 // Partial sums of the correction for its mean when no boundary anchors the

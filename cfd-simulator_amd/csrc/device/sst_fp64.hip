@@ -260,11 +260,6 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
       const double defect = sstDefect(I, r, I.k, c, i, j);
       part[0] = fabs(defect);
       part[1] = fabs(r.d);
-      if (I.precision == 1) {
-        I.fkP[c] = static_cast<float>(r.d); I.fkS[c] = static_cast<float>(r.aS); I.fkN[c] = static_cast<float>(r.aN);
-        I.fkW[c] = static_cast<float>(r.aW); I.fkE[c] = static_cast<float>(r.aE);
-        I.fkR[c] = static_cast<float>(defect); I.fkD[c] = 0.0f;
-      }
     }
     // omega equation (wall value is omega_inflow: KOmegaSST.cpp:507-509)
     {
@@ -287,11 +282,6 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
       const double defect = sstDefect(I, r, I.w, c, i, j);
       part[2] = fabs(defect);
       part[3] = fabs(r.d);
-      if (I.precision == 1) {
-        I.fwP[c] = static_cast<float>(r.d); I.fwS[c] = static_cast<float>(r.aS); I.fwN[c] = static_cast<float>(r.aN);
-        I.fwW[c] = static_cast<float>(r.aW); I.fwE[c] = static_cast<float>(r.aE);
-        I.fwR[c] = static_cast<float>(defect); I.fwD[c] = 0.0f;
-      }
     }
   }
   blockSum<4>(part, sm);

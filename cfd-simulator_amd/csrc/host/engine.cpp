@@ -247,10 +247,7 @@ void carveInst(Carver& cv, Inst& I, const Grid& grid) {
     for (double2** a : vecArrays) *a = cv.take<double2>(N);
     I.Fj = cv.take<double>(nJf);
     I.Fi = cv.take<double>(nIf);
-    float** fArrays[] = {&I.faP, &I.faS, &I.faN, &I.faW, &I.faE, &I.fbx, &I.fby, &I.fdx, &I.fdy, &I.fpP, &I.fpN,
-                         &I.fpE, &I.fpWk, &I.fx, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.fb,
-                         &I.fkP, &I.fkS, &I.fkN, &I.fkW, &I.fkE, &I.fkR, &I.fkD, &I.fwP, &I.fwS, &I.fwN,
-                         &I.fwW, &I.fwE, &I.fwR, &I.fwD, &I.fLinv, &I.fLc};
+    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp};
     for (float** a : fArrays) *a = cv.take<float>(N);
     const size_t skew = static_cast<size_t>(skewLength(grid.nI, grid.nJ, gsThreadsFor(grid.nJ)));
     for (double2*& a : I.skMomRow) a = cv.take<double2>(skew);
@@ -294,13 +291,11 @@ void Engine::allocate() {
   h_.N = grid_.cells();
   h_.nBlocks = (h_.N + kBlock - 1) / kBlock;
   h_.cgBlocks = (h_.N + kBlock * kCgCells - 1) / (kBlock * kCgCells);
-  h_.lineBlocks = (h_.nI + 63) / 64;
   h_.gsR = gsThreadsFor(grid_.nJ);
   h_.gsSkewLen = skewLength(grid_.nI, grid_.nJ, h_.gsR);
   check(hipMalloc(&dInst_, sizeof(Inst)), "hipMalloc (descriptor)");
   cfg_.maxBlocks = h_.nBlocks;
   cfg_.maxCgBlocks = h_.cgBlocks;
-  cfg_.maxLineBlocks = h_.lineBlocks;
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
   cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
@@ -339,8 +334,6 @@ void Engine::fillInstScalars() {
   h_.nonOrthPasses = std::max(1, settings_.non_orthogonal_correctors);
   h_.gsOrder = settings_.gs_ordering;
   h_.precision = settings_.precision;
-  h_.psolve = settings_.pressure_solver;
-  h_.refinePasses = std::max(1, settings_.refinement_passes);
   h_.refMassFlow = refMassFlow_;
   h_.kIn = kIn_;
   h_.wIn = wIn_;
@@ -948,7 +941,7 @@ Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
   lds = gsLdsFor(0, maxNI, lanes);
   if (a.gsOrder == CFDGPU_GS_LEXICOGRAPHIC && lds > 64 * 1024)
     throw std::invalid_argument("batch too wide for the lexicographic sweep's LDS; use CFDGPU_GS_REDBLACK");
-  cfg_ = LaunchCfg{blocks, cgBlocks, 1, static_cast<int>(members_.size()), lanes, lds};
+  cfg_ = LaunchCfg{blocks, cgBlocks, static_cast<int>(members_.size()), lanes, lds};
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   check(hipMalloc(&dInsts_, sizeof(Inst) * members_.size()), "hipMalloc (batch descriptors)");
   host_.resize(members_.size());

@@ -24,7 +24,6 @@ namespace cfdgpu {
 struct LaunchCfg {
   int maxBlocks;   // grid.x for cell-parallel kernels
   int maxCgBlocks; // grid.x for the CG kernels
-  int maxLineBlocks;  // grid.x for the column (line-preconditioner) kernels
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep

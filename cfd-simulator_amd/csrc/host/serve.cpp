@@ -286,16 +286,15 @@ void Server::launchChunk() {
   if (active_.empty()) return;
   if (rosterDirty_) {
     host_.resize(active_.size());
-    int lanes = 64, blocks = 1, cgBlocks = 1, lineBlocks = 1, maxNI = 0;
+    int lanes = 64, blocks = 1, cgBlocks = 1, maxNI = 0;
     for (size_t q = 0; q < active_.size(); ++q) {
       host_[q] = active_[q]->engine->refreshInst();
       lanes = std::max(lanes, host_[q].gsR);
       blocks = std::max(blocks, host_[q].nBlocks);
       cgBlocks = std::max(cgBlocks, host_[q].cgBlocks);
-      lineBlocks = std::max(lineBlocks, host_[q].lineBlocks);
       maxNI = std::max(maxNI, host_[q].nI);
     }
-    cfg_ = LaunchCfg{blocks, cgBlocks, lineBlocks, static_cast<int>(active_.size()), lanes, gsLdsFor(0, maxNI, lanes)};
+    cfg_ = LaunchCfg{blocks, cgBlocks, static_cast<int>(active_.size()), lanes, gsLdsFor(0, maxNI, lanes)};
     if (opt_.settings.gs_ordering == CFDGPU_GS_LEXICOGRAPHIC && cfg_.gsLds > 64 * 1024)
       throw std::invalid_argument("resident meshes too wide for the lexicographic sweep's LDS");
     hipCheck(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),

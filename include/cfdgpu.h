@@ -43,8 +43,10 @@ enum { CFDGPU_MODEL_NONE = 0, CFDGPU_MODEL_LAMINAR = 1, CFDGPU_MODEL_SST = 2 };
 /* arithmetic of the hot path */
 enum {
   CFDGPU_PRECISION_FP64 = 0, /* plain FP64 path: the drop-in and the judge */
-  CFDGPU_PRECISION_MIXED = 1 /* FP32 sweeps/CG bulk, FP64 state and reductions; currently
-                                rejected with CFDGPU_NOT_IMPLEMENTED (disabled) */
+  CFDGPU_PRECISION_MIXED = 1 /* the pressure-correction CG stores its matrix and r, z, p, Ap
+                                in FP32 (half the bytes per iteration); its arithmetic, its
+                                inner products and norms, x, the sweeps, the residuals and
+                                the stop test stay FP64 */
 };
 /* Gauss-Seidel ordering of the momentum and turbulence sweeps */
 enum {
@@ -54,8 +56,8 @@ enum {
 /* pressure-correction solver */
 enum {
   CFDGPU_PSOLVE_JACOBI_CG = 0, /* reference: Jacobi-PCG, L1 stop (LinearSystem.cpp:118-210) */
-  CFDGPU_PSOLVE_LINE_CG = 1    /* CG preconditioned by wall-normal line (tridiagonal) solves;
-                                  mixed path only, so currently rejected (NOT_IMPLEMENTED) */
+  CFDGPU_PSOLVE_LINE_CG = 1    /* reserved (line-preconditioned CG): rejected with
+                                  CFDGPU_NOT_IMPLEMENTED */
 };
 
 /* stages of one outer iteration, for cfdgpu_iterate_timed */
@@ -113,7 +115,7 @@ typedef struct {
   int precision;      /* CFDGPU_PRECISION_* */
   int gs_ordering;    /* CFDGPU_GS_* */
   int pressure_solver;/* CFDGPU_PSOLVE_* */
-  int refinement_passes; /* FP64-residual refinement passes of the pressure solve (mixed) */
+  int refinement_passes; /* reserved: must be 1 (one pressure solve per corrector) */
 } cfdgpu_settings;
 
 /* solver::SSTConstants (include/cfd/solver/KOmegaSST.hpp:147-171) */

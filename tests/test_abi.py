@@ -36,10 +36,14 @@ def test_host_side_validation_needs_no_gpu(G):
     opts = G.server_options()
     assert opts.max_slots == 64 and opts.chunk_iterations == 20 and opts.warm_start == 1
     assert opts.hbm_budget_bytes == 200e9 and opts.model == G.MODEL_SST
-    mixed = G.server_options(settings=G.default_settings(precision=G.MIXED))
     import ctypes as C
     h = C.c_void_p()
-    assert lib.cfdgpu_server_create(C.byref(mixed), C.byref(h)) == 5  # NotImplemented
+    line = G.server_options(settings=G.default_settings(pressure_solver=G.PSOLVE_LINE))
+    assert lib.cfdgpu_server_create(C.byref(line), C.byref(h)) == 5  # NotImplemented
+    passes = G.server_options(settings=G.default_settings(refinement_passes=2))
+    assert lib.cfdgpu_server_create(C.byref(passes), C.byref(h)) == 5
+    bad = G.server_options(settings=G.default_settings(precision=7))
+    assert lib.cfdgpu_server_create(C.byref(bad), C.byref(h)) == 1  # InvalidArgument
 
 
 def test_server_create_fails_loudly_without_a_device(G):

@@ -1,0 +1,76 @@
+"""The mixed-precision path (CFDGPU_PRECISION_MIXED): the pressure-correction
+CG stores its matrix and r, z, p, Ap in FP32; its arithmetic and reductions,
+x, the sweeps, the residuals and the stop test stay FP64. Checked against the
+FP64 oracle on converged SST solves at the acceptance bar of the mixed path
+(|dCl| < 1e-3, |dCd| < 1e-4, in practice far inside it), on the first
+iterations (the FP32 rounding of the pressure system is the only difference,
+about 1e-7 relative in the fields), and in a batch against the same solve
+alone (bit-identical: members share no arithmetic)."""
+import numpy as np
+import pytest
+
+from helpers import SMALL, cgrid_pair, rel_err, solver_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(G, O, section, alpha, reynolds, precision):
+    gm, om = cgrid_pair(G, O, *section, **SMALL)
+    kind, vel, pres = G.face_conditions(gm, 1.0, alpha)
+    field = G.uniform_field(gm.cells, speed=1.0, alpha_deg=alpha, density=1.0, reynolds=reynolds)
+    st = G.default_settings(precision=precision)
+    gs, os_ = solver_pair(G, O, gm, om, kind, vel, pres, field, gsettings=st, model=G.MODEL_SST)
+    return gm, om, gs, os_
+
+
+def test_mixed_iterations_stay_within_fp32_rounding_of_the_oracle(G, O, gpu_available):
+    print("case: NACA0012 118x27 SST Re 1e5 alpha 4, mixed, five iterations", flush=True)
+    _, _, gs, os_ = _pair(G, O, (0, 0, 12), 4.0, 1e5, G.MIXED)
+    for it in range(5):
+        mg = gs.iterate()
+        mo = os_.iterate()
+        assert abs(mg.continuity - mo.continuity) <= 1e-4 * abs(mo.continuity), it
+        assert abs(mg.momentum_x - mo.momentum_x) <= 1e-4 * abs(mo.momentum_x), it
+    for a, b in zip(gs.field()[:3], os_.field()[:3]):
+        assert rel_err(a, b) < 1e-5
+
+
+@pytest.mark.parametrize("section,alpha", [((0, 0, 12), 4.0), ((2, 4, 12), 2.0)])
+def test_mixed_converged_forces_match_the_oracle(G, O, gpu_available, section, alpha):
+    print(f"case: NACA {section} 118x27 SST Re 1e5 alpha {alpha}, mixed, solve to 1e-6", flush=True)
+    _, om, gs, os_ = _pair(G, O, section, alpha, 1e5, G.MIXED)
+    rep = gs.solve(1e-6, 3000, check_every=1)
+    taken = 0
+    for taken in range(1, 3001):
+        if os_.iterate(1).worst() < 1e-6:
+            break
+    print(f"iterations: mixed {rep.iterations} oracle {taken}", flush=True)
+    assert rep.status == 0 and taken < 3000
+    fg, rg = gs.surface(speed=1.0, alpha_deg=alpha, density=1.0, reference_pressure=0.0)
+    fo, ro = O.surface(om, *os_.field(), speed=1.0, alpha_deg=alpha, density=1.0, reference_pressure=0.0)
+    print(f"cl {fg.cl:.10f} ({fo.cl:.10f}) cd {fg.cd:.10f} ({fo.cd:.10f}) cm {fg.cm:.10f} ({fo.cm:.10f})",
+          flush=True)
+    assert abs(fg.cl - fo.cl) < 1e-3 and abs(fg.cd - fo.cd) < 1e-4 and abs(fg.cm - fo.cm) < 1e-3
+    # measured far inside the bar: the converged field is the FP64 fixed point
+    assert abs(fg.cl - fo.cl) < 1e-5 * max(1.0, abs(fo.cl))
+    assert np.max(np.abs(rg[:, 7] - ro[:, 7])) < 1e-4 * np.max(np.abs(ro[:, 7]))
+
+
+def test_mixed_batch_member_matches_the_solve_alone(G, gpu_available):
+    print("case: mixed batch of two NACA sections against the first alone, 40 iterations", flush=True)
+
+    def make(m, p, t, alpha):
+        grid = G.cgrid(m, p, t, options=G.cgrid_options("coarse", **SMALL))
+        kind, vel, pres = G.face_conditions(grid, 1.0, alpha)
+        s = G.SimpleSolver(grid, kind, vel, pres, G.default_settings(precision=G.MIXED))
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(grid.cells, speed=1.0, alpha_deg=alpha, density=1.0, reynolds=1e5))
+        return grid, s
+
+    alone = make(0, 0, 12, 4.0)
+    pair = [make(0, 0, 12, 4.0), make(4, 4, 15, -2.0)]
+    alone[1].iterate(40)
+    G.Batch([s for _, s in pair]).iterate(40)
+    for a, b in zip(alone[1].field(), pair[0][1].field()):
+        assert np.array_equal(a, b)
+    assert np.all(np.isfinite(pair[1][1].field()[0]))
