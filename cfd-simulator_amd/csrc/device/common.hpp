@@ -159,19 +159,19 @@ struct OpMin {
   __device__ double operator()(double a, double b) const { return mn(a, b); }
 };
 
-template <int NS, class Op>
+template <int NS, class Op, int UNROLL = kPartialUnroll>
 __device__ __forceinline__ void wavePartials(const double* const (&p)[NS], int n, double identity, Op op,
                                              double (&out)[NS]) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
 #pragma unroll
   for (int q = 0; q < NS; ++q) out[q] = identity;
-  for (int b0 = 0; b0 < n; b0 += 64 * kPartialUnroll) {
-    double t[NS][kPartialUnroll];
+  for (int b0 = 0; b0 < n; b0 += 64 * UNROLL) {
+    double t[NS][UNROLL];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const CFD_GLOBAL double* g = gmem(p[q]);
 #pragma unroll
-      for (int u = 0; u < kPartialUnroll; ++u) {
+      for (int u = 0; u < UNROLL; ++u) {
         // every lane loads (clamped to the last partial, n >= 1) and the
         // value is then chosen: no branch around the load, so all loads of
         // the wave are in flight together
@@ -183,7 +183,7 @@ __device__ __forceinline__ void wavePartials(const double* const (&p)[NS], int n
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
-      for (int u = 0; u < kPartialUnroll; ++u) out[q] = op(out[q], t[q][u]);
+      for (int u = 0; u < UNROLL; ++u) out[q] = op(out[q], t[q][u]);
     }
   }
 #pragma unroll
@@ -214,14 +214,14 @@ __device__ __forceinline__ double waveMinPartials(const double* p, int n, double
 }
 
 // Sums of NS slots' partials, broadcast to every thread of the block. Wave 0
-// sums them (up to kPartialUnroll loads per lane in flight per slot) while
-// the other waves wait at the barrier; splitting the sum over all four waves
-// measured slower.
-template <int NS>
+// sums them (up to UNROLL loads per lane in flight per slot) while the other
+// waves wait at the barrier; splitting the sum over all four waves measured
+// slower.
+template <int NS, int UNROLL = kPartialUnroll>
 __device__ __forceinline__ void blockSumPartials(const double* const (&p)[NS], int n, double* sm,
                                                  double (&out)[NS]) {
   if (threadIdx.x < 64) {
-    wavePartials<NS>(p, n, 0.0, OpSum{}, out);
+    wavePartials<NS, OpSum, UNROLL>(p, n, 0.0, OpSum{}, out);
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int q = 0; q < NS; ++q) sm[q] = out[q];
@@ -233,10 +233,11 @@ __device__ __forceinline__ void blockSumPartials(const double* const (&p)[NS], i
   __syncthreads();
 }
 
+template <int UNROLL = kPartialUnroll>
 __device__ __forceinline__ double blockSumPartials(const double* p, int n, double* sm) {
   const double* const s[1] = {p};
   double r[1];
-  blockSumPartials<1>(s, n, sm, r);
+  blockSumPartials<1, UNROLL>(s, n, sm, r);
   return r[0];
 }
 

@@ -11,10 +11,6 @@ namespace cfdgpu {
 __global__ void k_begin_iteration(const Inst*, int);
 __global__ void k_grad(const Inst*);
 __global__ void k_mom(const Inst*);
-__global__ void k_gs_lex_mom(const Inst*, int);
-__global__ void k_gs_lex_turb(const Inst*, int);
-__global__ void k_gs_lex_mom_wide(const Inst*, int);
-__global__ void k_gs_lex_turb_wide(const Inst*, int);
 __global__ void k_gs_wave_mom(const Inst*, int);
 __global__ void k_gs_wave_turb(const Inst*, int);
 __global__ void k_gs_wave_mom_wide(const Inst*, int);
@@ -42,19 +38,13 @@ __global__ void k_cg_b_f32(const Inst*, int);
 
 int gsThreadsFor(int nJ) { return gsLanes(nJ); }
 
-// The lexicographic sweep kernel: decoupled waves (DPP within a wave, LDS
-// step counters between waves) or the barrier-per-step wavefront.
-constexpr bool kDecoupledSweep = true;
-
+// Dynamic LDS of the lexicographic sweep (gsLexWave): row 0, the [waves][64]
+// hand-off ring and a dummy per lane (doubles); the partner map, the step
+// counters and a dummy per lane (ints).
 size_t gsLdsFor(int nJ, int nI, int threads) {
   (void)nJ;
   const size_t t = static_cast<size_t>(threads), n = static_cast<size_t>(nI), waves = (t + 63) / 64;
-  // barrier wavefront: [2][2][threads] step buffer + [2][nI] row 0 (doubles) + [nI] partner map (ints)
-  const size_t barrier = (2 * 2 * t + 2 * n) * sizeof(double) + n * sizeof(int);
-  // decoupled waves: row 0, the [waves][64] hand-off ring and a dummy per lane (doubles); partner map, step
-  // counters and a dummy per lane (ints)
-  const size_t decoupled = (n + waves * 64 + t) * sizeof(double) + (n + waves + t) * sizeof(int);
-  return barrier > decoupled ? barrier : decoupled;
+  return (n + waves * 64 + t) * sizeof(double) + (n + waves + t) * sizeof(int);
 }
 
 void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s, StageRecorder* rec) {
@@ -75,10 +65,10 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   end(CFDGPU_STAGE_MOMENTUM_ASSEMBLY);
   begin(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   if (lex) {
-    // decoupled: u and v in two workgroups; barrier wavefront: both in one
-    const dim3 momGrid(kDecoupledSweep ? 2 : 1, cfg.batch);
-    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom : k_gs_lex_mom, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
-    else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_mom_wide : k_gs_lex_mom_wide, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    // u and v in two workgroups side by side
+    const dim3 momGrid(2, cfg.batch);
+    if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_wave_mom, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
+    else hipLaunchKernelGGL(k_gs_wave_mom_wide, momGrid, dim3(cfg.gsThreads), cfg.gsLds, s, d, h.momSweeps);
     end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
     begin(CFDGPU_STAGE_SWEEP_COPY);
     hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 0, h.momSweeps);
@@ -133,8 +123,8 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     end(CFDGPU_STAGE_SST_ASSEMBLY);
     begin(CFDGPU_STAGE_SST_SWEEPS);
     if (lex) {
-      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_turb : k_gs_lex_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
-      else hipLaunchKernelGGL(kDecoupledSweep ? k_gs_wave_turb_wide : k_gs_lex_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      if (cfg.gsThreads <= 256) hipLaunchKernelGGL(k_gs_wave_turb, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
+      else hipLaunchKernelGGL(k_gs_wave_turb_wide, dim3(2, cfg.batch), dim3(cfg.gsThreads), cfg.gsLds, s, d, 3);
       end(CFDGPU_STAGE_SST_SWEEPS);
       begin(CFDGPU_STAGE_SWEEP_COPY);
       hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 1, 3);

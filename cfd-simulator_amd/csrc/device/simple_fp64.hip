@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
 // reference bit for bit for any partner map. Off-diagonal terms are summed in
 // the reference's cellFaces order: below, right, above, left (Mesh.cpp:206-213).
 //
-// One workgroup per system; NC components share one matrix (u and v).
+// One workgroup per component; u and v share one matrix.
 // ---------------------------------------------------------------------------
 
 struct GsSys {
@@ -317,8 +317,8 @@ struct GsSys {
 // momentum, and the two ping-pong copies of the unknowns as pairs (x[0] holds
 // the starting values, written by the assembly kernel; after n sweeps the
 // result is in x[n & 1], which k_gs_unskew copies to the field). All pointers
-// are global-space (see gmem), so the per-step LDS barrier never waits for
-// HBM traffic.
+// are global-space (see gmem), so a wait on LDS never also waits for HBM
+// traffic.
 template <int NC>
 struct GsSkew {
   const CFD_GLOBAL dpair* row[3];
@@ -326,173 +326,14 @@ struct GsSkew {
   CFD_GLOBAL dpair* x[2];
 };
 
-// What a lane needs for its cell on one wavefront step that does not depend
-// on this sweep's updates: the matrix row, the right-hand side, and the old
-// values of the cell, the cell above and the cell to the right. (The last is
-// also the next step's xC; fetching it again keeps each ring slot
-// self-contained, so the wait for a slot never covers the slot after it.)
-template <int NC>
-struct GsCell {
-  double aP, aS, aN, aW, aE;
-  double b[NC], xC[NC], xN[NC], xE[NC];
-};
-
-// Fetch lane's cell on step k: three (NC = 1) or four (NC = 2) loads of the
-// row and three of the unknowns. Loads are unconditional with the step clamped
-// into [0, steps - 1], so every lane issues the same loads on every step and
-// the compiler can keep the newer ones in flight while it waits for the
-// oldest; values of cells off the wavefront are never used. Indices are
-// k * R + lane, (k + 1) * R + lane and (k + 1) * R + lane + 1, below
-// skewLength(nI, nJ, R).
-template <int NC>
-__device__ __forceinline__ void gsFetch(const GsSkew<NC>& S, const CFD_GLOBAL dpair* xOld, int R, int steps,
-                                        int lane, int k, GsCell<NC>& g) {
-  const int at = min(k, steps - 1) * R + lane;
-  const int atE = min(k + 1, steps - 1) * R + lane;
-  const int atN = atE + 1;
-  const dpair r0 = S.row[0][at];
-  const dpair r1 = S.row[1][at];
-  const dpair r2 = S.row[2][at];
-  g.aP = r0[0];
-  g.aS = r0[1];
-  g.aN = r1[0];
-  g.aW = r1[1];
-  g.aE = r2[0];
-  g.b[0] = r2[1];
-  if constexpr (NC == 2) g.b[1] = S.b1[at];
-  if constexpr (NC == 2) {
-    const dpair c = xOld[at];
-    const dpair e = xOld[atE];
-    const dpair n = xOld[atN];
-    g.xC[0] = c[0];
-    g.xE[0] = e[0];
-    g.xN[0] = n[0];
-    g.xC[1] = c[1];
-    g.xE[1] = e[1];
-    g.xN[1] = n[1];
-  } else {
-    // one component: 8-byte loads of the pair's first half. A 16-byte load
-    // with an unused half leaves registers the compiler reuses at once, and
-    // reusing a register a load is still writing forces a wait for every
-    // load in flight.
-    const CFD_GLOBAL double* x1 = reinterpret_cast<const CFD_GLOBAL double*>(xOld);
-    g.xC[0] = x1[2 * at];
-    g.xE[0] = x1[2 * atE];
-    g.xN[0] = x1[2 * atN];
-  }
-}
-
-// Barrier for the LDS hand-off between wavefront steps only: waits for this
-// wave's LDS traffic, not for its outstanding global loads (the read-ahead),
-// which __syncthreads() would drain at every step. Global memory is not
-// shared between lanes within a sweep (old values are read from one buffer,
-// new values written to another), so no global fence is needed here.
-__device__ __forceinline__ void ldsBarrier() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt untouched
-  __builtin_amdgcn_s_barrier();
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-
 // Lexicographic Gauss-Seidel on the anti-diagonal schedule: lane j owns row j
-// and handles cell (k - j, j) on step k. The value below comes from the lane
-// underneath through LDS (written on step k - 1); row 0 takes its wake
-// partner's current value from an LDS copy of the row, updated in place, so
+// and handles cell (k - j, j) on step k; row 0 takes its wake partner's
+// current value from an LDS copy of the row, updated in place by lane 0, so
 // the partner coupling follows the reference's order for any partner map.
-// D is the read-ahead: a ring of D cells per lane, slot q holding step
-// r * D + q and refilled with the step D ahead right after use.
-template <int NC, int D>
-// R is the instance's layout stride (its lane count); the block may be wider
-// (a batch launches with its widest member's lanes). Lanes past R take no
-// cell; their global addresses are clamped to lane R - 1 and LDS is laid out
-// by the block width B.
-__device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSkew<NC>& S, int sweeps) {
-  HIP_DYNAMIC_SHARED(double, lds)
-  const int B = static_cast<int>(blockDim.x);
-  double* buf = lds;                                        // [2][NC][B]: value just written by each row
-  double* row0 = lds + 2 * NC * B;                          // [NC][nI]: row j = 0, updated in place
-  int* partner = reinterpret_cast<int*>(row0 + NC * nI);    // [nI]
-  const int lane = static_cast<int>(threadIdx.x);
-  const int laneG = min(lane, R - 1);
-  const bool active = lane < nJ;
-  const int steps = skewSteps(nI, nJ);
-  // Steps past nI + nJ - 2 touch no cell (k - j >= nI for every row), so the
-  // count is rounded up to whole rings and the loop body has no tail.
-  const int rings = (steps + D - 1) / D;
-  const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
-  for (int t = lane; t < nI; t += B) partner[t] = partnerSrc[t];
-  for (int sweep = 0; sweep < sweeps; ++sweep) {
-    // selects, not S.x[sweep & 1]: a run-time index would put S in scratch
-    const CFD_GLOBAL dpair* xOld = (sweep & 1) ? S.x[1] : S.x[0];
-    CFD_GLOBAL dpair* xNew = (sweep & 1) ? S.x[0] : S.x[1];
-    for (int t = lane; t < nI; t += B) {
-      const dpair v = xOld[skewIndex(t, 0, R)];
-      row0[t] = v[0];
-      if constexpr (NC == 2) row0[nI + t] = v[1];
-    }
-    __syncthreads();
-    double xw[NC];
-#pragma unroll
-    for (int q = 0; q < NC; ++q) xw[q] = 0.0;
-    GsCell<NC> ring[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) gsFetch<NC>(S, xOld, R, steps, laneG, q, ring[q]);
-    for (int r = 0; r < rings; ++r) {
-#pragma unroll
-      for (int q = 0; q < D; ++q) {
-        const int k = r * D + q;
-        const int i = k - lane;
-        const bool on = active && i >= 0 && i < nI;
-        const GsCell<NC>& g = ring[q];
-        const int pr = partner[min(max(i, 0), nI - 1)];
-        const bool hasBelow = (lane > 0) || (pr >= 0);
-        double xn[NC];
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-          const double below =
-              (lane > 0) ? buf[((k - 1) & 1) * NC * B + m * B + (lane - 1)] : row0[m * nI + max(pr, 0)];
-          // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
-          // a missing neighbour's term is skipped, not added as zero
-          double off = 0.0;
-          off = hasBelow ? off + g.aS * below : off;
-          off = (i < nI - 1) ? off + g.aE * g.xE[m] : off;
-          off = (lane < nJ - 1) ? off + g.aN * g.xN[m] : off;
-          off = (i > 0) ? off + g.aW * xw[m] : off;
-          // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88)
-          xn[m] = (fabs(g.aP) > 0.0) ? (g.b[m] - off) / g.aP : g.xC[m];
-        }
-        {
-          // Every lane stores on every step, so the number of memory operations
-          // in flight is the same on every path and the compiler's waits stay
-          // partial. Lanes off the wavefront write the padding row past the
-          // last step, which no fetch reads.
-          const int at = on ? k * R + lane : (steps + 1) * R + laneG;
-          const dpair v = {xn[0], NC == 2 ? xn[NC - 1] : 0.0};
-          xNew[at] = v;
-        }
-        if (on && lane == 0) {
-#pragma unroll
-          for (int m = 0; m < NC; ++m) row0[m * nI + i] = xn[m];
-        }
-        // Off the wavefront these are never read: the row above is off it on
-        // the next step too, and this row's next cell has i + 1 <= 0 or >= nI.
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-          buf[(k & 1) * NC * B + m * B + lane] = xn[m];
-          xw[m] = xn[m];
-        }
-        gsFetch<NC>(S, xOld, R, steps, laneG, k + D, ring[q]);
-        ldsBarrier();
-      }
-    }
-    __syncthreads();  // this sweep's writes are the next sweep's old values
-  }
-}
-
-// The same sweep with the waves decoupled, one component per workgroup.
-// Inside a wave the value below comes from the lane underneath by a
-// whole-wave DPP shift of the previous step's results, so no LDS round trip
-// and no block barrier separate two steps. Only the row at a wave boundary
+// One component per workgroup, and the waves decoupled: inside a wave the
+// value below comes from the lane underneath by a whole-wave DPP shift of the
+// previous step's results, so no LDS round trip and no block barrier separate
+// two steps. Only the row at a wave boundary
 // crosses waves: lane 63 of wave w posts its value for step k in an LDS ring
 // (kSweepHand slots) and then raises the wave's step counter; wave w + 1
 // reads it on step k + 1 once the counter shows k. A wave runs ahead of the
@@ -501,8 +342,9 @@ __device__ void gsLexSkew(int nI, int nJ, int R, const int* partnerG, const GsSk
 // the bound is hit the sweep gives up and counts a stall in
 // flags[fSweepStall], so no wave can hang the device. The step has no
 // divergent branch: writes that only one lane makes go, on the other lanes,
-// to a per-lane dummy slot. The arithmetic and its order are those of
-// gsLexSkew.
+// to a per-lane dummy slot. Off-diagonal terms are summed in the reference's
+// cellFaces order (below, right, above, left; Mesh.cpp:206-213), a missing
+// neighbour's term skipped, not added as zero.
 constexpr int kSweepHand = 64;  // a power of two
 constexpr int kSpinLimit = 1 << 22;
 
@@ -539,7 +381,8 @@ struct GsCell1 {
 // Every value is loaded at its own width, none with an unused half: a load
 // whose result is partly unused leaves registers the compiler reuses at once,
 // and reusing a register a load is still writing forces a wait for every load
-// in flight. Steps are clamped into [0, steps - 1] as in gsFetch.
+// in flight. Steps are clamped into [0, steps - 1], so every lane issues the
+// same loads on every step; values of cells off the wavefront are never used.
 template <bool SepB>
 __device__ __forceinline__ void gsFetch1(const GsOne& S, const CFD_GLOBAL double* xOld, int R, int steps, int lane,
                                          int k, GsCell1& g) {
@@ -688,7 +531,7 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
 // Momentum: u and v share one matrix, one workgroup. Turbulence: blockIdx.x
 // 0 sweeps k, 1 sweeps omega (independent systems, run side by side). The
 // launch gives the block I.gsR lanes (the layout's stride).
-template <int D, bool Decoupled>
+template <int D>
 __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped] || sweeps <= 0) return;
@@ -696,21 +539,17 @@ __device__ __forceinline__ void gsLexMomentum(const Inst* insts, int sweeps) {
                      gpair(static_cast<const double2*>(I.skMomRow[2]))},
                     gmem(static_cast<const double*>(I.skMomBy)),
                     {gpair(I.skMomX[0]), gpair(I.skMomX[1])}};
-  if constexpr (Decoupled) {
-    // u (blockIdx.x 0) and v (1) in two workgroups, side by side
-    const int half = (blockIdx.x == 0) ? 0 : 1;
-    const GsOne C{{S.row[0], S.row[1], S.row[2]},
-                  S.b1,
-                  {reinterpret_cast<CFD_GLOBAL double*>(S.x[0]), reinterpret_cast<CFD_GLOBAL double*>(S.x[1])},
-                  half};
-    if (half == 0) gsLexWave<D, false>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
-    else gsLexWave<D, true>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
-  } else {
-    gsLexSkew<2, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
-  }
+  // u (blockIdx.x 0) and v (1) in two workgroups, side by side
+  const int half = (blockIdx.x == 0) ? 0 : 1;
+  const GsOne C{{S.row[0], S.row[1], S.row[2]},
+                S.b1,
+                {reinterpret_cast<CFD_GLOBAL double*>(S.x[0]), reinterpret_cast<CFD_GLOBAL double*>(S.x[1])},
+                half};
+  if (half == 0) gsLexWave<D, false>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
+  else gsLexWave<D, true>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
 }
 
-template <int D, bool Decoupled>
+template <int D>
 __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
   const Inst& I = insts[blockIdx.y];
   if (I.flags[fStopped] || sweeps <= 0) return;
@@ -720,42 +559,30 @@ __device__ __forceinline__ void gsLexTurbulence(const Inst* insts, int sweeps) {
                      gpair(static_cast<const double2*>(row[2]))},
                     nullptr,
                     {gpair(I.skTurbX[sys][0]), gpair(I.skTurbX[sys][1])}};
-  if constexpr (Decoupled) {
-    const GsOne C{{S.row[0], S.row[1], S.row[2]},
-                  nullptr,
-                  {reinterpret_cast<CFD_GLOBAL double*>(S.x[0]), reinterpret_cast<CFD_GLOBAL double*>(S.x[1])},
-                  0};
-    gsLexWave<D, false>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
-  } else {
-    gsLexSkew<1, D>(I.nI, I.nJ, I.gsR, I.partner, S, sweeps);
-  }
+  const GsOne C{{S.row[0], S.row[1], S.row[2]},
+                nullptr,
+                {reinterpret_cast<CFD_GLOBAL double*>(S.x[0]), reinterpret_cast<CFD_GLOBAL double*>(S.x[1])},
+                0};
+  gsLexWave<D, false>(I.nI, I.nJ, I.gsR, I.partner, C, sweeps, I.flags + fSweepStall);
 }
 
-// Up to 256 rows: one wave per 64 rows, each lane holding a ring of cells
-// fetched ahead (4 for u and v, 6 for k or omega).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_gs_lex_mom(const Inst* insts, int sweeps) {
-  gsLexMomentum<4, false>(insts, sweeps);
-}
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_gs_lex_turb(const Inst* insts, int sweeps) {
-  gsLexTurbulence<6, false>(insts, sweeps);
-}
-// The decoupled-wave sweep (gsLexWave): one workgroup per component (u and v
-// side by side for momentum, k and omega for turbulence).
+// The sweep kernels (gsLexWave): one workgroup per component (u and v side by
+// side for momentum, k and omega for turbulence). Up to 256 rows: one wave
+// per 64 rows, each lane holding a ring of 8 cells fetched ahead; up to 1024
+// rows: a ring of 3.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_wave_mom(const Inst* insts, int sweeps) {
-  gsLexMomentum<8, true>(insts, sweeps);
+  gsLexMomentum<8>(insts, sweeps);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gs_wave_turb(const Inst* insts, int sweeps) {
-  gsLexTurbulence<8, true>(insts, sweeps);
+  gsLexTurbulence<8>(insts, sweeps);
 }
 __global__ void __launch_bounds__(1024) k_gs_wave_mom_wide(const Inst* insts, int sweeps) {
-  gsLexMomentum<3, true>(insts, sweeps);
+  gsLexMomentum<3>(insts, sweeps);
 }
 __global__ void __launch_bounds__(1024) k_gs_wave_turb_wide(const Inst* insts, int sweeps) {
-  gsLexTurbulence<3, true>(insts, sweeps);
+  gsLexTurbulence<3>(insts, sweeps);
 }
 
 // After `sweeps` sweeps the result is in the skewed copy x[sweeps & 1]:
@@ -776,14 +603,6 @@ __global__ void __launch_bounds__(kBlock) k_gs_unskew(const Inst* insts, int sys
     I.k[c] = mx(k[0], 0.0);
     I.w[c] = mx(w[0], 1.0e-10);
   }
-}
-
-// More than 256 rows (up to 1024): a ring of two cells.
-__global__ void __launch_bounds__(1024) k_gs_lex_mom_wide(const Inst* insts, int sweeps) {
-  gsLexMomentum<2, false>(insts, sweeps);
-}
-__global__ void __launch_bounds__(1024) k_gs_lex_turb_wide(const Inst* insts, int sweeps) {
-  gsLexTurbulence<2, false>(insts, sweeps);
 }
 
 // ---------------------------------------------------------------------------
@@ -1054,6 +873,12 @@ __device__ __forceinline__ CgStore<float> cgStore<float>(const Inst& I) {
   return CgStore<float>{I.fpP, I.fpN, I.fpE, I.fpWk, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp};
 }
 
+// Loads in flight per lane when a CG launch sums the previous launch's
+// partials: 64 x 8 = 512 partials (a 262,144-cell solve's CG blocks) in one
+// round trip, at half the registers of the general sum (launch A holds two
+// slots' loads beside its stencil operands).
+constexpr int kCgPartialUnroll = 8;
+
 // a value as it is after a store and a load of type T
 template <typename T>
 __device__ __forceinline__ double asStored(double v) {
@@ -1123,6 +948,8 @@ __device__ __forceinline__ void cgInit(const Inst* insts, double* sm) {
 // the two memory round trips overlap.
 template <typename T>
 __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
+  constexpr int CELLS = kCgCells;
+  constexpr int THREADS = kBlock;
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
@@ -1138,11 +965,11 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
   const CFD_GLOBAL double* red = gmem(static_cast<const double*>(I.red));
   const CFD_GLOBAL int* partnerG = gmem(static_cast<const int*>(I.partner));
-  int cq[kCgCells], iq[kCgCells], jq[kCgCells], prq[kCgCells];
-  bool live[kCgCells];
+  int cq[CELLS], iq[CELLS], jq[CELLS], prq[CELLS];
+  bool live[CELLS];
 #pragma unroll
-  for (int q = 0; q < kCgCells; ++q) {
-    const int c = base + q * kBlock;
+  for (int q = 0; q < CELLS; ++q) {
+    const int c = base + q * THREADS;
     live[q] = c < I.N;
     cq[q] = live[q] ? c : 0;
     iq[q] = cq[q] % nI;
@@ -1163,12 +990,12 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   const CFD_GLOBAL T* pN = gmem(S.pN);
   const CFD_GLOBAL T* pE = gmem(S.pE);
   const CFD_GLOBAL T* pWk = gmem(S.pWk);
-  double zc[kCgCells], zS[kCgCells], zN[kCgCells], zW[kCgCells], zE[kCgCells];
-  double oc[kCgCells], oS[kCgCells], oN[kCgCells], oW[kCgCells], oE[kCgCells];
-  double aP[kCgCells], aS[kCgCells], aN[kCgCells], aW[kCgCells], aE[kCgCells];
-  bool hasS[kCgCells], hasN[kCgCells], hasW[kCgCells], hasE[kCgCells];
+  double zc[CELLS], zS[CELLS], zN[CELLS], zW[CELLS], zE[CELLS];
+  double oc[CELLS], oS[CELLS], oN[CELLS], oW[CELLS], oE[CELLS];
+  double aP[CELLS], aS[CELLS], aN[CELLS], aW[CELLS], aE[CELLS];
+  bool hasS[CELLS], hasN[CELLS], hasW[CELLS], hasE[CELLS];
 #pragma unroll
-  for (int q = 0; q < kCgCells; ++q) {
+  for (int q = 0; q < CELLS; ++q) {
     const int cc = cq[q], i = iq[q], j = jq[q];
     hasN[q] = j < nJ - 1;
     hasW[q] = i > 0;
@@ -1190,7 +1017,7 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
     hasS[q] = j > 0;
   }
 #pragma unroll
-  for (int q = 0; q < kCgCells; ++q) {
+  for (int q = 0; q < CELLS; ++q) {
     if (jq[q] == 0 && prq[q] >= 0) {
       hasS[q] = true;
       aS[q] = pWk[cq[q]];
@@ -1201,7 +1028,7 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   double sums[2];
   {
     const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
-    blockSumPartials<2>(slots, I.cgBlocks, sm, sums);
+    blockSumPartials<2, kCgPartialUnroll>(slots, I.cgBlocks, sm, sums);
   }
   const double nrm = sums[0];
   const double rzNew = sums[1];
@@ -1227,9 +1054,9 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   // p = z + beta p_old (LinearSystem.cpp:199-201), as stored; iteration 0 has p = z
   auto pv = [&](double zq, double oq) { return (it == 0) ? oq : asStored<T>(zq + beta * oq); };
 #pragma unroll
-  for (int q = 0; q < kCgCells; ++q) {
+  for (int q = 0; q < CELLS; ++q) {
     if (!live[q]) continue;
-    const int c = base + q * kBlock;
+    const int c = base + q * THREADS;
     const double pc = pv(zc[q], oc[q]);
     // LinearSystem::multiply order: diagonal, then the faces of the cell
     // (below/wake, above, left, right as the structured sweep visits them)
@@ -1274,7 +1101,7 @@ __device__ __forceinline__ void cgB(const Inst* insts, int it, double* sm) {
     apc[q] = gmem(static_cast<const T*>(S.Ap))[cc];
     d[q] = gmem(S.pP)[cc];
   }
-  const double pap = blockSumPartials(partialSlot(I, sCgPAp), I.cgBlocks, sm);
+  const double pap = blockSumPartials<kCgPartialUnroll>(partialSlot(I, sCgPAp), I.cgBlocks, sm);
   if (stopped || cgDone) return;
   if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
     if (threadIdx.x == 0 && blockIdx.x == 0) gmem(I.flags)[fCgDone] = 1;

@@ -6,6 +6,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -631,6 +632,56 @@ void checkSweepStall(const int* flags) {
     throw std::runtime_error("the lexicographic sweep stalled (a wave's wait for its neighbour timed out)");
 }
 
+namespace {
+bool graphsEnabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("CFDGPU_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+}  // namespace
+
+bool IterationGraph::Key::operator==(const Key& o) const {
+  return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && batch == o.batch &&
+         gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder && precision == o.precision &&
+         momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgMax == o.cgMax && hasPRef == o.hasPRef &&
+         model == o.model;
+}
+
+IterationGraph::Key IterationGraph::keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h) {
+  return Key{d, cfg.maxBlocks, cfg.maxCgBlocks, cfg.batch, cfg.gsThreads, cfg.gsLds, h.gsOrder, h.precision,
+             h.momSweeps, h.nonOrthPasses, h.cgMax, h.hasPRef, h.model};
+}
+
+void IterationGraph::reset() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void IterationGraph::run(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s, int count) {
+  if (count <= 0) return;
+  if (!graphsEnabled()) {
+    for (int n = 0; n < count; ++n) launchIteration(d, cfg, h, s);
+    return;
+  }
+  const Key k = keyOf(d, cfg, h);
+  if (!exec_ || !(k == key_)) {
+    reset();
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    launchIteration(d, cfg, h, s);
+    const hipError_t launched = hipGetLastError();
+    const hipError_t ended = hipStreamEndCapture(s, &graph_);
+    check(launched, "launch (capture)");
+    check(ended, "hipStreamEndCapture");
+    check(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    key_ = k;
+  }
+  for (int n = 0; n < count; ++n) check(hipGraphLaunch(exec_, s), "hipGraphLaunch");
+}
+
 void Engine::checkStall() const {
   int flags[kIFlags] = {};
   check(hipMemcpyAsync(flags, h_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream_), "flags");
@@ -643,7 +694,7 @@ int Engine::iterate(int count, cfdgpu_monitor* out) {
   // a plain iterate() never stops by itself (SimpleSolver::iterate never fails)
   int zero[2] = {0, 0};
   check(hipMemcpyAsync(h_.flags + fStopped, zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
-  for (int n = 0; n < count; ++n) launchIteration(dInst_, cfg_, h_, stream_);
+  graph_.run(dInst_, cfg_, h_, stream_, count);
   check(hipGetLastError(), "launch");
   if (out) readMonitor(out);
   else check(hipStreamSynchronize(stream_), "iterate sync");
@@ -685,7 +736,7 @@ int Engine::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
   int flags[kIFlags] = {};
   for (;;) {
     const int n = (rule.max_iterations > 0) ? std::min(chunk, rule.max_iterations - launched) : chunk;
-    for (int q = 0; q < n; ++q) launchIteration(dInst_, cfg_, h_, stream_);
+    graph_.run(dInst_, cfg_, h_, stream_, n);
     launched += n;
     check(hipGetLastError(), "launch");
     check(hipMemcpyAsync(flags, h_.flags, sizeof(flags), hipMemcpyDeviceToHost, stream_), "flags");
@@ -966,7 +1017,7 @@ void Batch::gather() {
 int Batch::iterate(int count, cfdgpu_monitor* out) {
   gather();
   launchResumeAll(dInsts_, size(), stream_);
-  for (int n = 0; n < count; ++n) launchIteration(dInsts_, cfg_, host_.front(), stream_);
+  graph_.run(dInsts_, cfg_, host_.front(), stream_, count);
   check(hipGetLastError(), "launch");
   check(hipStreamSynchronize(stream_), "batch sync");
   for (Engine* e : members_) e->checkStall();
@@ -998,7 +1049,7 @@ int Batch::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
   int launched = 0;
   for (;;) {
     const int m = (rule.max_iterations > 0) ? std::min(chunk, rule.max_iterations - launched) : chunk;
-    for (int q = 0; q < m; ++q) launchIteration(dInsts_, cfg_, host_.front(), stream_);
+    graph_.run(dInsts_, cfg_, host_.front(), stream_, m);
     launched += m;
     check(hipGetLastError(), "launch");
     for (size_t q = 0; q < n; ++q)

@@ -55,6 +55,39 @@ class StageClock;
 // throws when flags (kIFlags ints of one instance) record a stalled sweep
 void checkSweepStall(const int* flags);
 
+
+// The launches of one outer iteration captured once into a HIP graph and
+// replayed: the same kernels with the same arguments in the same order (the
+// iteration's scalars and stop flags live on the device, so one capture
+// serves every iteration). It is captured again whenever anything the launch
+// schedule reads changes: the descriptor array, the launch shape, or the
+// settings that pick kernels and loop counts. Set CFDGPU_GRAPHS=0 in the
+// environment to queue the launches one by one instead.
+class IterationGraph {
+ public:
+  IterationGraph() = default;
+  ~IterationGraph() { reset(); }
+  IterationGraph(const IterationGraph&) = delete;
+  IterationGraph& operator=(const IterationGraph&) = delete;
+  // queue `count` outer iterations on s; h is the host copy of the first
+  // instance's descriptor (it drives the launch schedule)
+  void run(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s, int count);
+  void reset();
+
+ private:
+  struct Key {
+    const Inst* d;
+    int maxBlocks, maxCgBlocks, batch, gsThreads;
+    size_t gsLds;
+    int gsOrder, precision, momSweeps, nonOrthPasses, cgMax, hasPRef, model;
+    bool operator==(const Key& o) const;
+  };
+  static Key keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h);
+  Key key_{};
+  hipGraph_t graph_{nullptr};
+  hipGraphExec_t exec_{nullptr};
+};
+
 // Device arena for one instance: every array of Inst carved out of one allocation.
 struct DeviceArena {
   void* base{nullptr};
@@ -149,6 +182,7 @@ class Engine {
   Inst* dInst_{nullptr};
   hipStream_t stream_{nullptr};
   LaunchCfg cfg_{};
+  IterationGraph graph_;
   std::unique_ptr<StageClock> clock_;
   // device surface integration: the wall table and its scratch, made on first use
   bool wallBuilt_{false}, hasWall_{false};
@@ -178,6 +212,7 @@ class Batch {
   Inst* dInsts_{nullptr};
   hipStream_t stream_{nullptr};
   LaunchCfg cfg_{};
+  IterationGraph graph_;
 };
 
 }  // namespace cfdgpu

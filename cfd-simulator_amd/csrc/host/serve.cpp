@@ -301,7 +301,7 @@ void Server::launchChunk() {
              "upload roster");
     rosterDirty_ = false;
   }
-  for (int n = 0; n < opt_.chunk; ++n) launchIteration(dInsts_, cfg_, host_.front(), stream_);
+  graph_.run(dInsts_, cfg_, host_.front(), stream_, opt_.chunk);
   hipCheck(hipGetLastError(), "launch");
   for (size_t q = 0; q < host_.size(); ++q)
     hipCheck(hipMemcpyAsync(hFlags_ + q * kIFlags, host_[q].flags, sizeof(int) * kIFlags, hipMemcpyDeviceToHost,

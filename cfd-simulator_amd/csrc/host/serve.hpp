@@ -107,6 +107,7 @@ class Server {
   int* hFlags_{nullptr};  // pinned, kIFlags per slot
   hipStream_t stream_{nullptr};
   LaunchCfg cfg_{};
+  IterationGraph graph_;
   bool launched_{false};
   bool rosterDirty_{true};
 

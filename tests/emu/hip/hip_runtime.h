@@ -245,10 +245,46 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
 }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 
+// Graphs: a capture records each launch as a closure (arguments by value);
+// a launch of the instantiated graph runs them in order.
+struct emu_graph {
+  std::vector<std::function<void()>> ops;
+};
+using hipGraph_t = emu_graph*;
+using hipGraphExec_t = emu_graph*;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+namespace cfdemu {
+inline thread_local emu_graph* capturing = nullptr;
+}
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  cfdemu::capturing = new emu_graph();
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  *g = cfdemu::capturing;
+  cfdemu::capturing = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t) {
+  *e = new emu_graph(*g);
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& f : e->ops) f();
+  return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+
 template <class... P, class... A>
 void hipLaunchKernelGGL(void (*kernel)(P...), dim3 grid, dim3 block, size_t shmem, hipStream_t, A... args) {
-  for (unsigned by = 0; by < grid.y; ++by)
-    for (unsigned bx = 0; bx < grid.x; ++bx)
-      cfdemu::runBlock(dim3(bx, by), block, shmem, [&] { kernel(static_cast<P>(args)...); });
+  auto run = [=] {
+    for (unsigned by = 0; by < grid.y; ++by)
+      for (unsigned bx = 0; bx < grid.x; ++bx)
+        cfdemu::runBlock(dim3(bx, by), block, shmem, [&] { kernel(static_cast<P>(args)...); });
+  };
+  if (cfdemu::capturing) cfdemu::capturing->ops.push_back(run);
+  else run();
 }
 
