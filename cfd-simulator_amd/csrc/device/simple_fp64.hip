@@ -958,10 +958,11 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   const int nJ = I.nJ;
   const CgStore<T> S = cgStore<T>(I);
   // Every operand is read through the global address space, without a branch
-  // around its load: the flags, the CG scalars and the cells' wake partners
-  // are one memory round trip, the cells' operands and the partials a second
-  // (a FLAT or branch-guarded load would make a wait drain every load in
-  // flight).
+  // around its load, and issued before the stop flags are tested: the flags,
+  // the CG scalars, the cells' wake partners and the stencil's operands are
+  // one memory round trip (a FLAT or branch-guarded load would make a wait
+  // drain every load in flight). Only the lanes of row 0 then wait for their
+  // partner's values, and the partials are a second round trip.
   const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
   const CFD_GLOBAL double* red = gmem(static_cast<const double*>(I.red));
   const CFD_GLOBAL int* partnerG = gmem(static_cast<const int*>(I.partner));
@@ -980,7 +981,6 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   const int cgDone = flagsG[fCgDone];
   const double norm0 = red[sCgNorm0];
   const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
-  if (stopped || cgDone) return;
   // operands: z and p_old at each cell and its four neighbours, the matrix row
   const CFD_GLOBAL T* z = gmem(static_cast<const T*>(S.z));
   const CFD_GLOBAL T* pOld = gmem(static_cast<const T*>((it & 1) ? S.pd0 : S.pd1));
@@ -1016,6 +1016,7 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
     if (it > 0) { zc[q] = z[cc]; zS[q] = z[cS]; zN[q] = z[cN]; zW[q] = z[cW]; zE[q] = z[cE]; }
     hasS[q] = j > 0;
   }
+  if (stopped || cgDone) return;  // the whole block: the flags are uniform
 #pragma unroll
   for (int q = 0; q < CELLS; ++q) {
     if (jq[q] == 0 && prq[q] >= 0) {
