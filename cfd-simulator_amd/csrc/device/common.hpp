@@ -21,12 +21,14 @@ __device__ __forceinline__ bool imposes(const FaceBC& bc, double F) {
   return bc.kind == kWall || bc.kind == kInlet || (bc.kind == kFarField && F < bc.dead);
 }
 
-// Lane l of the wave receives lane l - 1's value (lane 0 receives 0): two
-// 32-bit DPP moves with the whole-wave shift wave_shr:1.
-__device__ __forceinline__ double waveShiftUp(double v) {
+// Lane l of the wave receives lane l - 1's value and lane 0 receives `fill`
+// (a DPP lane with no source keeps the old value): two 32-bit DPP moves with
+// the whole-wave shift wave_shr:1.
+__device__ __forceinline__ double waveShiftUp(double v, double fill = 0.0) {
   const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x138, 0xF, 0xF, false);
+  const long long f = __builtin_bit_cast(long long, fill);
+  const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(f), static_cast<int>(b), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(f >> 32), static_cast<int>(b >> 32), 0x138, 0xF, 0xF, false);
   return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
 

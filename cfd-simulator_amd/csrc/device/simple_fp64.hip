@@ -471,9 +471,9 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         // writes: post its hand-off value (counters run one step behind)
         ldsOrder();
         ldsPost((k - 1 >= kBegin && l == 63) ? &posted[w] : &dummyInt[lane], k - 1);
-        const double low = lowNext;
-        const int pr = prCur;
-        computedHere(low);  // every lane: no branch around the value
+        // the value below: lane l - 1's result of the last step, and on lane
+        // 0 the value read ahead for this step (both put in place by the shift)
+        const double below = up;
         // read ahead for step k + 1
         if (w == 0) {
           prCur = prNext;
@@ -487,15 +487,17 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
           }
           lowNext = hand[(w - 1) * kSweepHand + (k & (kSweepHand - 1))];
         }
-        const bool hasBelow = (lane > 0) || (pr >= 0);
-        const double below = (l > 0) ? up : low;
-        // cellFaces order: below, right, above, left (Mesh.cpp:206-213);
-        // a missing neighbour's term is skipped, not added as zero
+        // cellFaces order: below, right, above, left (Mesh.cpp:206-213). The
+        // reference skips a missing neighbour; its coefficient is zero in the
+        // assembled row, so here its term adds (+-0) to a sum that is never
+        // -0 (it starts at +0, and x + y is -0 only if both are), which
+        // leaves the sum as it is. (The neighbour's value is a finite one:
+        // a cell of the field, the zeroed padding, or row 0.)
         double off = 0.0;
-        off = hasBelow ? off + g.aS * below : off;
-        off = (i < nI - 1) ? off + g.aE * g.xE : off;
-        off = (lane < nJ - 1) ? off + g.aN * g.xN : off;
-        off = (i > 0) ? off + g.aW * xw : off;
+        off = off + g.aS * below;
+        off = off + g.aE * g.xE;
+        off = off + g.aN * g.xN;
+        off = off + g.aW * xw;
         // a row with a zero diagonal keeps its value (LinearSystem.cpp:86-88);
         // the quotient is formed on every lane so the choice is a select
         const double quotient = (g.b - off) / g.aP;
@@ -511,7 +513,7 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         const bool feeds = (w + 1 < W) && k >= kBegin;
         if (feeds && (k & 15) == 0 && ok) ok = waitStep(&posted[w + 1], k + 16 - kSweepHand, stall);
         *((feeds && l == 63) ? &hand[w * kSweepHand + (k & (kSweepHand - 1))] : &dummy[lane]) = xn;
-        up = waveShiftUp(xn);
+        up = waveShiftUp(xn, lowNext);
         xw = xn;
         // the refill may not be hoisted above the shift: the slot's old
         // values and this step's results are dead by here, so the loads land
