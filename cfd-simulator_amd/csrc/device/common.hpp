@@ -53,6 +53,10 @@ __device__ __forceinline__ void holdUntilHere(double v) { asm volatile("" : : "v
 // Compiler-only: v is computed on every lane at this point (the compiler may
 // not sink its computation into a branch of the select that uses it).
 __device__ __forceinline__ void computedHere(double v) { asm volatile("" : : "v"(v)); }
+// Compiler-only: the offset is an opaque 32-bit register value here, so the
+// accesses that use it stay (scalar base + 32-bit offset) instead of being
+// strength-reduced into 64-bit pointer increments.
+__device__ __forceinline__ void opaqueOffset(unsigned& o) { asm volatile("" : "+v"(o)); }
 template <int N>
 __device__ __forceinline__ void holdUntilHere(const double (&v)[N]) {
 #pragma unroll

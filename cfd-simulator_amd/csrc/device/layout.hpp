@@ -96,16 +96,30 @@ struct DevMonitor {
 
 constexpr int kMaxRedBlocks = 4096;  // partial-sum capacity per slot
 
-// Skewed layout of the lexicographic sweep. Every index a sweep forms is
-// (k * R + lane) or ((k + 1) * R + lane + 1) with the step k clamped to
-// [0, steps - 1] and lane < R, so it stays below skewLength.
+// Skewed layout of the lexicographic sweep: cell (i, j) at row i + j (its
+// wavefront step) plus kSkewFront, column j, of rows R wide. The sweep reads
+// and writes row k + kSkewFront on step k for every k it visits, from D steps
+// before its first (the read-ahead fill) to 2 D + 1 past the last, without
+// clamping: kSkewFront and kSkewTail rows of padding cover them. Slots of no
+// cell (i outside [0, nI), or j >= nJ) are zero when allocated and stay zero:
+// their matrix row is zero, so a sweep stores back the old value. Lanes of a
+// block wider than R (a batch member narrower than the widest) store into
+// kSkewPark parking slots past the last row, which nothing reads.
+constexpr int kSkewFront = 16;    // >= the sweeps' read-ahead depth D
+constexpr int kSkewTail = 34;     // >= 2 D + 2
+constexpr int kSkewPark = 1024;   // >= the widest block's lanes
 // Lanes of the sweep for nJ rows: one per row, rounded up to whole waves.
 __host__ __device__ constexpr int gsLanes(int nJ) {
   return nJ <= 64 ? 64 : (nJ >= 1024 ? 1024 : ((nJ + 63) / 64) * 64);
 }
 __host__ __device__ constexpr int skewSteps(int nI, int nJ) { return nI + nJ - 1; }
-__host__ __device__ constexpr int skewIndex(int i, int j, int R) { return (i + j) * R + j; }
-__host__ __device__ constexpr int skewLength(int nI, int nJ, int R) { return (skewSteps(nI, nJ) + 1) * R + R; }
+__host__ __device__ constexpr int skewIndex(int i, int j, int R) { return (i + j + kSkewFront) * R + j; }
+__host__ __device__ constexpr int skewParkIndex(int nI, int nJ, int R, int lane) {
+  return (kSkewFront + skewSteps(nI, nJ) + kSkewTail) * R + (lane - R);
+}
+__host__ __device__ constexpr int skewLength(int nI, int nJ, int R) {
+  return (kSkewFront + skewSteps(nI, nJ) + kSkewTail) * R + kSkewPark;
+}
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)

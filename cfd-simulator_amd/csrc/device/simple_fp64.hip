@@ -378,34 +378,53 @@ struct GsCell1 {
   double aP, aS, aN, aW, aE, b, xC, xN, xE;
 };
 
-// Every value is loaded at its own width, none with an unused half: a load
-// whose result is partly unused leaves registers the compiler reuses at once,
-// and reusing a register a load is still writing forces a wait for every load
-// in flight. Steps are clamped into [0, steps - 1], so every lane issues the
-// same loads on every step; values of cells off the wavefront are never used.
+// A wave-uniform base plus a 32-bit byte offset per lane: the form of a
+// global access with the base in scalar registers and one offset register.
+template <class T>
+__device__ __forceinline__ T ldOff(const CFD_GLOBAL char* base, unsigned off) {
+  return *reinterpret_cast<const CFD_GLOBAL T*>(base + off);
+}
+template <class T>
+__device__ __forceinline__ void stOff(CFD_GLOBAL char* base, unsigned off, T v) {
+  *reinterpret_cast<CFD_GLOBAL T*>(base + off) = v;
+}
+
+// One sweep's arrays as byte bases; the unknowns' bases already point at
+// this component's half of each pair, so a slot's byte offset (16 bytes a
+// slot) addresses the matrix row and the unknown alike.
+struct GsBytes {
+  const CFD_GLOBAL char *row0, *row1, *row2, *b;
+  const CFD_GLOBAL char* xOld;
+  CFD_GLOBAL char* xNew;
+};
+
+// Fetch the lane's cell from slot byte offset o, with oE = o + 16 R the same
+// lane one row on: the matrix row, the right-hand side, and the old values of
+// the cell, the cell to the right (oE) and the cell above (oE + 16). Every
+// value is loaded at its own width, none with an unused half: a load whose
+// result is partly unused leaves registers the compiler reuses at once, and
+// reusing a register a load is still writing forces a wait for every load in
+// flight. Every lane loads on every step (the padding rows keep the offsets
+// in bounds); values of cells off the wavefront are never used.
 template <bool SepB>
-__device__ __forceinline__ void gsFetch1(const GsOne& S, const CFD_GLOBAL double* xOld, int R, int steps, int lane,
-                                         int k, GsCell1& g) {
-  const int at = min(k, steps - 1) * R + lane;
-  const int atE = min(k + 1, steps - 1) * R + lane;
-  const int atN = atE + 1;
-  const dpair r0 = S.row[0][at];
-  const dpair r1 = S.row[1][at];
+__device__ __forceinline__ void gsFetch1(const GsBytes& P, unsigned o, unsigned oE, GsCell1& g) {
+  const dpair r0 = ldOff<dpair>(P.row0, o);
+  const dpair r1 = ldOff<dpair>(P.row1, o);
   g.aP = r0[0];
   g.aS = r0[1];
   g.aN = r1[0];
   g.aW = r1[1];
   if constexpr (SepB) {
-    g.aE = reinterpret_cast<const CFD_GLOBAL double*>(S.row[2])[2 * at];
-    g.b = S.b[at];
+    g.aE = ldOff<double>(P.row2, o);
+    g.b = ldOff<double>(P.b, o >> 1);
   } else {
-    const dpair r2 = S.row[2][at];
+    const dpair r2 = ldOff<dpair>(P.row2, o);
     g.aE = r2[0];
     g.b = r2[1];
   }
-  g.xC = xOld[2 * at + S.half];
-  g.xE = xOld[2 * atE + S.half];
-  g.xN = xOld[2 * atN + S.half];
+  g.xC = ldOff<double>(P.xOld, o);
+  g.xE = ldOff<double>(P.xOld, oE);
+  g.xN = ldOff<double>(P.xOld, oE + 16u);
 }
 
 template <int D, bool SepB>
@@ -437,10 +456,19 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
   const CFD_GLOBAL int* partnerSrc = gmem(partnerG);
   for (int t = lane; t < nI; t += B) partner[t] = partnerSrc[t];
   bool ok = true;
+  const unsigned R16 = static_cast<unsigned>(R) * 16u;
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     const CFD_GLOBAL double* xOld = (sweep & 1) ? S.x[1] : S.x[0];
     CFD_GLOBAL double* xNew = (sweep & 1) ? S.x[0] : S.x[1];
     for (int t = lane; t < nI; t += B) row0[t] = xOld[2 * skewIndex(t, 0, R) + S.half];
+    const GsBytes P{reinterpret_cast<const CFD_GLOBAL char*>(S.row[0]), reinterpret_cast<const CFD_GLOBAL char*>(S.row[1]),
+                    reinterpret_cast<const CFD_GLOBAL char*>(S.row[2]), reinterpret_cast<const CFD_GLOBAL char*>(S.b),
+                    reinterpret_cast<const CFD_GLOBAL char*>(xOld + S.half), reinterpret_cast<CFD_GLOBAL char*>(xNew + S.half)};
+    // byte offsets: the slot fetched (step k + D) and the slot stored (step
+    // k), one row on per step; a lane past R parks its stores
+    unsigned oF = static_cast<unsigned>((kFirst + D + kSkewFront) * R + laneG) * 16u;
+    unsigned oS = static_cast<unsigned>(lane < R ? (kFirst + kSkewFront) * R + lane : skewParkIndex(nI, nJ, R, lane)) * 16u;
+    const unsigned incS = lane < R ? R16 : 0u;
     if (l == 0) ldsPost(&posted[w], kBegin - 1);
     __syncthreads();
     double xw = 0.0, up = 0.0;
@@ -462,7 +490,7 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
 #pragma unroll
       for (int q = 0; q < D; ++q) {
         // no per-step condition: steps past kEnd touch no cell (they store into
-        // the padding row), so the body stays branch-free for the read-ahead
+        // slots of no cell), so the body stays branch-free for the read-ahead
         const int k = kFirst + r * D + q;
         const int i = k - lane;
         const bool on = active && i >= 0 && i < nI;
@@ -503,7 +531,11 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         const double quotient = (g.b - off) / g.aP;
         computedHere(quotient);
         const double xn = (fabs(g.aP) > 0.0) ? quotient : g.xC;
-        xNew[2 * (on ? k * R + lane : (steps + 1) * R + laneG) + S.half] = xn;
+        // every lane stores on every step: off the wavefront into a slot of
+        // no cell (the old value, zero, goes back) or a parking slot
+        stOff<double>(P.xNew, oS, xn);
+        oS += incS;
+        opaqueOffset(oS);
         *((on && lane == 0) ? &row0[max(i, 0)] : &dummy[lane]) = xn;
         // the value read ahead is this cell's old one when the next cell's
         // partner is this cell (the write above comes after that read)
@@ -520,7 +552,9 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
         // in the slot's own registers (no copies of loads in flight, which
         // would force a full wait at the loop latch)
         holdUntilHere(up);
-        gsFetch1<SepB>(S, xOld, R, steps, laneG, k + D, ring[q]);
+        gsFetch1<SepB>(P, oF, oF + R16, ring[q]);
+        oF += R16;
+        opaqueOffset(oF);
       }
     }
     ldsOrder();  // the last step's hand-off value is written
