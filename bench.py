@@ -49,7 +49,7 @@ ITERATIONS_TO_CONVERGE_MIXED = 12057
 # HBM bytes per launch of the dominant kernels from rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE, scaled by the same counters on a copy of known size
 # (profiles/naca2412_1024x256_sst_traffic.md); None until measured for this build.
-TRAFFIC_BYTES = {"k_gs_wave_mom": 140.38e6, "k_gs_wave_turb": 133.66e6, "k_cg_a + k_cg_b": 32.19e6}
+TRAFFIC_BYTES = {"k_gs_wave_mom": 142.32e6, "k_gs_wave_turb": 135.59e6, "k_cg_a + k_cg_b": 32.08e6}
 
 WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
                 first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
