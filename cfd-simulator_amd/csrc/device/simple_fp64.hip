@@ -343,8 +343,10 @@ struct GsSkew {
 // flags[fSweepStall], so no wave can hang the device. The step has no
 // divergent branch: writes that only one lane makes go, on the other lanes,
 // to a per-lane dummy slot. Off-diagonal terms are summed in the reference's
-// cellFaces order (below, right, above, left; Mesh.cpp:206-213), a missing
-// neighbour's term skipped, not added as zero.
+// cellFaces order (below, right, above, left; Mesh.cpp:206-213). A missing
+// neighbour's coefficient is zero in the assembled row, so its term adds a
+// +-0 product to a sum that is never -0, which leaves the sum bit for bit as
+// the reference's skip does.
 constexpr int kSweepHand = 64;  // a power of two
 constexpr int kSpinLimit = 1 << 22;
 
@@ -361,6 +363,10 @@ __device__ __forceinline__ bool waitStep(const int* counter, int target, int* st
     }
     spinPause();
   }
+  // The poll is a relaxed atomic load; the slot it guards is read with a plain
+  // load. This fence (no instruction) keeps the compiler from moving that read
+  // above the poll.
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   return true;
 }
 
@@ -512,6 +518,7 @@ __device__ void gsLexWave(int nI, int nJ, int R, const int* partnerG, const GsOn
           if (ok && i1 >= 0 && i1 < nI && seenBelow < k) {
             ok = waitStep(&posted[w - 1], k, stall);
             seenBelow = __builtin_amdgcn_readfirstlane(ldsPoll(&posted[w - 1]));
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
           }
           lowNext = hand[(w - 1) * kSweepHand + (k & (kSweepHand - 1))];
         }

@@ -970,6 +970,13 @@ int Engine::debugArray(const std::string& name, double* out, int capacity) const
 
 namespace cfdgpu {
 
+// Members of a batch share everything that shapes the launch schedule.
+static bool sameSchedule(const Inst& b, const Inst& a) {
+  return b.model == a.model && b.momSweeps == a.momSweeps && b.cgMax == a.cgMax &&
+         b.nonOrthPasses == a.nonOrthPasses && b.gsOrder == a.gsOrder && b.precision == a.precision &&
+         b.hasPRef == a.hasPRef;
+}
+
 Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
   if (members_.empty()) throw std::invalid_argument("a batch needs at least one solver");
   if (members_.size() > 65535) throw std::invalid_argument("too many solvers for one batch");
@@ -979,9 +986,7 @@ Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
   for (Engine* e : members_) {
     if (!e || !e->ready()) throw std::invalid_argument("every solver of a batch must be initialised");
     const Inst& b = e->refreshInst();
-    if (b.model != a.model || b.momSweeps != a.momSweeps || b.cgMax != a.cgMax ||
-        b.nonOrthPasses != a.nonOrthPasses || b.gsOrder != a.gsOrder || b.precision != a.precision ||
-        b.hasPRef != a.hasPRef)
+    if (!sameSchedule(b, a))
       throw std::invalid_argument("batched solvers must share closure, sweeps, CG cap, correctors, ordering and "
                                   "pressure anchoring");
     lanes = std::max(lanes, b.gsR);
@@ -1009,6 +1014,11 @@ void Batch::gather() {
   for (size_t q = 0; q < members_.size(); ++q) {
     check(hipStreamSynchronize(members_[q]->stream()), "member sync");
     host_[q] = members_[q]->refreshInst();
+    // every launch takes its schedule from the first member: a member whose
+    // settings changed since the batch was made must not run under it
+    if (!sameSchedule(host_[q], host_.front()))
+      throw std::invalid_argument("a batched solver's closure, sweeps, CG cap, correctors, ordering, precision, "
+                                  "pressure solver or anchoring changed after the batch was made");
   }
   check(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),
         "upload batch descriptors");
