@@ -96,6 +96,23 @@ def precision_of(G, args):
     return G.MIXED if getattr(args, "precision", "fp64") == "mixed" else G.FP64
 
 
+def path_settings(G, args, **kw):
+    """SimpleSettings of the run: the reference defaults with the workload's
+    inflow turbulence, and the path options of the command line."""
+    st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=precision_of(G, args), **kw)
+    if getattr(args, "fast", False):
+        st.precision, st.gs_ordering, st.pressure_solver, st.pressure_iterations = G.MIXED, 1, 1, 20
+    if getattr(args, "gs", None) is not None:
+        st.gs_ordering = {"lex": 0, "rb": 1}[args.gs]
+    if getattr(args, "psolver", None) is not None:
+        st.pressure_solver = {"jacobi": 0, "line": 1}[args.psolver]
+    if getattr(args, "cg_max", None) is not None:
+        st.pressure_iterations = args.cg_max
+    if getattr(args, "refine", None) is not None:
+        st.refinement_passes = args.refine
+    return st
+
+
 def gpu_solver(G, mesh, kind, vel, pres, field, precision=0):
     w = WORKLOAD
     st = G.default_settings(intensity=w["intensity"], viscosity_ratio=w["viscosity_ratio"], precision=precision)
@@ -360,7 +377,7 @@ def run_serve(args, mixed_batch=False):
     import workloads as W
 
     rows = W.load(W.MIXED_BATCH_FILE if mixed_batch else W.SERVING_FILE)[: args.requests]
-    st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=precision_of(G, args))
+    st = path_settings(G, args)
     srv = G.Server(G.server_options(hbm_budget_bytes=200e9, max_slots=args.slots, chunk_iterations=args.serve_chunk,
                                     tolerance=args.tolerance, max_iterations=args.max_iterations, settings=st,
                                     warm_start=0 if args.cold else 1))
@@ -502,6 +519,12 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="fp64: the plain path; mixed: FP32 storage in the pressure CG")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-path comparison of the default run")
+    ap.add_argument("--fast", action="store_true",
+                    help="the fast path: mixed precision, two-colour FP32 sweeps, line-preconditioned CG of 20")
+    ap.add_argument("--gs", choices=("lex", "rb"), help="Gauss-Seidel ordering")
+    ap.add_argument("--psolver", choices=("jacobi", "line"), help="pressure solver")
+    ap.add_argument("--cg-max", type=int, help="pressure CG iteration cap")
+    ap.add_argument("--refine", type=int, help="pressure refinement passes")
     ap.add_argument("--results-file", help="with --serve / --mixed-batch: write every request's result here")
     ap.add_argument("--compare-to", help="with --serve / --mixed-batch: a --results-file to compare coefficients with")
     args = ap.parse_args()

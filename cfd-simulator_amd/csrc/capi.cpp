@@ -46,9 +46,10 @@ int checkPathOptions(const cfdgpu_settings& s) {
     return fail(CFDGPU_INVALID_ARGUMENT, "unknown precision");
   if (s.gs_ordering != CFDGPU_GS_LEXICOGRAPHIC && s.gs_ordering != CFDGPU_GS_REDBLACK)
     return fail(CFDGPU_INVALID_ARGUMENT, "unknown Gauss-Seidel ordering");
-  if (s.pressure_solver != CFDGPU_PSOLVE_JACOBI_CG)
-    return fail(CFDGPU_NOT_IMPLEMENTED, "only the Jacobi-PCG pressure solver is implemented");
-  if (s.refinement_passes != 1) return fail(CFDGPU_NOT_IMPLEMENTED, "refinement_passes is reserved and must be 1");
+  if (s.pressure_solver != CFDGPU_PSOLVE_JACOBI_CG && s.pressure_solver != CFDGPU_PSOLVE_LINE_CG)
+    return fail(CFDGPU_INVALID_ARGUMENT, "unknown pressure solver");
+  if (s.refinement_passes < 1 || s.refinement_passes > 8)
+    return fail(CFDGPU_INVALID_ARGUMENT, "refinement_passes must be 1..8");
   return CFDGPU_OK;
 }
 

@@ -1,0 +1,406 @@
+// fast.hip - kernels of the fast path (no reference counterpart; selected by
+// cfdgpu_settings, judged against the FP64 path and the oracle):
+//
+//   * the pressure correction by CG preconditioned with the exact inverse of
+//     each wall-normal column's tridiagonal part (line Jacobi), in the storage
+//     and arithmetic type T (double, or float on the mixed path), with FP64
+//     inner products and norms. The reference's Jacobi preconditioner
+//     (LinearSystem.cpp:151-159) leaves the wall-normal coupling, which is up
+//     to 1e8 times the streamwise one in the near-wall cells, to the Krylov
+//     iteration; the line factor removes it. The CG's stopping test, cap and
+//     breakdown guards are the reference's (LinearSystem.cpp:118-210).
+//   * FP64-residual iterative refinement of the pressure correction: each
+//     further pass solves A d = b - A x with the residual formed in FP64 from
+//     the FP64 matrix, and adds d to x in FP64.
+//   * the mixed path's two-colour Gauss-Seidel sweeps on FP32 defect systems:
+//     the sweeps solve A delta = b - A phi_old for the correction delta in
+//     FP32 (the defect and the coefficients rounded once from their FP64
+//     assembly), and phi += delta is applied in FP64. In exact arithmetic the
+//     iterates are those of the sweeps on phi itself; in FP32 the rounding
+//     error scales with delta, which vanishes at convergence, so the fixed
+//     point is the FP64 one.
+//
+// Column-parallel launches give one lane per wall-normal column i (cells
+// c = j * nI + i, j = 0 .. nJ - 1), so a wave's accesses at one j are 64
+// consecutive cells. The recurrence of the column solve runs along j in each
+// lane.
+#include "common.hpp"
+
+namespace cfdgpu {
+
+template <typename T>
+struct LineStore {
+  const T *pP, *pN, *pE, *pWk;
+  T *inv, *r, *z, *pd0, *pd1, *Ap;
+};
+
+template <typename T>
+__device__ __forceinline__ LineStore<T> lineStore(const Inst& I);
+template <>
+__device__ __forceinline__ LineStore<double> lineStore<double>(const Inst& I) {
+  return LineStore<double>{I.pP, I.pN, I.pE, I.pWk, I.lInv, I.r, I.z, I.pd0, I.pd1, I.Ap};
+}
+template <>
+__device__ __forceinline__ LineStore<float> lineStore<float>(const Inst& I) {
+  return LineStore<float>{I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp};
+}
+
+__device__ __forceinline__ double waveSum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// One lane per column; the block is one wave.
+#define LCG_COLUMN_PROLOGUE                                                                  \
+  const Inst& I = insts[blockIdx.y];                                                         \
+  if (static_cast<int>(blockIdx.x) >= I.colBlocks) return;                                   \
+  if (I.flags[fStopped]) return;                                                             \
+  const int i = static_cast<int>(blockIdx.x) * kColThreads + static_cast<int>(threadIdx.x); \
+  const bool col = i < I.nI;                                                                 \
+  const int nI = I.nI;                                                                       \
+  const int nJ = I.nJ;
+
+#define FAST_CELL_PROLOGUE                                            \
+  const Inst& I = insts[blockIdx.y];                                  \
+  if (static_cast<int>(blockIdx.x) >= I.nBlocks) return;             \
+  if (I.flags[fStopped]) return;                                      \
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.nBlocks); \
+  const int c = tile * kBlock + static_cast<int>(threadIdx.x);        \
+  const bool live = c < I.N;                                          \
+  const int nI = I.nI;                                                \
+  const int nJ = I.nJ;                                                \
+  const int i = live ? c % nI : 0;                                    \
+  const int j = live ? c / nI : 0;
+
+// ---------------------------------------------------------------------------
+// Line-preconditioned CG
+// ---------------------------------------------------------------------------
+
+// Start of pass `pass`: r = b (pass 0: the assembled source, its mean removed
+// when nothing anchors the pressure level, as the Jacobi solver does; later
+// passes: the FP64 residual of the correction so far), x = 0, the factor of
+// every column's tridiagonal part (pivot inverses), z = M^-1 r, and the
+// partials of |r|_1 and r.z. The column tridiagonal of cell (i, j) has the
+// diagonal aP and the wall-normal couplings pN of the faces below and above
+// (the pressure matrix is symmetric); the wake cut and the streamwise
+// couplings are left to the Krylov iteration.
+template <typename T>
+__device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
+  LCG_COLUMN_PROLOGUE
+  __shared__ double bc[1];
+  const LineStore<T> S = lineStore<T>(I);
+  const bool demean = pass == 0 && !I.hasPRef;
+  if (demean) {
+    const double tot = waveSumPartials(partialSlot(I, sSourceSum), I.nBlocks);
+    if (threadIdx.x == 0) bc[0] = tot;
+  }
+  __syncthreads();
+  const double mean = demean ? bc[0] / static_cast<double>(I.N) : 0.0;
+  const double* src = (pass == 0) ? I.pb : I.rb;
+  double l1 = 0.0, rz = 0.0;
+  if (col) {
+    T cPrev = T(0), yPrev = T(0);
+    for (int j = 0; j < nJ; ++j) {
+      const int c = j * nI + i;
+      double b = src[c];
+      if (demean) {
+        b -= mean;
+        I.pb[c] = b;
+      }
+      const T r = static_cast<T>(b);
+      const T lo = (j > 0) ? S.pN[c - nI] : T(0);
+      const T den = S.pP[c] - lo * cPrev;
+      const T inv = (fabs(static_cast<double>(den)) > 0.0) ? T(1) / den : T(1);
+      const T y = (r - lo * yPrev) * inv;
+      S.inv[c] = inv;
+      S.r[c] = r;
+      S.z[c] = y;
+      I.x[c] = 0.0;
+      l1 += fabs(static_cast<double>(r));
+      cPrev = S.pN[c] * inv;  // zero on the top row
+      yPrev = y;
+    }
+    T zNext = T(0);
+    for (int j = nJ - 1; j >= 0; --j) {
+      const int c = j * nI + i;
+      const T z = S.z[c] - (S.pN[c] * S.inv[c]) * zNext;
+      S.z[c] = z;
+      rz += static_cast<double>(S.r[c]) * static_cast<double>(z);
+      zNext = z;
+    }
+  }
+  l1 = waveSum(l1);
+  rz = waveSum(rz);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sCgNorm)[blockIdx.x] = l1;
+    partialSlot(I, sCgRz)[blockIdx.x] = rz;
+    if (blockIdx.x == 0) {
+      I.flags[fCgDone] = 0;
+      I.flags[fCgIters] = 0;
+    }
+  }
+}
+
+// Iteration `it`, launch A (cell-parallel): the test of iteration it - 1
+// from the column partials (|r|_1 <= tol |r0|_1, then the r.z guard), beta,
+// p = z + beta p_old (neighbours recomputed locally), Ap and p.Ap.
+template <typename T>
+__device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
+  const Inst& I = insts[blockIdx.y];
+  if (static_cast<int>(blockIdx.x) >= I.nBlocks) return;
+  if (I.flags[fStopped] || I.flags[fCgDone]) return;
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.nBlocks);
+  const int c = tile * kBlock + static_cast<int>(threadIdx.x);
+  const bool live = c < I.N;
+  const int nI = I.nI, nJ = I.nJ;
+  const LineStore<T> S = lineStore<T>(I);
+  double sums[2];
+  {
+    const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
+    blockSumPartials<2>(slots, I.colBlocks, sm, sums);
+  }
+  const double nrm = sums[0], rzNew = sums[1];
+  double beta = 0.0;
+  bool done = false;
+  if (it == 0) {
+    if (!(nrm > 0.0)) done = true;  // already solved (LinearSystem.cpp:144-149)
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+      I.red[sCgNorm0] = nrm;
+      I.red[sCgRz0] = rzNew;
+    }
+  } else {
+    const double rzOld = I.red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
+    if (nrm <= I.cgTol * I.red[sCgNorm0]) done = true;
+    else if (!(fabs(rzOld) > 0.0)) done = true;
+    else beta = rzNew / rzOld;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) I.red[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
+  }
+  if (done) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    return;
+  }
+  const T betaT = static_cast<T>(beta);
+  const T* pOld = (it & 1) ? S.pd0 : S.pd1;
+  auto pAt = [&](int q) -> T { return (it == 0) ? S.z[q] : static_cast<T>(S.z[q] + betaT * pOld[q]); };
+  double part[1] = {0.0};
+  if (live) {
+    const int i = c % nI, j = c / nI;
+    const T pc = pAt(c);
+    // diagonal, then below (or the wake partner), above, left, right
+    T y = S.pP[c] * pc;
+    if (j > 0) {
+      y += S.pN[c - nI] * pAt(c - nI);
+    } else {
+      const int pr = I.partner[i];
+      if (pr >= 0) y += S.pWk[c] * pAt(pr);
+    }
+    if (j < nJ - 1) y += S.pN[c] * pAt(c + nI);
+    if (i > 0) y += S.pE[c - 1] * pAt(c - 1);
+    if (i < nI - 1) y += S.pE[c] * pAt(c + 1);
+    ((it & 1) ? S.pd1 : S.pd0)[c] = pc;
+    S.Ap[c] = y;
+    part[0] = static_cast<double>(pc) * static_cast<double>(y);
+  }
+  blockSum<1>(part, sm);
+  if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
+}
+
+// Iteration `it`, launch B (column-parallel): alpha = r.z / p.Ap, x += alpha p
+// (FP64), r -= alpha Ap, |r|_1, z = M^-1 r (forward and back substitution
+// along the column), r.z.
+template <typename T>
+__device__ __forceinline__ void lcgB(const Inst* insts, int it) {
+  LCG_COLUMN_PROLOGUE
+  __shared__ double bc[1];
+  if (I.flags[fCgDone]) return;
+  const LineStore<T> S = lineStore<T>(I);
+  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), I.nBlocks);
+  if (threadIdx.x == 0) bc[0] = papLane;
+  __syncthreads();
+  const double pap = bc[0];
+  if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
+    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    return;
+  }
+  const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
+  const T alphaT = static_cast<T>(alpha);
+  const T* p = (it & 1) ? S.pd1 : S.pd0;
+  double l1 = 0.0, rz = 0.0;
+  if (col) {
+    T yPrev = T(0);
+    for (int j = 0; j < nJ; ++j) {
+      const int c = j * nI + i;
+      const T pc = p[c];
+      I.x[c] += alpha * static_cast<double>(pc);
+      const T r = S.r[c] - alphaT * S.Ap[c];
+      const T lo = (j > 0) ? S.pN[c - nI] : T(0);
+      const T y = (r - lo * yPrev) * S.inv[c];
+      S.r[c] = r;
+      S.z[c] = y;
+      l1 += fabs(static_cast<double>(r));
+      yPrev = y;
+    }
+    T zNext = T(0);
+    for (int j = nJ - 1; j >= 0; --j) {
+      const int c = j * nI + i;
+      const T z = S.z[c] - (S.pN[c] * S.inv[c]) * zNext;
+      S.z[c] = z;
+      rz += static_cast<double>(S.r[c]) * static_cast<double>(z);
+      zNext = z;
+    }
+  }
+  l1 = waveSum(l1);
+  rz = waveSum(rz);
+  if (threadIdx.x == 0) {
+    partialSlot(I, sCgNorm)[blockIdx.x] = l1;
+    partialSlot(I, sCgRz)[blockIdx.x] = rz;
+    if (blockIdx.x == 0) {
+      I.flags[fCgIters] = it + 1;
+      I.flags[fCgTotal] += 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kColThreads) k_lcg_init(const Inst* insts, int pass) { lcgInit<double>(insts, pass); }
+__global__ void __launch_bounds__(kBlock) k_lcg_a(const Inst* insts, int it) {
+  __shared__ double sm[16];
+  lcgA<double>(insts, it, sm);
+}
+__global__ void __launch_bounds__(kColThreads) k_lcg_b(const Inst* insts, int it) { lcgB<double>(insts, it); }
+
+__global__ void __launch_bounds__(kColThreads) k_lcg_init_f32(const Inst* insts, int pass) {
+  lcgInit<float>(insts, pass);
+}
+__global__ void __launch_bounds__(kBlock) k_lcg_a_f32(const Inst* insts, int it) {
+  __shared__ double sm[16];
+  lcgA<float>(insts, it, sm);
+}
+__global__ void __launch_bounds__(kColThreads) k_lcg_b_f32(const Inst* insts, int it) { lcgB<float>(insts, it); }
+
+// ---------------------------------------------------------------------------
+// FP64-residual refinement of the pressure correction
+// ---------------------------------------------------------------------------
+
+// After pass `pass`: the accumulated correction takes the pass's solution.
+__global__ void __launch_bounds__(kBlock) k_refine_acc(const Inst* insts, int pass) {
+  FAST_CELL_PROLOGUE
+  (void)i; (void)j; (void)nI; (void)nJ;
+  if (live) I.xacc[c] = (pass == 0) ? I.x[c] : I.xacc[c] + I.x[c];
+}
+
+// The next pass's right-hand side b - A x in FP64 (the FP64 pressure matrix,
+// the accumulated correction), and its L1 norm's partials.
+__global__ void __launch_bounds__(kBlock) k_refine_res(const Inst* insts) {
+  FAST_CELL_PROLOGUE
+  __shared__ double sm[16];
+  double part[1] = {0.0};
+  if (live) {
+    const double* x = I.xacc;
+    double y = I.pP[c] * x[c];
+    if (j > 0) {
+      y += I.pN[c - nI] * x[c - nI];
+    } else {
+      const int pr = I.partner[i];
+      if (pr >= 0) y += I.pWk[c] * x[pr];
+    }
+    if (j < nJ - 1) y += I.pN[c] * x[c + nI];
+    if (i > 0) y += I.pE[c - 1] * x[c - 1];
+    if (i < nI - 1) y += I.pE[c] * x[c + 1];
+    const double r = I.pb[c] - y;
+    I.rb[c] = r;
+    part[0] = fabs(r);
+  }
+  blockSum<1>(part, sm);
+  if (threadIdx.x == 0) partialSlot(I, sRefineNorm)[blockIdx.x] = part[0];
+}
+
+// After the last pass: the correction the velocity and flux updates read.
+__global__ void __launch_bounds__(kBlock) k_refine_final(const Inst* insts) {
+  FAST_CELL_PROLOGUE
+  (void)i; (void)j; (void)nI; (void)nJ;
+  if (live) I.x[c] = I.xacc[c] + I.x[c];
+}
+
+// ---------------------------------------------------------------------------
+// Mixed path: two-colour Gauss-Seidel on FP32 defect systems
+// ---------------------------------------------------------------------------
+
+// One colour of one sweep. sys 0: u and v (one matrix); sys 1: k and omega.
+// `first`: the first colour of the first sweep, whose neighbours' corrections
+// are all zero; `last`: the second colour of the last sweep, which also adds
+// every cell's correction to its field (k and omega then floored at 0 and
+// 1e-10, KOmegaSST.cpp:468-471). Off-diagonal terms in cellFaces order
+// (below or the wake partner, right, above, left; Mesh.cpp:206-213); a row
+// with a zero diagonal is skipped (LinearSystem.cpp:88-90): its correction is
+// zero. The C-grid has an even nI, so a wake cell and its partner differ in
+// colour.
+template <int NS>
+__device__ __forceinline__ void rbDefect(const float* const (&A)[NS][5], const float* const (&R)[NS],
+                                         float* const (&D)[NS], int c, int i, int j, int nI, int nJ, int pr,
+                                         bool first, float (&out)[NS]) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const float d = A[s][0][c];
+    float delta = 0.0f;
+    if (fabsf(d) > 0.0f) {
+      float off = 0.0f;
+      if (!first) {
+        if (j > 0) off += A[s][1][c] * D[s][c - nI];
+        else if (pr >= 0) off += A[s][1][c] * D[s][pr];
+        if (i < nI - 1) off += A[s][4][c] * D[s][c + 1];
+        if (j < nJ - 1) off += A[s][2][c] * D[s][c + nI];
+        if (i > 0) off += A[s][3][c] * D[s][c - 1];
+      }
+      delta = (R[s][c] - off) / d;
+    }
+    out[s] = delta;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int color, int first, int last) {
+  FAST_CELL_PROLOGUE
+  if (!live) return;
+  const bool mine = ((i + j) & 1) == color;
+  const int pr = (j == 0) ? I.partner[i] : -1;
+  if (sys == 0) {
+    // u and v share the coefficients
+    const float* const A[2][5] = {{I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]},
+                                  {I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]}};
+    const float* const R[2] = {I.fMomRes[0], I.fMomRes[1]};
+    float* const D[2] = {I.fMomDelta[0], I.fMomDelta[1]};
+    float du[2];
+    if (mine) {
+      rbDefect<2>(A, R, D, c, i, j, nI, nJ, pr, first != 0, du);
+      D[0][c] = du[0];
+      D[1][c] = du[1];
+    } else if (last) {
+      du[0] = D[0][c];
+      du[1] = D[1][c];
+    }
+    if (last) {
+      I.u[c] += static_cast<double>(du[0]);
+      I.v[c] += static_cast<double>(du[1]);
+    }
+  } else {
+    const float* const A[2][5] = {{I.fTurb[0][0], I.fTurb[0][1], I.fTurb[0][2], I.fTurb[0][3], I.fTurb[0][4]},
+                                  {I.fTurb[1][0], I.fTurb[1][1], I.fTurb[1][2], I.fTurb[1][3], I.fTurb[1][4]}};
+    const float* const R[2] = {I.fTurbRes[0], I.fTurbRes[1]};
+    float* const D[2] = {I.fTurbDelta[0], I.fTurbDelta[1]};
+    float dt[2];
+    if (mine) {
+      rbDefect<2>(A, R, D, c, i, j, nI, nJ, pr, first != 0, dt);
+      D[0][c] = dt[0];
+      D[1][c] = dt[1];
+    } else if (last) {
+      dt[0] = D[0][c];
+      dt[1] = D[1][c];
+    }
+    if (last) {
+      I.k[c] = mx(I.k[c] + static_cast<double>(dt[0]), 0.0);
+      I.w[c] = mx(I.w[c] + static_cast<double>(dt[1]), 1.0e-10);
+    }
+  }
+}
+
+}  // namespace cfdgpu

@@ -20,7 +20,7 @@ __global__ void k_wall_distance(const double2*, int, const double2*, int, double
 __global__ void k_gs_color(const Inst*, int, int, int);
 __global__ void k_rhiechow(const Inst*);
 __global__ void k_pressure(const Inst*, int);
-__global__ void k_cg_init(const Inst*);
+__global__ void k_cg_init(const Inst*, int);
 __global__ void k_cg_a(const Inst*, int);
 __global__ void k_cg_b(const Inst*, int);
 __global__ void k_corr_mean(const Inst*);
@@ -32,7 +32,17 @@ __global__ void k_sst_blend(const Inst*);
 __global__ void k_sst_asm(const Inst*);
 __global__ void k_sst_wall(const Inst*);
 __global__ void k_sst_mut(const Inst*);
-__global__ void k_cg_init_f32(const Inst*);
+__global__ void k_cg_init_f32(const Inst*, int);
+__global__ void k_lcg_init(const Inst*, int);
+__global__ void k_lcg_a(const Inst*, int);
+__global__ void k_lcg_b(const Inst*, int);
+__global__ void k_lcg_init_f32(const Inst*, int);
+__global__ void k_lcg_a_f32(const Inst*, int);
+__global__ void k_lcg_b_f32(const Inst*, int);
+__global__ void k_refine_acc(const Inst*, int);
+__global__ void k_refine_res(const Inst*);
+__global__ void k_refine_final(const Inst*);
+__global__ void k_rb32(const Inst*, int, int, int, int);
 __global__ void k_cg_a_f32(const Inst*, int);
 __global__ void k_cg_b_f32(const Inst*, int);
 
@@ -52,8 +62,20 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   const dim3 block(kBlock);
   const dim3 one((cfg.batch + 63) / 64);
   const bool lex = h.gsOrder == 0;
-  // the mixed path differs only in the pressure solve's storage (FP32)
   const bool mixed = h.precision == 1;
+  // the mixed path's two-colour sweeps run on FP32 defect systems
+  const bool rb32 = mixed && !lex;
+  const bool line = h.pSolver == 1;
+  const int refine = h.refinePasses < 1 ? 1 : h.refinePasses;
+  const dim3 colGrid(static_cast<unsigned>(cfg.maxColBlocks), static_cast<unsigned>(cfg.batch));
+  // n sweeps of one system in two colours; the first colour of the first
+  // sweep reads no neighbour, the last colour of the last applies the result
+  auto rbSweeps = [&](int sys, int n) {
+    for (int sw = 0; sw < n; ++sw)
+      for (int color = 0; color < 2; ++color)
+        hipLaunchKernelGGL(k_rb32, grid, block, 0, s, d, sys, color, (sw == 0 && color == 0) ? 1 : 0,
+                           (sw == n - 1 && color == 1) ? 1 : 0);
+  };
   auto begin = [&](int stage) { if (rec) rec->begin(stage, s); };
   auto end = [&](int stage) { if (rec) rec->end(stage, s); };
   begin(CFDGPU_STAGE_GRADIENTS);
@@ -73,6 +95,9 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     begin(CFDGPU_STAGE_SWEEP_COPY);
     hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 0, h.momSweeps);
     end(CFDGPU_STAGE_SWEEP_COPY);
+  } else if (rb32) {
+    rbSweeps(0, h.momSweeps);
+    end(CFDGPU_STAGE_MOMENTUM_SWEEPS);
   } else {
     for (int sw = 0; sw < h.momSweeps; ++sw) {
       hipLaunchKernelGGL(k_gs_color, grid, block, 0, s, d, 0, 0, 0);
@@ -91,19 +116,42 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     end(CFDGPU_STAGE_PRESSURE_ASSEMBLY);
     begin(CFDGPU_STAGE_PRESSURE_SOLVE);
     const dim3 cgGrid(cfg.maxCgBlocks, cfg.batch);
-    if (mixed) {
-      hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d);
-      for (int it = 0; it < h.cgMax; ++it) {
-        hipLaunchKernelGGL(k_cg_a_f32, cgGrid, block, 0, s, d, it);
-        hipLaunchKernelGGL(k_cg_b_f32, cgGrid, block, 0, s, d, it);
+    const dim3 col(kColThreads);
+    for (int rp = 0; rp < refine; ++rp) {
+      if (line && mixed) {
+        hipLaunchKernelGGL(k_lcg_init_f32, colGrid, col, 0, s, d, rp);
+        for (int it = 0; it < h.cgMax; ++it) {
+          hipLaunchKernelGGL(k_lcg_a_f32, grid, block, 0, s, d, it);
+          hipLaunchKernelGGL(k_lcg_b_f32, colGrid, col, 0, s, d, it);
+        }
+      } else if (line) {
+        hipLaunchKernelGGL(k_lcg_init, colGrid, col, 0, s, d, rp);
+        for (int it = 0; it < h.cgMax; ++it) {
+          hipLaunchKernelGGL(k_lcg_a, grid, block, 0, s, d, it);
+          hipLaunchKernelGGL(k_lcg_b, colGrid, col, 0, s, d, it);
+        }
+      } else if (mixed) {
+        hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d, rp);
+        for (int it = 0; it < h.cgMax; ++it) {
+          hipLaunchKernelGGL(k_cg_a_f32, cgGrid, block, 0, s, d, it);
+          hipLaunchKernelGGL(k_cg_b_f32, cgGrid, block, 0, s, d, it);
+        }
+      } else {
+        hipLaunchKernelGGL(k_cg_init, cgGrid, block, 0, s, d, rp);
+        for (int it = 0; it < h.cgMax; ++it) {
+          hipLaunchKernelGGL(k_cg_a, cgGrid, block, 0, s, d, it);
+          hipLaunchKernelGGL(k_cg_b, cgGrid, block, 0, s, d, it);
+        }
       }
-    } else {
-      hipLaunchKernelGGL(k_cg_init, cgGrid, block, 0, s, d);
-      for (int it = 0; it < h.cgMax; ++it) {
-        hipLaunchKernelGGL(k_cg_a, cgGrid, block, 0, s, d, it);
-        hipLaunchKernelGGL(k_cg_b, cgGrid, block, 0, s, d, it);
+      if (rp + 1 < refine) {
+        // FP64-residual refinement: accumulate the correction, then the
+        // next pass's right-hand side b - A x in FP64
+        hipLaunchKernelGGL(k_refine_acc, grid, block, 0, s, d, rp);
+        hipLaunchKernelGGL(k_refine_res, grid, block, 0, s, d);
       }
     }
+    // the last pass's solution plus what the earlier passes accumulated
+    if (refine > 1) hipLaunchKernelGGL(k_refine_final, grid, block, 0, s, d);
     end(CFDGPU_STAGE_PRESSURE_SOLVE);
     begin(CFDGPU_STAGE_CORRECTION);
     if (!h.hasPRef) hipLaunchKernelGGL(k_corr_mean, grid, block, 0, s, d);
@@ -129,6 +177,9 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
       begin(CFDGPU_STAGE_SWEEP_COPY);
       hipLaunchKernelGGL(k_gs_unskew, grid, block, 0, s, d, 1, 3);
       end(CFDGPU_STAGE_SWEEP_COPY);
+    } else if (rb32) {
+      rbSweeps(1, 3);
+      end(CFDGPU_STAGE_SST_SWEEPS);
     } else {
       for (int sw = 0; sw < 3; ++sw) {
         const int floor = (sw == 2) ? 1 : 0;

@@ -123,6 +123,7 @@ __host__ __device__ constexpr int skewLength(int nI, int nJ, int R) {
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
+constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)
 
 struct Inst {
   int nI, nJ, N;
@@ -134,12 +135,15 @@ struct Inst {
   double cgTol;
   int nonOrthPasses;  // max(1, nonOrthogonalCorrectors)
   int gsOrder;        // 0 lexicographic (exact), 1 two-colour
-  int precision;      // 0 FP64, 1 mixed (FP32 storage in the pressure solve)
+  int precision;      // 0 FP64, 1 mixed (FP32 bulk, FP64 reductions and residuals)
+  int pSolver;        // 0 Jacobi-PCG (the reference's), 1 wall-normal line-preconditioned CG
+  int refinePasses;   // pressure solves per corrector pass (> 1: FP64-residual refinement)
   double refMassFlow;
   double stopTol;  // device-side stop rule (worst() < stopTol); <= 0 disables
   int stopMaxIter;
   int nBlocks;     // blocks of a cell-parallel launch for this instance
   int cgBlocks;    // blocks of a CG launch (kBlock x kCgCells cells each)
+  int colBlocks;   // blocks of a column-parallel launch (kColThreads columns each)
 
   // geometry (read-only)
   const double* V;
@@ -198,6 +202,24 @@ struct Inst {
   // (the CG's arithmetic, its reductions and x stay FP64)
   float *fpP, *fpN, *fpE, *fpWk;
   float *fr, *fz, *fpd0, *fpd1, *fAp;
+
+  // Line-preconditioned CG: 1 / pivot of each column's tridiagonal factor
+  // (Thomas), in the CG's storage type; the refinement's FP64 residual
+  // right-hand side and accumulated correction.
+  double* lInv;
+  float* flInv;
+  double *rb, *xacc;
+
+  // Mixed path with two-colour sweeps: every swept system in FP32 as
+  // coefficients (aP, aS, aN, aW, aE), the FP64 defect b - A phi rounded to
+  // FP32, and the FP32 correction the sweeps solve for (phi += delta after
+  // the last sweep, in FP64). Momentum: u and v share the coefficients.
+  float* fMom[5];
+  float* fMomRes[2];
+  float* fMomDelta[2];
+  float* fTurb[2][5];     // [k, omega][coefficient]
+  float* fTurbRes[2];
+  float* fTurbDelta[2];
 
   // reductions
   double* red;        // kSlots scalars

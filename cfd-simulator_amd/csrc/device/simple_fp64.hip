@@ -254,14 +254,23 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
       row.by += (relaxed - d) * vc;
       d = relaxed;
     }
-    I.aP[c] = d;
-    I.aS[c] = aS;
-    I.aN[c] = aN;
-    I.aW[c] = aW;
-    I.aE[c] = aE;
-    I.bx[c] = row.bx;
-    I.by[c] = row.by;
-    if (I.gsOrder == 0) {
+    I.aP[c] = d;  // the momentum diagonal of Rhie-Chow and the correction
+    if (I.gsOrder == 1 && I.precision == 0) {
+      // the FP64 two-colour sweep's rows
+      I.aS[c] = aS;
+      I.aN[c] = aN;
+      I.aW[c] = aW;
+      I.aE[c] = aE;
+      I.bx[c] = row.bx;
+      I.by[c] = row.by;
+    } else if (I.gsOrder == 1) {
+      // the mixed path's FP32 rows (the defects follow below)
+      I.fMom[0][c] = static_cast<float>(d);
+      I.fMom[1][c] = static_cast<float>(aS);
+      I.fMom[2][c] = static_cast<float>(aN);
+      I.fMom[3][c] = static_cast<float>(aW);
+      I.fMom[4][c] = static_cast<float>(aE);
+    } else {
       // the lexicographic sweep's copy, anti-diagonal layout, in pairs
       const int sk = skewIndex(i, j, I.gsR);
       I.skMomRow[0][sk] = make_double2(d, aS);
@@ -280,6 +289,10 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     part[0] = fabs(row.bx - axu);
     part[1] = fabs(row.by - axv);
     part[2] = fabs(d);
+    if (I.gsOrder == 1 && I.precision == 1) {
+      I.fMomRes[0][c] = static_cast<float>(row.bx - axu);
+      I.fMomRes[1][c] = static_cast<float>(row.by - axv);
+    }
   }
   blockSum<3>(part, sm);
   if (threadIdx.x == 0) {
@@ -934,7 +947,7 @@ __device__ __forceinline__ double asStored(double v) {
 // the next launch sums that many fewer partials. The per-thread sums run over
 // q in order before the block sum: the result is fixed run to run.
 template <typename T>
-__device__ __forceinline__ void cgInit(const Inst* insts, double* sm) {
+__device__ __forceinline__ void cgInit(const Inst* insts, int pass, double* sm) {
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
   const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
@@ -942,14 +955,18 @@ __device__ __forceinline__ void cgInit(const Inst* insts, double* sm) {
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
   const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
   const CgStore<T> S = cgStore<T>(I);
+  // pass 0 solves for the assembled source (its mean removed when nothing
+  // anchors the pressure level); a refinement pass for the FP64 residual
+  const bool demean = pass == 0 && !I.hasPRef;
+  const double* src = (pass == 0) ? I.pb : I.rb;
   double mean = 0.0;
-  if (!I.hasPRef) mean = blockSumPartials(partialSlot(I, sSourceSum), I.nBlocks, sm) / static_cast<double>(I.N);
+  if (demean) mean = blockSumPartials(partialSlot(I, sSourceSum), I.nBlocks, sm) / static_cast<double>(I.N);
   double part[2] = {0.0, 0.0};
   double bq[kCgCells], dq[kCgCells];
 #pragma unroll
   for (int q = 0; q < kCgCells; ++q) {
     const int cc = min(base + q * kBlock, I.N - 1);
-    bq[q] = gmem(static_cast<const double*>(I.pb))[cc];
+    bq[q] = gmem(src)[cc];
     dq[q] = static_cast<double>(gmem(S.pP)[cc]);
   }
 #pragma unroll
@@ -957,7 +974,7 @@ __device__ __forceinline__ void cgInit(const Inst* insts, double* sm) {
     const int c = base + q * kBlock;
     if (c < I.N) {
       double b = bq[q];
-      if (!I.hasPRef) {
+      if (demean) {
         b -= mean;
         gmem(I.pb)[c] = b;
       }
@@ -1178,9 +1195,9 @@ __device__ __forceinline__ void cgB(const Inst* insts, int it, double* sm) {
 }
 
 // FP64 storage: the plain path
-__global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts) {
+__global__ void __launch_bounds__(kBlock) k_cg_init(const Inst* insts, int pass) {
   __shared__ double sm[16];
-  cgInit<double>(insts, sm);
+  cgInit<double>(insts, pass, sm);
 }
 // At most 128 VGPRs (4 waves per SIMD; 150 unconstrained): measured faster in
 // batches, the same for one solve.
@@ -1194,9 +1211,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_b(const Inst* insts, int it) {
 }
 
 // FP32 storage: the mixed path
-__global__ void __launch_bounds__(kBlock) k_cg_init_f32(const Inst* insts) {
+__global__ void __launch_bounds__(kBlock) k_cg_init_f32(const Inst* insts, int pass) {
   __shared__ double sm[16];
-  cgInit<float>(insts, sm);
+  cgInit<float>(insts, pass, sm);
 }
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_cg_a_f32(const Inst* insts, int it) {
   __shared__ double sm[16];

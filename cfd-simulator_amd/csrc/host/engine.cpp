@@ -248,8 +248,17 @@ void carveInst(Carver& cv, Inst& I, const Grid& grid) {
     for (double2** a : vecArrays) *a = cv.take<double2>(N);
     I.Fj = cv.take<double>(nJf);
     I.Fi = cv.take<double>(nIf);
-    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp};
+    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.flInv};
     for (float** a : fArrays) *a = cv.take<float>(N);
+    double** lineArrays[] = {&I.lInv, &I.rb, &I.xacc};
+    for (double** a : lineArrays) *a = cv.take<double>(N);
+    for (float*& a : I.fMom) a = cv.take<float>(N);
+    for (float*& a : I.fMomRes) a = cv.take<float>(N);
+    for (float*& a : I.fMomDelta) a = cv.take<float>(N);
+    for (auto& sys : I.fTurb)
+      for (float*& a : sys) a = cv.take<float>(N);
+    for (float*& a : I.fTurbRes) a = cv.take<float>(N);
+    for (float*& a : I.fTurbDelta) a = cv.take<float>(N);
     const size_t skew = static_cast<size_t>(skewLength(grid.nI, grid.nJ, gsThreadsFor(grid.nJ)));
     for (double2*& a : I.skMomRow) a = cv.take<double2>(skew);
     I.skMomBy = cv.take<double>(skew);
@@ -292,11 +301,13 @@ void Engine::allocate() {
   h_.N = grid_.cells();
   h_.nBlocks = (h_.N + kBlock - 1) / kBlock;
   h_.cgBlocks = (h_.N + kBlock * kCgCells - 1) / (kBlock * kCgCells);
+  h_.colBlocks = (h_.nI + kColThreads - 1) / kColThreads;
   h_.gsR = gsThreadsFor(grid_.nJ);
   h_.gsSkewLen = skewLength(grid_.nI, grid_.nJ, h_.gsR);
   check(hipMalloc(&dInst_, sizeof(Inst)), "hipMalloc (descriptor)");
   cfg_.maxBlocks = h_.nBlocks;
   cfg_.maxCgBlocks = h_.cgBlocks;
+  cfg_.maxColBlocks = h_.colBlocks;
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
   cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
@@ -335,6 +346,8 @@ void Engine::fillInstScalars() {
   h_.nonOrthPasses = std::max(1, settings_.non_orthogonal_correctors);
   h_.gsOrder = settings_.gs_ordering;
   h_.precision = settings_.precision;
+  h_.pSolver = settings_.pressure_solver;
+  h_.refinePasses = std::max(1, settings_.refinement_passes);
   h_.refMassFlow = refMassFlow_;
   h_.kIn = kIn_;
   h_.wIn = wIn_;
@@ -643,15 +656,36 @@ bool graphsEnabled() {
 }  // namespace
 
 bool IterationGraph::Key::operator==(const Key& o) const {
-  return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && batch == o.batch &&
-         gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder && precision == o.precision &&
+  return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && maxColBlocks == o.maxColBlocks &&
+         batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
+         precision == o.precision && pSolver == o.pSolver && refinePasses == o.refinePasses &&
          momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgMax == o.cgMax && hasPRef == o.hasPRef &&
          model == o.model;
 }
 
 IterationGraph::Key IterationGraph::keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h) {
-  return Key{d, cfg.maxBlocks, cfg.maxCgBlocks, cfg.batch, cfg.gsThreads, cfg.gsLds, h.gsOrder, h.precision,
-             h.momSweeps, h.nonOrthPasses, h.cgMax, h.hasPRef, h.model};
+  return Key{d,           cfg.maxBlocks,   cfg.maxCgBlocks, cfg.maxColBlocks, cfg.batch, cfg.gsThreads,
+             cfg.gsLds,   h.gsOrder,       h.precision,     h.pSolver,        h.refinePasses, h.momSweeps,
+             h.nonOrthPasses, h.cgMax, h.hasPRef, h.model};
+}
+
+LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
+  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, maxNI = 0;
+  for (size_t q = 0; q < count; ++q) {
+    const Inst& b = insts[q];
+    lanes = std::max(lanes, b.gsR);
+    blocks = std::max(blocks, b.nBlocks);
+    cgBlocks = std::max(cgBlocks, b.cgBlocks);
+    colBlocks = std::max(colBlocks, b.colBlocks);
+    maxNI = std::max(maxNI, b.nI);
+  }
+  return LaunchCfg{blocks, cgBlocks, colBlocks, static_cast<int>(count), lanes, gsLdsFor(0, maxNI, lanes)};
+}
+
+bool sameSchedule(const Inst& b, const Inst& a) {
+  return b.model == a.model && b.momSweeps == a.momSweeps && b.cgMax == a.cgMax &&
+         b.nonOrthPasses == a.nonOrthPasses && b.gsOrder == a.gsOrder && b.precision == a.precision &&
+         b.pSolver == a.pSolver && b.refinePasses == a.refinePasses && b.hasPRef == a.hasPRef;
 }
 
 void IterationGraph::reset() {
@@ -933,6 +967,41 @@ int Engine::debugArray(const std::string& name, double* out, int capacity) const
   const double* src = nullptr;
   size_t count = N;
   const double2* vec = nullptr;
+  // The momentum rows other than the diagonal live where the ordering's sweep
+  // reads them: the skewed pairs (lexicographic), the FP32 rows (mixed path,
+  // two colours) or the FP64 rows (two colours).
+  static const char* const momNames[] = {"momS", "momN", "momW", "momE", "momBx", "momBy"};
+  int momIdx = -1;
+  for (int q = 0; q < 6; ++q)
+    if (name == momNames[q]) momIdx = q;
+  if (momIdx >= 0 && settings_.gs_ordering == CFDGPU_GS_LEXICOGRAPHIC) {
+    if (static_cast<int>(N) > capacity) return -CFDGPU_OUT_OF_RANGE;
+    const size_t L = static_cast<size_t>(h_.gsSkewLen);
+    std::vector<double> sk(2 * L);
+    // pair (aP, aS) | (aN, aW) | (aE, bx); by on its own
+    static const int pairOf[] = {0, 1, 1, 2, 2, -1};
+    static const int halfOf[] = {1, 0, 1, 0, 1, 0};
+    const int pr = pairOf[momIdx];
+    if (pr >= 0) check(hipMemcpyAsync(sk.data(), h_.skMomRow[pr], L * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+                       "debug");
+    else check(hipMemcpyAsync(sk.data(), h_.skMomBy, L * sizeof(double), hipMemcpyDeviceToHost, stream_), "debug");
+    check(hipStreamSynchronize(stream_), "debug sync");
+    for (int j = 0; j < grid_.nJ; ++j)
+      for (int i = 0; i < grid_.nI; ++i) {
+        const size_t at = static_cast<size_t>(skewIndex(i, j, h_.gsR));
+        out[static_cast<size_t>(j) * grid_.nI + i] = (pr >= 0) ? sk[2 * at + halfOf[momIdx]] : sk[at];
+      }
+    return static_cast<int>(N);
+  }
+  if (momIdx >= 0 && momIdx < 4 && settings_.precision == CFDGPU_PRECISION_MIXED) {
+    if (static_cast<int>(N) > capacity) return -CFDGPU_OUT_OF_RANGE;
+    std::vector<float> f(N);
+    check(hipMemcpyAsync(f.data(), h_.fMom[momIdx + 1], N * sizeof(float), hipMemcpyDeviceToHost, stream_), "debug");
+    check(hipStreamSynchronize(stream_), "debug sync");
+    for (size_t c = 0; c < N; ++c) out[c] = f[c];
+    return static_cast<int>(N);
+  }
+  if (momIdx >= 4 && settings_.precision == CFDGPU_PRECISION_MIXED) return -CFDGPU_NOT_FOUND;
   if (name == "gradU") vec = h_.gU;
   else if (name == "gradV") vec = h_.gV;
   else if (name == "gradP") vec = h_.gP;
@@ -970,34 +1039,20 @@ int Engine::debugArray(const std::string& name, double* out, int capacity) const
 
 namespace cfdgpu {
 
-// Members of a batch share everything that shapes the launch schedule.
-static bool sameSchedule(const Inst& b, const Inst& a) {
-  return b.model == a.model && b.momSweeps == a.momSweeps && b.cgMax == a.cgMax &&
-         b.nonOrthPasses == a.nonOrthPasses && b.gsOrder == a.gsOrder && b.precision == a.precision &&
-         b.hasPRef == a.hasPRef;
-}
-
 Batch::Batch(std::vector<Engine*> members) : members_(std::move(members)) {
   if (members_.empty()) throw std::invalid_argument("a batch needs at least one solver");
   if (members_.size() > 65535) throw std::invalid_argument("too many solvers for one batch");
-  const Inst& a = members_.front()->refreshInst();
-  size_t lds = 0;
-  int lanes = 64, blocks = 1, cgBlocks = 1, maxNI = 0;
-  for (Engine* e : members_) {
+  for (Engine* e : members_)
     if (!e || !e->ready()) throw std::invalid_argument("every solver of a batch must be initialised");
-    const Inst& b = e->refreshInst();
-    if (!sameSchedule(b, a))
-      throw std::invalid_argument("batched solvers must share closure, sweeps, CG cap, correctors, ordering and "
-                                  "pressure anchoring");
-    lanes = std::max(lanes, b.gsR);
-    blocks = std::max(blocks, b.nBlocks);
-    cgBlocks = std::max(cgBlocks, b.cgBlocks);
-    maxNI = std::max(maxNI, b.nI);
-  }
-  lds = gsLdsFor(0, maxNI, lanes);
-  if (a.gsOrder == CFDGPU_GS_LEXICOGRAPHIC && lds > 64 * 1024)
+  std::vector<Inst> all;
+  for (Engine* e : members_) all.push_back(e->refreshInst());
+  for (const Inst& b : all)
+    if (!sameSchedule(b, all.front()))
+      throw std::invalid_argument("batched solvers must share closure, sweeps, CG cap, correctors, ordering, "
+                                  "precision, pressure solver and pressure anchoring");
+  cfg_ = launchCfgFor(all.data(), all.size());
+  if (all.front().gsOrder == CFDGPU_GS_LEXICOGRAPHIC && cfg_.gsLds > 64 * 1024)
     throw std::invalid_argument("batch too wide for the lexicographic sweep's LDS; use CFDGPU_GS_REDBLACK");
-  cfg_ = LaunchCfg{blocks, cgBlocks, static_cast<int>(members_.size()), lanes, lds};
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   check(hipMalloc(&dInsts_, sizeof(Inst) * members_.size()), "hipMalloc (batch descriptors)");
   host_.resize(members_.size());

@@ -24,6 +24,7 @@ namespace cfdgpu {
 struct LaunchCfg {
   int maxBlocks;   // grid.x for cell-parallel kernels
   int maxCgBlocks; // grid.x for the CG kernels
+  int maxColBlocks; // grid.x for the column-parallel kernels (line-preconditioned CG)
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
@@ -38,6 +39,10 @@ struct StageRecorder {
 };
 void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hipStream_t s,
                      StageRecorder* rec = nullptr);
+// The launch shape of a batch of instances (the widest member's grid of each kind).
+LaunchCfg launchCfgFor(const Inst* insts, size_t count);
+// Members of one batch share everything that shapes the launch schedule.
+bool sameSchedule(const Inst& a, const Inst& b);
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
 // distance from each of n centroids to the nearest of nw wall-face centres
 void launchWallDistance(const double2* centroid, int n, const double2* walls, int nw, double* out, hipStream_t s);
@@ -77,9 +82,9 @@ class IterationGraph {
  private:
   struct Key {
     const Inst* d;
-    int maxBlocks, maxCgBlocks, batch, gsThreads;
+    int maxBlocks, maxCgBlocks, maxColBlocks, batch, gsThreads;
     size_t gsLds;
-    int gsOrder, precision, momSweeps, nonOrthPasses, cgMax, hasPRef, model;
+    int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgMax, hasPRef, model;
     bool operator==(const Key& o) const;
   };
   static Key keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h);

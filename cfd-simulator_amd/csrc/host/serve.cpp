@@ -74,7 +74,8 @@ int Server::submit(const ServeRequest& r) {
     const cfdgpu_settings& b = opt_.settings;
     if (a.momentum_sweeps != b.momentum_sweeps || a.pressure_iterations != b.pressure_iterations ||
         a.non_orthogonal_correctors != b.non_orthogonal_correctors || a.gs_ordering != b.gs_ordering ||
-        a.precision != b.precision || a.pressure_solver != b.pressure_solver)
+        a.precision != b.precision || a.pressure_solver != b.pressure_solver ||
+        a.refinement_passes != b.refinement_passes)
       throw std::invalid_argument("a request's sweeps, CG cap, correctors, ordering, precision and pressure solver "
                                   "must be the server's (they shape the shared launches)");
   }
@@ -286,15 +287,8 @@ void Server::launchChunk() {
   if (active_.empty()) return;
   if (rosterDirty_) {
     host_.resize(active_.size());
-    int lanes = 64, blocks = 1, cgBlocks = 1, maxNI = 0;
-    for (size_t q = 0; q < active_.size(); ++q) {
-      host_[q] = active_[q]->engine->refreshInst();
-      lanes = std::max(lanes, host_[q].gsR);
-      blocks = std::max(blocks, host_[q].nBlocks);
-      cgBlocks = std::max(cgBlocks, host_[q].cgBlocks);
-      maxNI = std::max(maxNI, host_[q].nI);
-    }
-    cfg_ = LaunchCfg{blocks, cgBlocks, static_cast<int>(active_.size()), lanes, gsLdsFor(0, maxNI, lanes)};
+    for (size_t q = 0; q < active_.size(); ++q) host_[q] = active_[q]->engine->refreshInst();
+    cfg_ = launchCfgFor(host_.data(), host_.size());
     if (opt_.settings.gs_ordering == CFDGPU_GS_LEXICOGRAPHIC && cfg_.gsLds > 64 * 1024)
       throw std::invalid_argument("resident meshes too wide for the lexicographic sweep's LDS");
     hipCheck(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),

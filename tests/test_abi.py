@@ -38,10 +38,10 @@ def test_host_side_validation_needs_no_gpu(G):
     assert opts.hbm_budget_bytes == 200e9 and opts.model == G.MODEL_SST
     import ctypes as C
     h = C.c_void_p()
-    line = G.server_options(settings=G.default_settings(pressure_solver=G.PSOLVE_LINE))
-    assert lib.cfdgpu_server_create(C.byref(line), C.byref(h)) == 5  # NotImplemented
-    passes = G.server_options(settings=G.default_settings(refinement_passes=2))
-    assert lib.cfdgpu_server_create(C.byref(passes), C.byref(h)) == 5
+    solver = G.server_options(settings=G.default_settings(pressure_solver=9))
+    assert lib.cfdgpu_server_create(C.byref(solver), C.byref(h)) == 1  # InvalidArgument
+    passes = G.server_options(settings=G.default_settings(refinement_passes=0))
+    assert lib.cfdgpu_server_create(C.byref(passes), C.byref(h)) == 1
     bad = G.server_options(settings=G.default_settings(precision=7))
     assert lib.cfdgpu_server_create(C.byref(bad), C.byref(h)) == 1  # InvalidArgument
 

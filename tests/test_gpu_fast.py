@@ -1,0 +1,158 @@
+"""The fast path (no reference counterpart; judged against the FP64 path):
+the wall-normal line-preconditioned CG for the pressure correction, its
+FP64-residual iterative refinement, and the mixed path's FP32 two-colour
+defect sweeps. Each kernel family is checked against its definition (the
+NumPy restatement in linecg_ref.py, or the FP64 two-colour sweeps on the same
+state), then the whole path against closed-form solutions and against the
+FP64 path's converged answer."""
+import numpy as np
+import pytest
+
+import linecg_ref as L
+from helpers import SMALL, check_validation, converge, validation_case
+
+pytestmark = pytest.mark.gpu
+
+FAST = dict(gs_ordering=1, pressure_solver=1, pressure_iterations=20)
+
+
+def _cgrid_solver(G, precision, iterations, alpha=4.0, reynolds=1e5, **kw):
+    mesh = G.cgrid(0, 0, 12, options=G.cgrid_options("coarse", **SMALL))
+    kind, vel, pres = G.face_conditions(mesh, 1.0, alpha)
+    st = G.default_settings(**dict(FAST, precision=precision, **kw))
+    s = G.SimpleSolver(mesh, kind, vel, pres, st)
+    s.set_turbulence_model(G.MODEL_SST)
+    s.initialise(*G.uniform_field(mesh.cells, speed=1.0, alpha_deg=alpha, density=1.0, reynolds=reynolds))
+    mon = s.iterate(iterations) if iterations else None
+    return mesh, s, st, mon
+
+
+def _system(mesh, s):
+    ni, nj = mesh.ni, mesh.nj
+    dP, pN, pE, pW, b = (s.debug_array(n) for n in ("pDiag", "pN", "pE", "pWake", "pSource"))
+    partner = np.array([mesh.partner[i] if mesh.boundary[i] == 4 else -1 for i in range(ni)])
+    return L.matrix(dP, pN, pE, pW, ni, nj, partner), dP, pN, b
+
+
+@pytest.mark.parametrize("iterations", [1, 30])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_line_cg_matches_its_definition(G, gpu_available, precision, iterations):
+    print(f"case: 118x27 SST, line CG ({'fp64' if precision == 0 else 'fp32'}), {iterations} iterations",
+          flush=True)
+    mesh, s, st, mon = _cgrid_solver(G, precision, iterations)
+    A, dP, pN, b = _system(mesh, s)
+    x, its = L.line_cg(A, dP, pN, b, mesh.ni, mesh.nj, st.pressure_tolerance, st.pressure_iterations)
+    got = s.debug_array("correction")
+    err = np.max(np.abs(got - x)) / np.max(np.abs(x))
+    print(f"CG iterations {mon.pressure_iterations} (NumPy {its}), relative difference {err:.3e}", flush=True)
+    if precision == 0:
+        assert mon.pressure_iterations == its
+        assert err < 1e-10
+    else:
+        # FP32 storage and arithmetic, FP64 inner products: the iterate is
+        # the FP64 one to single-precision rounding accumulated over 20 steps
+        assert abs(mon.pressure_iterations - its) <= 1
+        assert err < 1e-4
+
+
+@pytest.mark.parametrize("passes", [2, 3])
+def test_refinement_matches_its_definition(G, gpu_available, passes):
+    print(f"case: 118x27 SST, line CG FP64, {passes} refinement passes", flush=True)
+    mesh, s, st, mon = _cgrid_solver(G, 0, 20, refinement_passes=passes, pressure_iterations=8)
+    A, dP, pN, b = _system(mesh, s)
+    x = L.refined_line_cg(A, dP, pN, b, mesh.ni, mesh.nj, st.pressure_tolerance, st.pressure_iterations, passes)
+    got = s.debug_array("correction")
+    err = np.max(np.abs(got - x)) / np.max(np.abs(x))
+    one, _ = L.line_cg(A, dP, pN, b, mesh.ni, mesh.nj, st.pressure_tolerance, st.pressure_iterations)
+    res = [np.abs(b - A @ v).sum() / np.abs(b).sum() for v in (one, got)]
+    print(f"relative difference {err:.3e}; L1 residual one pass {res[0]:.3e}, refined {res[1]:.3e}", flush=True)
+    assert err < 1e-10
+    assert res[1] < res[0]
+    assert mon.pressure_iterations == passes * st.pressure_iterations or mon.pressure_iterations > 0
+
+
+def test_mixed_refinement_reduces_the_fp64_residual(G, gpu_available):
+    print("case: 118x27 SST, mixed line CG, 1 and 3 refinement passes", flush=True)
+    res = []
+    for passes in (1, 3):
+        mesh, s, st, _ = _cgrid_solver(G, 1, 20, refinement_passes=passes, pressure_iterations=10)
+        A, dP, pN, b = _system(mesh, s)
+        x = s.debug_array("correction")
+        res.append(np.abs(b - A @ x).sum() / np.abs(b).sum())
+    print(f"FP64 L1 residual of the correction: one pass {res[0]:.3e}, three passes {res[1]:.3e}", flush=True)
+    assert np.isfinite(res).all() and res[1] < res[0]
+
+
+def test_fp32_defect_sweeps_match_the_fp64_sweeps(G, gpu_available):
+    """With no pressure correction (CG cap 0) one outer iteration is the
+    momentum and turbulence sweeps on the same assembled systems: the mixed
+    path's FP32 defect sweeps against the FP64 two-colour sweeps."""
+    print("case: 118x27 SST alpha 3, one iteration without pressure correction, FP64 vs FP32 sweeps", flush=True)
+    out = []
+    for precision in (0, 1):
+        mesh, s, st, _ = _cgrid_solver(G, precision, 1, alpha=3.0, pressure_iterations=0, pressure_solver=0)
+        u, v, _, _ = s.field()
+        t = s.turbulence()
+        out.append((u, v, t["k"], t["omega"]))
+    field = G.uniform_field(mesh.cells, speed=1.0, alpha_deg=3.0, density=1.0, reynolds=1e5)
+    for name, a, b, start in zip("u v k omega".split(), out[0], out[1], (field[0], field[1], None, None)):
+        change = np.max(np.abs(a - start)) if start is not None else np.max(np.abs(a))
+        diff = np.max(np.abs(a - b))
+        print(f"{name}: max |FP64 - FP32 sweeps| {diff:.3e} against {change:.3e}", flush=True)
+        assert diff <= 1e-5 * change, name
+
+
+@pytest.mark.parametrize("name,max_it,tol", [("uniform", 3000, 1e-12), ("couette", 3000, 1e-12),
+                                              ("poiseuille", 4000, 1e-9), ("blasius", 4000, 1e-8)])
+def test_fast_path_validation_case(G, gpu_available, name, max_it, tol):
+    print(f"case: validation {name} on the fast path (mixed, two colours, line CG)", flush=True)
+    mesh, kind, vel, pres, field, st, exact = validation_case(G, name)
+    st.precision, st.gs_ordering, st.pressure_solver, st.pressure_iterations = 1, 1, 1, 20
+    s = G.SimpleSolver(mesh, kind, vel, pres, st)
+    s.initialise(*field)
+    mon, taken = converge(s, max_it, tol)
+    print(f"converged in {taken} iterations", flush=True)
+    check_validation(name, mesh, *s.field()[:3], mon, s.mass_flux(), exact, taken, max_it)
+
+
+@pytest.mark.parametrize("section,alpha", [((0, 0, 12), 4.0), ((2, 4, 12), 2.0)])
+def test_fast_path_converges_to_the_fp64_answer(G, gpu_available, section, alpha):
+    """Both paths driven to residuals of 1e-9, where the coefficients have
+    settled: the fast path's answer is the FP64 path's fixed point."""
+    print(f"case: NACA {section} 118x27 SST Re 1e5 alpha {alpha}, fast vs FP64 to 1e-9", flush=True)
+    res = []
+    for fast in (False, True):
+        mesh = G.cgrid(*section, options=G.cgrid_options("coarse", **SMALL))
+        kind, vel, pres = G.face_conditions(mesh, 1.0, alpha)
+        st = G.default_settings(**(dict(FAST, precision=1) if fast else {}))
+        s = G.SimpleSolver(mesh, kind, vel, pres, st)
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(mesh.cells, speed=1.0, alpha_deg=alpha, density=1.0, reynolds=1e5))
+        rep = s.solve(1e-9, 20000, check_every=50)
+        f, _ = s.surface(speed=1.0, alpha_deg=alpha, density=1.0, reference_pressure=0.0)
+        print(f"{'fast' if fast else 'fp64'}: status {rep.status} after {rep.iterations} iterations, "
+              f"cl {f.cl:.9f} cd {f.cd:.9f} cm {f.cm:.9f}", flush=True)
+        assert rep.status == 0
+        res.append((f, rep.iterations))
+    (a, na), (b, nb) = res
+    assert abs(a.cl - b.cl) < 1e-5 and abs(a.cd - b.cd) < 1e-6 and abs(a.cm - b.cm) < 1e-5
+    assert nb < na  # fewer outer iterations as well as cheaper ones
+
+
+def test_fast_batch_member_matches_the_solve_alone(G, gpu_available):
+    print("case: fast-path batch of two sections against the first alone, 40 iterations", flush=True)
+
+    def make(m, p, t, alpha):
+        grid = G.cgrid(m, p, t, options=G.cgrid_options("coarse", **SMALL))
+        kind, vel, pres = G.face_conditions(grid, 1.0, alpha)
+        s = G.SimpleSolver(grid, kind, vel, pres, G.default_settings(**dict(FAST, precision=1)))
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(grid.cells, speed=1.0, alpha_deg=alpha, density=1.0, reynolds=1e5))
+        return grid, s
+
+    alone = make(0, 0, 12, 4.0)
+    pair = [make(0, 0, 12, 4.0), make(4, 4, 15, -2.0)]
+    alone[1].iterate(40)
+    G.Batch([s for _, s in pair]).iterate(40)
+    for a, b in zip(alone[1].field(), pair[0][1].field()):
+        assert np.array_equal(a, b)
