@@ -96,6 +96,12 @@ def precision_of(G, args):
     return G.MIXED if getattr(args, "precision", "fp64") == "mixed" else G.FP64
 
 
+def settings_summary(st):
+    return {"precision": ["fp64", "mixed"][st.precision], "gs_ordering": ["lexicographic", "two-colour"][st.gs_ordering],
+            "pressure_solver": ["jacobi-cg", "line-cg"][st.pressure_solver], "cg_cap": st.pressure_iterations,
+            "refinement_passes": st.refinement_passes}
+
+
 def path_settings(G, args, **kw):
     """SimpleSettings of the run: the reference defaults with the workload's
     inflow turbulence, and the path options of the command line."""
@@ -377,6 +383,9 @@ def run_serve(args, mixed_batch=False):
     import workloads as W
 
     rows = W.load(W.MIXED_BATCH_FILE if mixed_batch else W.SERVING_FILE)[: args.requests]
+    if args.only:
+        keep = [int(q) for q in args.only.split(",")]
+        rows = [rows[q] for q in keep]
     st = path_settings(G, args)
     srv = G.Server(G.server_options(hbm_budget_bytes=200e9, max_slots=args.slots, chunk_iterations=args.serve_chunk,
                                     tolerance=args.tolerance, max_iterations=args.max_iterations, settings=st,
@@ -416,7 +425,7 @@ def run_serve(args, mixed_batch=False):
     statuses = {G.STATUS_NAMES[k]: sum(1 for r in results if r.status == k) for k in G.STATUS_NAMES}
     name = ("mixed-size batch, 64 random NACA sections, coarse/medium/fine C-grids, SST" if mixed_batch else
             f"served stream, first {len(rows)} requests of the 20 req/s Poisson manifest, SST")
-    print(json.dumps({"workload": name, "precision": args.precision, "requests": len(rows), "finished": len(results),
+    print(json.dumps({"workload": name, "settings": settings_summary(st), "requests": len(rows), "finished": len(results),
                       "unfinished": len(rows) - len(results), "statuses": statuses,
                       "seconds": wall, "converged_solves_per_s": len(conv) / wall if wall > 0 else None,
                       "time_to_convergence_p50_s": _percentile(ttc, 0.5),
@@ -525,6 +534,7 @@ def main():
     ap.add_argument("--psolver", choices=("jacobi", "line"), help="pressure solver")
     ap.add_argument("--cg-max", type=int, help="pressure CG iteration cap")
     ap.add_argument("--refine", type=int, help="pressure refinement passes")
+    ap.add_argument("--only", help="with --mixed-batch / --serve: comma-separated manifest rows to run")
     ap.add_argument("--results-file", help="with --serve / --mixed-batch: write every request's result here")
     ap.add_argument("--compare-to", help="with --serve / --mixed-batch: a --results-file to compare coefficients with")
     args = ap.parse_args()

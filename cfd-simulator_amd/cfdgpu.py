@@ -35,7 +35,7 @@ EXPORTS = [
     "cfdgpu_mesh_quality", "cfdgpu_face_conditions", "cfdgpu_uniform_field", "cfdgpu_default_settings",
     "cfdgpu_default_sst", "cfdgpu_create", "cfdgpu_destroy", "cfdgpu_set_turbulence", "cfdgpu_initialise",
     "cfdgpu_iterate", "cfdgpu_iterate_timed", "cfdgpu_solve", "cfdgpu_batch_create", "cfdgpu_batch_destroy",
-    "cfdgpu_batch_iterate", "cfdgpu_batch_solve", "cfdgpu_polar_sweep", "cfdgpu_polar_csv", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
+    "cfdgpu_batch_iterate", "cfdgpu_batch_iterate_timed", "cfdgpu_batch_solve", "cfdgpu_polar_sweep", "cfdgpu_polar_csv", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
     "cfdgpu_set_field", "cfdgpu_get_massflux", "cfdgpu_set_massflux", "cfdgpu_get_turbulence",
     "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_surface",
     "cfdgpu_sweep_angles", "cfdgpu_debug_array", "cfdgpu_server_defaults", "cfdgpu_server_create",
@@ -197,6 +197,7 @@ def load(path: str = LIB_PATH):
         "cfdgpu_batch_create": (C.c_int, [pp, pp, C.c_int]),
         "cfdgpu_batch_destroy": (None, [vp]),
         "cfdgpu_batch_iterate": (C.c_int, [vp, C.c_int, C.POINTER(Monitor)]),
+        "cfdgpu_batch_iterate_timed": (C.c_int, [vp, C.c_int, C.POINTER(Monitor), D, C.POINTER(C.c_int)]),
         "cfdgpu_batch_solve": (C.c_int, [vp, C.POINTER(StopRule), C.POINTER(SolveReport)]),
         "cfdgpu_get_settings": (C.c_int, [vp, C.POINTER(Settings)]),
         "cfdgpu_set_settings": (C.c_int, [vp, C.POINTER(Settings)]),
@@ -479,6 +480,14 @@ class Batch:
         mons = (Monitor * len(self.solvers))()
         _check(lib().cfdgpu_batch_iterate(self.h, count, mons))
         return list(mons)
+
+    def iterate_timed(self, count: int = 1):
+        """(monitors, {stage: ms summed over the iterations}, {stage: calls})."""
+        mons = (Monitor * len(self.solvers))()
+        ms = (C.c_double * len(STAGES))()
+        calls = (C.c_int * len(STAGES))()
+        _check(lib().cfdgpu_batch_iterate_timed(self.h, count, mons, ms, calls))
+        return list(mons), dict(zip(STAGES, list(ms))), dict(zip(STAGES, list(calls)))
 
     def solve(self, tolerance: float, max_iterations: int, check_every: int = 25):
         rule = StopRule(tolerance, max_iterations, check_every)

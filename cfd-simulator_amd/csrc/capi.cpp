@@ -405,6 +405,13 @@ int cfdgpu_batch_iterate(cfdgpu_batch* b, int count, cfdgpu_monitor* out) {
   return guarded([&]() -> int { return b->batch->iterate(count, out); });
 }
 
+int cfdgpu_batch_iterate_timed(cfdgpu_batch* b, int count, cfdgpu_monitor* out, double* stage_ms,
+                               int* stage_calls) {
+  if (!b || !b->batch) return fail(CFDGPU_INVALID_ARGUMENT, "null batch");
+  if (count < 0) return fail(CFDGPU_INVALID_ARGUMENT, "negative iteration count");
+  return guarded([&]() -> int { return b->batch->iterateTimed(count, out, stage_ms, stage_calls); });
+}
+
 int cfdgpu_batch_solve(cfdgpu_batch* b, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out) {
   if (!b || !b->batch) return fail(CFDGPU_INVALID_ARGUMENT, "null batch");
   if (!rule) return fail(CFDGPU_INVALID_ARGUMENT, "null stop rule");

@@ -1091,6 +1091,22 @@ int Batch::iterate(int count, cfdgpu_monitor* out) {
   return CFDGPU_OK;
 }
 
+int Batch::iterateTimed(int count, cfdgpu_monitor* out, double* stageMs, int* stageCalls) {
+  gather();
+  if (!clock_) clock_ = std::make_unique<StageClock>();
+  StageClock& clock = *clock_;
+  clock.reset();
+  launchResumeAll(dInsts_, size(), stream_);
+  for (int n = 0; n < count; ++n) launchIteration(dInsts_, cfg_, host_.front(), stream_, &clock);
+  check(hipGetLastError(), "launch");
+  check(hipStreamSynchronize(stream_), "batch sync");
+  for (Engine* e : members_) e->checkStall();
+  clock.collect(stageMs, stageCalls);
+  if (out)
+    for (size_t q = 0; q < members_.size(); ++q) members_[q]->readMonitorOn(stream_, &out[q]);
+  return CFDGPU_OK;
+}
+
 // The stop rule of each member, evaluated on the device; finished members
 // return at once from every later launch while the others continue.
 int Batch::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {

@@ -208,10 +208,14 @@ class Batch {
   Batch& operator=(const Batch&) = delete;
   int size() const { return static_cast<int>(members_.size()); }
   int iterate(int count, cfdgpu_monitor* out);
+  // iterate() launched directly with HIP events on the batch's stream around
+  // every stage (per-stage device time summed over the iterations)
+  int iterateTimed(int count, cfdgpu_monitor* out, double* stageMs, int* stageCalls);
   int solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out);
 
  private:
   void gather();
+  std::unique_ptr<StageClock> clock_;
   std::vector<Engine*> members_;
   std::vector<Inst> host_;
   Inst* dInsts_{nullptr};

@@ -43,10 +43,13 @@ enum { CFDGPU_MODEL_NONE = 0, CFDGPU_MODEL_LAMINAR = 1, CFDGPU_MODEL_SST = 2 };
 /* arithmetic of the hot path */
 enum {
   CFDGPU_PRECISION_FP64 = 0, /* plain FP64 path: the drop-in and the judge */
-  CFDGPU_PRECISION_MIXED = 1 /* the pressure-correction CG stores its matrix and r, z, p, Ap
-                                in FP32 (half the bytes per iteration); its arithmetic, its
-                                inner products and norms, x, the sweeps, the residuals and
-                                the stop test stay FP64 */
+  CFDGPU_PRECISION_MIXED = 1 /* FP32 bulk, FP64 reductions. Pressure solve: the matrix and
+                                r, z, p, Ap in FP32 (line CG: FP32 arithmetic too; Jacobi CG:
+                                FP64 arithmetic), inner products, norms and the correction x in
+                                FP64. With CFDGPU_GS_REDBLACK the momentum and k-omega sweeps
+                                run in FP32 on the defect b - A phi formed in FP64 and add
+                                their correction to the FP64 field. Assembly, fluxes, pressure,
+                                residual norms, the stop test and the force integrals: FP64 */
 };
 /* Gauss-Seidel ordering of the momentum and turbulence sweeps */
 enum {
@@ -56,8 +59,10 @@ enum {
 /* pressure-correction solver */
 enum {
   CFDGPU_PSOLVE_JACOBI_CG = 0, /* reference: Jacobi-PCG, L1 stop (LinearSystem.cpp:118-210) */
-  CFDGPU_PSOLVE_LINE_CG = 1    /* reserved (line-preconditioned CG): rejected with
-                                  CFDGPU_NOT_IMPLEMENTED */
+  CFDGPU_PSOLVE_LINE_CG = 1    /* the fast path's CG preconditioned by the exact inverse of each
+                                  wall-normal column's tridiagonal part; the reference's L1 stop,
+                                  cap and breakdown guards; FP32 storage and arithmetic with FP64
+                                  inner products under CFDGPU_PRECISION_MIXED */
 };
 
 /* stages of one outer iteration, for cfdgpu_iterate_timed */
@@ -115,7 +120,9 @@ typedef struct {
   int precision;      /* CFDGPU_PRECISION_* */
   int gs_ordering;    /* CFDGPU_GS_* */
   int pressure_solver;/* CFDGPU_PSOLVE_* */
-  int refinement_passes; /* reserved: must be 1 (one pressure solve per corrector) */
+  int refinement_passes; /* 1..8 pressure solves per corrector: pass k > 1 solves for the
+                            FP64 residual b - A x of the correction so far and adds its
+                            solution in FP64 (iterative refinement); 1 = the reference */
 } cfdgpu_settings;
 
 /* solver::SSTConstants (include/cfd/solver/KOmegaSST.hpp:147-171) */
@@ -244,6 +251,11 @@ int cfdgpu_batch_create(cfdgpu_batch** out, cfdgpu_solver* const* solvers, int c
 void cfdgpu_batch_destroy(cfdgpu_batch* batch);
 /* count outer iterations of every member; out[i] = member i's last monitor (may be NULL) */
 int cfdgpu_batch_iterate(cfdgpu_batch* batch, int count, cfdgpu_monitor* out);
+/* cfdgpu_batch_iterate launched directly, with HIP events on the batch's stream around every
+   stage: stage_ms / stage_calls as in cfdgpu_iterate_timed, summed over all members (either may
+   be NULL). No reference counterpart: the measurement hook of the batched path. */
+int cfdgpu_batch_iterate_timed(cfdgpu_batch* batch, int count, cfdgpu_monitor* out, double* stage_ms,
+                               int* stage_calls);
 /* every member's stop rule on the device until all have stopped; out[i] per member (may be NULL) */
 int cfdgpu_batch_solve(cfdgpu_batch* batch, const cfdgpu_stop_rule* rule, cfdgpu_solve_report* out);
 int cfdgpu_get_settings(const cfdgpu_solver* solver, cfdgpu_settings* out);
