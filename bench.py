@@ -68,6 +68,11 @@ def traffic_model():
         "rhie_chow_pressure": 34,
         "correction": 23,
         "sst_update": 130,
+        # bytes per cell of one line-CG iteration on the mixed path (FP32 storage, FP64 x):
+        # launch A reads z, p_old, aP, aN, aE (4 B each) and writes p, Ap = 28; launch B reads
+        # p, Ap, r, 1/pivot, aN (20) and x (8), writes x (8), r, z (8), then the back
+        # substitution reads z, aN, 1/pivot, r (16) and writes z (4) = 64
+        "line_cg_iteration_mixed": 92,
     }
 
 
@@ -163,6 +168,165 @@ def cpu_baseline(iterations: int, iters_per_solve):
             "iterations_per_s": 1.0 / sec, "s_per_iteration": sec, "cores": 1, "kind": "port",
             "sample": f"{iterations} outer iterations of the same solve (FP64 oracle, 1 thread, -O2), "
                       f"after 1 untimed"}
+
+
+def _host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model or "unknown", os.cpu_count() or 1
+
+
+def _oracle_instance(row, level):
+    """The oracle's solver for one manifest instance (the same inputs as the GPU path)."""
+    sys.path.insert(0, os.path.join(_HERE, "oracle"))
+    import oracle_py as O  # the checker: timed here as the CPU baseline only
+    import workloads as W
+
+    s_pts, w_pts, n_pts = W.LEVELS[level]
+    opts = O.cgrid_options("coarse", upstream=15.0, vertical=15.0, downstream=20.0, surface_points=s_pts,
+                           wake_points=w_pts, normal_points=n_pts, first_layer=W.first_layer(row["reynolds"]))
+    mesh = O.cgrid(row["m"], row["p"], row["t"], options=opts)
+    kind, vel, pres = O.face_conditions(mesh, 1.0, row["alpha"])
+    st = O.default_settings()
+    st.intensity, st.viscosity_ratio, st.length_scale = 0.001, 10.0, 0.0
+    s = O.Solver(mesh, kind, vel, pres, st)
+    s.set_model(2)
+    s.initialise(*O.uniform_field(mesh.cells, speed=1.0, alpha_deg=row["alpha"], density=1.0,
+                                  reynolds=row["reynolds"]))
+    return s
+
+
+def _oracle_rate(job):
+    """Outer iterations per second of the oracle on one instance (one process, one thread)."""
+    row, level, iterations = job
+    s = _oracle_instance(row, level)
+    s.iterate(1)
+    t0 = time.perf_counter()
+    s.iterate(iterations)
+    return iterations / (time.perf_counter() - t0)
+
+
+def cpu_baseline_batch(rows, results, sample_iterations=(12, 4, 2), parallel_iterations=12):
+    """The FP64 oracle (a restatement of the reference solver; one solve per
+    thread, as the reference's SolverWorker runs, SolverWorker.cpp:52) on the
+    same instances, timed on this host. Latency: each level's seconds per
+    outer iteration on one core (the first instance of the level, after one
+    untimed iteration) times every solve's outer iterations from the plain
+    GPU run (the same iterates up to reduction order), p50/p99 over the
+    converged solves. Throughput: P independent solver processes, one per
+    core, on a medium instance together; their parallel efficiency scales the
+    single-core work of the whole batch to P cores."""
+    import multiprocessing as mp
+
+    model, nproc = _host_cpu()
+    per_level = {}
+    for level, n in zip(("coarse", "medium", "fine"), sample_iterations):
+        row = next((r for r in rows if r["level"] == level), None)
+        if row is None:
+            continue
+        per_level[level] = 1.0 / _oracle_rate((row, level, n))
+    lat = sorted(per_level[rows[q]["level"]] * r["iterations"] for q, r in results.items() if r["status"] == 0)
+    work = sum(per_level[rows[q]["level"]] * r["iterations"] for q, r in results.items())
+    converged = sum(1 for r in results.values() if r["status"] == 0)
+    procs = max(1, min(nproc, 16))
+    medium = next((r for r in rows if r["level"] == "medium"), rows[0])
+    lvl = medium["level"]
+    with mp.get_context("spawn").Pool(procs) as pool:
+        rates = pool.map(_oracle_rate, [(medium, lvl, parallel_iterations)] * procs)
+    efficiency = (sum(rates) / procs) * per_level[lvl]
+    seconds = work / (procs * efficiency)
+    return {"value": converged / seconds if seconds > 0 else None, "unit": UNIT, "cores": procs, "kind": "port",
+            "cpu_model": model, "nproc": nproc,
+            "flags": "g++ -O2 -std=c++17 -ffp-contract=off -fno-builtin-sin -fno-builtin-cos (oracle/Makefile)",
+            "latency_p50_s": _percentile(lat, 0.5), "latency_p99_s": _percentile(lat, 0.99),
+            "s_per_iteration": per_level, "parallel_efficiency": efficiency,
+            "single_core_seconds": work, "seconds_on_cores": seconds,
+            "sample": (f"oracle seconds per outer iteration on one core per mesh level ({sample_iterations} "
+                       f"iterations after 1 untimed), times the plain GPU run's outer iterations per solve; "
+                       f"{procs} concurrent processes x {parallel_iterations} iterations of a 512x128 instance "
+                       f"for the parallel efficiency")}
+
+
+def run_headline(args):
+    """The benchmark line: the metric on the 64-instance mixed-size batch."""
+    import cfdgpu as G
+    import workloads as W
+
+    if G.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU: no HIP device visible")
+    H = HEADLINE
+    rows = W.load(W.MIXED_BATCH_FILE)
+    fast_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=G.MIXED, gs_ordering=1,
+                                 pressure_solver=1, pressure_iterations=args.fast_cg)
+    plain_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0)
+    common = dict(tolerance=H["tolerance"], max_iterations=H["max_iterations"], chunk=H["chunk"], slots=H["slots"],
+                  settle_window=H["settle_window"], settle_cl=H["settle_cl"], settle_cd=H["settle_cd"])
+    # the timed unit of the roofline: K outer iterations of the whole batch after W untimed
+    cells, ms_per_step, stage_ms, calls, cg_its = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
+    fast = serve_batch(G, W, rows, fast_st, "fast", **common)
+    plain = serve_batch(G, W, rows, plain_st, "plain FP64", **common)
+    acc = acceptance(fast["results"], plain["results"])
+    accepted = sum(1 for a in acc if a["accepted"])
+    fs, ps = summarize(fast, rows), summarize(plain, rows)
+    value = accepted / fast["seconds"]
+    # plain FP64 one solve at a time: each level's ms per outer iteration alone x the plain run's iterations
+    alone_ms = alone_rates(G, W, rows, plain_st)
+    alone_s = sum(r["iterations"] * alone_ms[r["level"]] * 1e-3 for r in plain["results"].values())
+    plain_conv = ps["statuses"]["converged"]
+    alone_rate = plain_conv / alone_s if alone_s > 0 else None
+    fast_alone_ms = alone_rates(G, W, rows, fast_st)
+
+    tm = traffic_model()
+    unit_ms = stage_ms["pressure_solve"] / max(cg_its, 1)
+    algo = tm["line_cg_iteration_mixed"] * cells
+    achieved = algo / (unit_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get("line_cg_iteration_mixed"),
+                "kernel": "k_lcg_a_f32 + k_lcg_b_f32", "stage": "pressure_solve",
+                "algorithmic_bytes_per_launch": algo, "launch_ms": unit_ms,
+                "bytes_per_cell": tm["line_cg_iteration_mixed"], "cells": cells,
+                "note": "one line-CG iteration of the whole batch (launch A + launch B) from HIP events on the "
+                        "batch's stream; the init launch of each solve is included in the stage time"}
+    line = {"metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "mixed-size batch: 64 random NACA four-digit sections (seed 0), C-grids of 256x64, "
+                                   "512x128 and 1024x256 cells (8.03 M cells), k-omega SST, Re 2e5-5e6, alpha -5..12; "
+                                   "served at once with continuous batching",
+                       "instances": len(rows), "cells": cells,
+                       "stop_rule": f"residuals.worst() < {H['tolerance']:g}, limit {H['max_iterations']}, and Cl, Cd "
+                                    f"settled (estimated remaining change <= {H['settle_cl']:g}, {H['settle_cd']:g} "
+                                    f"from samples every {H['settle_window']} iterations)",
+                       "fast_path": settings_summary(fast_st), "plain_path": settings_summary(plain_st),
+                       "step": "one outer iteration of all 64 resident solves (fast path), timed by HIP events"},
+            "accepted": accepted, "converged_both": len(acc),
+            "acceptance": {"bar": {"dcl": ACCEPT_CL, "dcd": ACCEPT_CD},
+                           "max_dcl": max((a["dcl"] for a in acc), default=None),
+                           "max_dcd": max((a["dcd"] for a in acc), default=None),
+                           "rejected_rows": [a["row"] for a in acc if not a["accepted"]]},
+            "fast": fs, "plain_batched": ps,
+            "plain_one_at_a_time": {"converged_solves_per_s": alone_rate, "seconds_estimate": alone_s,
+                                    "ms_per_iteration": alone_ms},
+            "fast_one_at_a_time_ms_per_iteration": fast_alone_ms,
+            "speedup": {"vs_plain_one_at_a_time": value / alone_rate if alone_rate else None,
+                        "vs_plain_batched": value / ps["converged_solves_per_s"] if ps["converged_solves_per_s"]
+                        else None},
+            "latency": {"p50_s": fs["time_to_convergence_p50_s"], "p99_s": fs["time_to_convergence_p99_s"]},
+            "roofline": roofline,
+            "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()}}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_batch(rows, plain["results"])
+    if args.results_file:
+        with open(args.results_file, "w") as fh:
+            json.dump({"fast": fast, "plain": plain, "acceptance": acc}, fh)
+    print(json.dumps(line), flush=True)
 
 
 def run_bench(args):
@@ -442,6 +606,126 @@ def run_serve(args, mixed_batch=False):
           flush=True)
 
 
+# The headline workload: the 64-instance mixed-size batch (seed 0), all
+# arriving at once, served with continuous batching (64 slots, chunks of 20
+# outer iterations), worker stop rule residuals.worst() < 1e-6 with the
+# reference's 10,000-iteration limit, plus coefficient settling (Cl and Cd
+# integrated every 500 iterations; their estimated remaining change at most
+# 5e-4 and 5e-5, half the acceptance bar) for both paths.
+HEADLINE = dict(tolerance=1e-6, max_iterations=10000, chunk=20, slots=64, settle_window=500, settle_cl=5e-4,
+                settle_cd=5e-5)
+ACCEPT_CL, ACCEPT_CD = 1e-3, 1e-4
+
+
+def serve_batch(G, W, rows, st, label, tolerance, max_iterations, chunk, slots, settle_window=0, settle_cl=5e-4,
+                settle_cd=5e-5, wall_limit=0.0, progress=15.0):
+    """Serves the rows (all arriving at once) and returns per-row results,
+    the wall time and the server statistics."""
+    srv = G.Server(G.server_options(hbm_budget_bytes=200e9, max_slots=slots, chunk_iterations=chunk,
+                                    tolerance=tolerance, max_iterations=max_iterations, settings=st,
+                                    warm_start=0, settle_window=settle_window, settle_cl=settle_cl,
+                                    settle_cd=settle_cd))
+    tickets = {}
+    for q, r in enumerate(rows):
+        t = srv.submit(G.serve_request(r["m"], r["p"], r["t"], r["alpha"], r["reynolds"],
+                                       W.mesh_options(G, r["level"], r["reynolds"]), arrival_s=0.0))
+        tickets[t] = q
+    results, last = [], 0.0
+    while True:
+        stats = srv.step()
+        results += srv.collect()
+        if stats.clock_s - last >= progress:
+            last = stats.clock_s
+            print(f"[{label}] {stats.clock_s:7.1f}s resident {stats.active} finished {stats.finished} "
+                  f"converged {stats.converged}", flush=True)
+        if stats.queued == 0 and stats.active == 0:
+            break
+        if wall_limit > 0 and stats.clock_s >= wall_limit:
+            break
+    out = {tickets[r.ticket]: {"level": rows[tickets[r.ticket]]["level"], "status": r.status,
+                               "iterations": r.iterations, "cl": r.cl, "cd": r.cd, "cm": r.cm,
+                               "ttc_s": r.finished_s - r.arrival_s, "cells": r.cells} for r in results}
+    return {"results": out, "seconds": stats.clock_s, "iterations": stats.member_iterations,
+            "peak_gb": stats.peak_resident_bytes / 1e9}
+
+
+def summarize(run, rows):
+    res = run["results"]
+    conv = [r for r in res.values() if r["status"] == 0]
+    ttc = sorted(r["ttc_s"] for r in conv)
+    statuses = {G_STATUS[k]: sum(1 for r in res.values() if r["status"] == k) for k in G_STATUS}
+    return {"seconds": run["seconds"], "statuses": statuses,
+            "converged_solves_per_s": len(conv) / run["seconds"] if run["seconds"] > 0 else None,
+            "time_to_convergence_p50_s": _percentile(ttc, 0.5), "time_to_convergence_p99_s": _percentile(ttc, 0.99),
+            "outer_iterations": run["iterations"],
+            "cell_iterations_per_s": sum(r["cells"] * r["iterations"] for r in res.values()) / run["seconds"],
+            "peak_resident_gb": run["peak_gb"]}
+
+
+G_STATUS = {0: "converged", 1: "iteration-limit", 2: "diverged", 3: "rejected"}
+
+
+def alone_rates(G, W, rows, settings, iterations=50, warmup=10):
+    """ms per outer iteration of one solve alone on the given path, per mesh
+    level (the level's first manifest instance, after warm-up)."""
+    out = {}
+    for lv in W.LEVELS:
+        r = next((x for x in rows if x["level"] == lv), None)
+        if r is None:
+            continue
+        mesh = G.cgrid(r["m"], r["p"], r["t"], options=W.mesh_options(G, lv, r["reynolds"]))
+        kind, vel, pres = G.face_conditions(mesh, 1.0, r["alpha"])
+        s = G.SimpleSolver(mesh, kind, vel, pres, settings)
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(mesh.cells, speed=1.0, alpha_deg=r["alpha"], density=1.0,
+                                      reynolds=r["reynolds"]))
+        s.iterate(warmup)
+        t0 = time.perf_counter()
+        s.iterate(iterations)
+        out[lv] = (time.perf_counter() - t0) * 1e3 / iterations
+    return out
+
+
+def acceptance(fast, plain):
+    """Per instance converged on both paths: |dCl| and |dCd| of the fast
+    path's result against the plain FP64 path's (the judge)."""
+    rows = []
+    for q, a in fast.items():
+        b = plain.get(q)
+        if b is None or a["status"] != 0 or b["status"] != 0:
+            continue
+        dcl, dcd = abs(a["cl"] - b["cl"]), abs(a["cd"] - b["cd"])
+        rows.append({"row": q, "level": a["level"], "dcl": dcl, "dcd": dcd,
+                     "accepted": dcl < ACCEPT_CL and dcd < ACCEPT_CD})
+    return rows
+
+
+def roofline_batch(G, W, rows, st, steps, warmup):
+    """Stage times of the fast path's outer iteration over the whole batch
+    (64 instances resident, 8.0 M cells), from HIP events on the batch's
+    stream inside the timed region; the line CG iteration (launch A + launch
+    B) is the unit of the roofline."""
+    solvers = []
+    cells = 0
+    for r in rows:
+        mesh = G.cgrid(r["m"], r["p"], r["t"], options=W.mesh_options(G, r["level"], r["reynolds"]))
+        kind, vel, pres = G.face_conditions(mesh, 1.0, r["alpha"])
+        s = G.SimpleSolver(mesh, kind, vel, pres, st)
+        s.set_turbulence_model(G.MODEL_SST)
+        s.initialise(*G.uniform_field(mesh.cells, speed=1.0, alpha_deg=r["alpha"], density=1.0,
+                                      reynolds=r["reynolds"]))
+        solvers.append((mesh, s))
+        cells += mesh.cells
+    batch = G.Batch([s for _, s in solvers])
+    if warmup:
+        batch.iterate(warmup)
+    t0 = time.perf_counter()
+    mons, stage_ms, calls = batch.iterate_timed(steps)
+    wall = time.perf_counter() - t0
+    cg_its = max(m.pressure_iterations for m in mons) * steps
+    return cells, wall * 1e3 / steps, stage_ms, calls, cg_its
+
+
 def compare_results(mine, path):
     """Per-request coefficients against another run of the same manifest (the
     --results-file of a plain FP64 run): the largest |dCl|, |dCd|, |dCm| over
@@ -503,8 +787,8 @@ def run_calibrate_copy(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-iterations", type=int, default=6, help="oracle iterations in the CPU sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--converge", action="store_true", help="run the solve to convergence instead")
@@ -535,6 +819,9 @@ def main():
     ap.add_argument("--cg-max", type=int, help="pressure CG iteration cap")
     ap.add_argument("--refine", type=int, help="pressure refinement passes")
     ap.add_argument("--only", help="with --mixed-batch / --serve: comma-separated manifest rows to run")
+    ap.add_argument("--single", action="store_true",
+                    help="the single 1024x256 SST solve (plain FP64 step time, roofline, mixed comparison)")
+    ap.add_argument("--fast-cg", type=int, default=20, help="line-CG cap of the fast path in the headline run")
     ap.add_argument("--results-file", help="with --serve / --mixed-batch: write every request's result here")
     ap.add_argument("--compare-to", help="with --serve / --mixed-batch: a --results-file to compare coefficients with")
     args = ap.parse_args()
@@ -552,8 +839,10 @@ def main():
         run_solve(args)
     elif args.converge:
         run_converge(args)
-    else:
+    elif args.single:
         run_bench(args)
+    else:
+        run_headline(args)
 
 
 if __name__ == "__main__":

@@ -128,7 +128,8 @@ class Forces(C.Structure):
 class ServerOptions(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_double), ("max_slots", C.c_int), ("chunk_iterations", C.c_int),
                 ("rule", StopRule), ("settings", Settings), ("model", C.c_int), ("warm_start", C.c_int),
-                ("keep_fields", C.c_int)]
+                ("keep_fields", C.c_int), ("settle_window", C.c_int), ("settle_cl", C.c_double),
+                ("settle_cd", C.c_double)]
 
 
 class ServeRequest(C.Structure):

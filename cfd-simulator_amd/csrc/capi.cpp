@@ -546,6 +546,9 @@ void cfdgpu_server_defaults(cfdgpu_server_options* o) {
   o->model = CFDGPU_MODEL_SST;
   o->warm_start = 1;
   o->keep_fields = 0;
+  o->settle_window = 0;
+  o->settle_cl = 5e-4;
+  o->settle_cd = 5e-5;
 }
 
 int cfdgpu_server_create(const cfdgpu_server_options* o, cfdgpu_server** out) {
@@ -564,6 +567,11 @@ int cfdgpu_server_create(const cfdgpu_server_options* o, cfdgpu_server** out) {
     so.model = o->model;
     so.warmStart = o->warm_start != 0;
     so.keepFields = o->keep_fields != 0;
+    if (o->settle_window < 0 || !(o->settle_cl >= 0.0) || !(o->settle_cd >= 0.0))
+      return fail(CFDGPU_INVALID_ARGUMENT, "settle_window, settle_cl and settle_cd must not be negative");
+    so.settleWindow = o->settle_window;
+    so.settleCl = o->settle_cl;
+    so.settleCd = o->settle_cd;
     auto* s = new cfdgpu_server();
     try {
       s->server = std::make_unique<cfdgpu::Server>(so);

@@ -77,6 +77,55 @@ __device__ __forceinline__ double waveSum(double v) {
 // Line-preconditioned CG
 // ---------------------------------------------------------------------------
 
+// A column sweep walks its nJ rows in blocks of kColUnroll. The operands of
+// the next block are loaded before the current block is computed (rows past
+// the end are clamped to the last row and not used), so the recurrence waits
+// for memory once per block instead of once per row.
+constexpr int kColUnroll = 8;
+
+// Back substitution z_j = y_j - (pN_j inv_j) z_{j+1}, j = nJ-1 .. 0, in place
+// over z (which holds y); returns the lane's sum of r_j z_j in FP64.
+template <typename T>
+__device__ __forceinline__ double lineBack(const LineStore<T>& S, int ic, bool col, int nI, int nJ) {
+  constexpr int U = kColUnroll;
+  T yA[U], nA[U], iA[U], rA[U];
+  T yB[U] = {}, nB[U] = {}, iB[U] = {}, rB[U] = {};
+  auto load = [&](int top, T (&y)[U], T (&n)[U], T (&iv)[U], T (&r)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = max(top - u, 0) * nI + ic;
+      y[u] = S.z[c];
+      n[u] = S.pN[c];
+      iv[u] = S.inv[c];
+      r[u] = S.r[c];
+    }
+  };
+  double rz = 0.0;
+  T zNext = T(0);
+  load(nJ - 1, yA, nA, iA, rA);
+  for (int top = nJ - 1; top >= 0; top -= U) {
+    if (top - U >= 0) load(top - U, yB, nB, iB, rB);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = top - u;
+      if (j >= 0) {
+        const T z = yA[u] - (nA[u] * iA[u]) * zNext;
+        if (col) S.z[j * nI + ic] = z;
+        rz += static_cast<double>(rA[u]) * static_cast<double>(z);
+        zNext = z;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      yA[u] = yB[u];
+      nA[u] = nB[u];
+      iA[u] = iB[u];
+      rA[u] = rB[u];
+    }
+  }
+  return col ? rz : 0.0;
+}
+
 // Start of pass `pass`: r = b (pass 0: the assembled source, its mean removed
 // when nothing anchors the pressure level, as the Jacobi solver does; later
 // passes: the FP64 residual of the correction so far), x = 0, the factor of
@@ -88,6 +137,7 @@ __device__ __forceinline__ double waveSum(double v) {
 template <typename T>
 __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
   LCG_COLUMN_PROLOGUE
+  constexpr int U = kColUnroll;
   __shared__ double bc[1];
   const LineStore<T> S = lineStore<T>(I);
   const bool demean = pass == 0 && !I.hasPRef;
@@ -98,39 +148,59 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
   __syncthreads();
   const double mean = demean ? bc[0] / static_cast<double>(I.N) : 0.0;
   const double* src = (pass == 0) ? I.pb : I.rb;
-  double l1 = 0.0, rz = 0.0;
-  if (col) {
-    T cPrev = T(0), yPrev = T(0);
-    for (int j = 0; j < nJ; ++j) {
-      const int c = j * nI + i;
-      double b = src[c];
-      if (demean) {
-        b -= mean;
-        I.pb[c] = b;
-      }
-      const T r = static_cast<T>(b);
-      const T lo = (j > 0) ? S.pN[c - nI] : T(0);
-      const T den = S.pP[c] - lo * cPrev;
-      const T inv = (fabs(static_cast<double>(den)) > 0.0) ? T(1) / den : T(1);
-      const T y = (r - lo * yPrev) * inv;
-      S.inv[c] = inv;
-      S.r[c] = r;
-      S.z[c] = y;
-      I.x[c] = 0.0;
-      l1 += fabs(static_cast<double>(r));
-      cPrev = S.pN[c] * inv;  // zero on the top row
-      yPrev = y;
+  const int ic = col ? i : nI - 1;  // idle lanes read a real column and store nothing
+  double bA[U], bB[U] = {};
+  T dA[U], nA[U], dB[U] = {}, nB[U] = {};
+  auto load = [&](int j0, double (&b)[U], T (&d)[U], T (&n)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = min(j0 + u, nJ - 1) * nI + ic;
+      b[u] = src[c];
+      d[u] = S.pP[c];
+      n[u] = S.pN[c];
     }
-    T zNext = T(0);
-    for (int j = nJ - 1; j >= 0; --j) {
-      const int c = j * nI + i;
-      const T z = S.z[c] - (S.pN[c] * S.inv[c]) * zNext;
-      S.z[c] = z;
-      rz += static_cast<double>(S.r[c]) * static_cast<double>(z);
-      zNext = z;
+  };
+  double l1 = 0.0;
+  T cPrev = T(0), yPrev = T(0), nPrev = T(0);
+  load(0, bA, dA, nA);
+  for (int j0 = 0; j0 < nJ; j0 += U) {
+    if (j0 + U < nJ) load(j0 + U, bB, dB, nB);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u;
+      if (j < nJ) {
+        const int c = j * nI + ic;
+        double b = bA[u];
+        if (demean) {
+          b -= mean;
+          if (col) I.pb[c] = b;
+        }
+        const T r = static_cast<T>(b);
+        const T lo = nPrev;  // the coupling with the row below (zero on row 0)
+        const T den = dA[u] - lo * cPrev;
+        const T inv = (fabs(static_cast<double>(den)) > 0.0) ? T(1) / den : T(1);
+        const T y = (r - lo * yPrev) * inv;
+        if (col) {
+          S.inv[c] = inv;
+          S.r[c] = r;
+          S.z[c] = y;
+          I.x[c] = 0.0;
+        }
+        l1 += fabs(static_cast<double>(r));
+        cPrev = nA[u] * inv;  // zero on the top row
+        yPrev = y;
+        nPrev = nA[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bA[u] = bB[u];
+      dA[u] = dB[u];
+      nA[u] = nB[u];
     }
   }
-  l1 = waveSum(l1);
+  double rz = lineBack<T>(S, ic, col, nI, nJ);
+  l1 = waveSum(col ? l1 : 0.0);
   rz = waveSum(rz);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = l1;
@@ -212,6 +282,7 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
 template <typename T>
 __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
   LCG_COLUMN_PROLOGUE
+  constexpr int U = kColUnroll;
   __shared__ double bc[1];
   if (I.flags[fCgDone]) return;
   const LineStore<T> S = lineStore<T>(I);
@@ -226,31 +297,56 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
   const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
   const T alphaT = static_cast<T>(alpha);
   const T* p = (it & 1) ? S.pd1 : S.pd0;
-  double l1 = 0.0, rz = 0.0;
-  if (col) {
-    T yPrev = T(0);
-    for (int j = 0; j < nJ; ++j) {
-      const int c = j * nI + i;
-      const T pc = p[c];
-      I.x[c] += alpha * static_cast<double>(pc);
-      const T r = S.r[c] - alphaT * S.Ap[c];
-      const T lo = (j > 0) ? S.pN[c - nI] : T(0);
-      const T y = (r - lo * yPrev) * S.inv[c];
-      S.r[c] = r;
-      S.z[c] = y;
-      l1 += fabs(static_cast<double>(r));
-      yPrev = y;
+  const int ic = col ? i : nI - 1;
+  T pA[U], aA[U], rA[U], iA[U], nA[U];
+  T pB[U] = {}, aB[U] = {}, rB[U] = {}, iB[U] = {}, nB[U] = {};
+  double xA[U], xB[U] = {};
+  auto load = [&](int j0, T (&pv)[U], T (&av)[U], T (&rv)[U], T (&iv)[U], T (&nv)[U], double (&xv)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = min(j0 + u, nJ - 1) * nI + ic;
+      pv[u] = p[c];
+      av[u] = S.Ap[c];
+      rv[u] = S.r[c];
+      iv[u] = S.inv[c];
+      nv[u] = S.pN[c];
+      xv[u] = I.x[c];
     }
-    T zNext = T(0);
-    for (int j = nJ - 1; j >= 0; --j) {
-      const int c = j * nI + i;
-      const T z = S.z[c] - (S.pN[c] * S.inv[c]) * zNext;
-      S.z[c] = z;
-      rz += static_cast<double>(S.r[c]) * static_cast<double>(z);
-      zNext = z;
+  };
+  double l1 = 0.0;
+  T yPrev = T(0), nPrev = T(0);
+  load(0, pA, aA, rA, iA, nA, xA);
+  for (int j0 = 0; j0 < nJ; j0 += U) {
+    if (j0 + U < nJ) load(j0 + U, pB, aB, rB, iB, nB, xB);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u;
+      if (j < nJ) {
+        const int c = j * nI + ic;
+        const T r = rA[u] - alphaT * aA[u];
+        const T y = (r - nPrev * yPrev) * iA[u];
+        if (col) {
+          I.x[c] = xA[u] + alpha * static_cast<double>(pA[u]);
+          S.r[c] = r;
+          S.z[c] = y;
+        }
+        l1 += fabs(static_cast<double>(r));
+        yPrev = y;
+        nPrev = nA[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      pA[u] = pB[u];
+      aA[u] = aB[u];
+      rA[u] = rB[u];
+      iA[u] = iB[u];
+      nA[u] = nB[u];
+      xA[u] = xB[u];
     }
   }
-  l1 = waveSum(l1);
+  double rz = lineBack<T>(S, ic, col, nI, nJ);
+  l1 = waveSum(col ? l1 : 0.0);
   rz = waveSum(rz);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = l1;

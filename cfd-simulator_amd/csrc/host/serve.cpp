@@ -35,7 +35,22 @@ struct Server::Slot {
   double admitted{0.0};
   size_t bytes{0};
   int lastIteration{0};
+  // coefficient settling: Cl and Cd every settle window, the last three kept
+  int nextSample{0};
+  std::deque<std::pair<double, double>> samples;
 };
+
+double remainingChange(double c0, double c1, double c2) {
+  const double d1 = c1 - c0, d2 = c2 - c1;
+  if (!std::isfinite(d1) || !std::isfinite(d2)) return INFINITY;
+  if (d2 == 0.0) return 0.0;
+  if (d1 * d2 < 0.0) return std::fabs(d2);
+  if (std::fabs(d2) < std::fabs(d1)) {
+    const double rho = d2 / d1;
+    return std::fabs(d2) * rho / (1.0 - rho);
+  }
+  return INFINITY;
+}
 
 Server::Server(const ServerOptions& options) : opt_(options), t0_(std::chrono::steady_clock::now()) {
   if (opt_.maxSlots < 1 || opt_.maxSlots > 65535) throw std::invalid_argument("max_slots must be in 1..65535");
@@ -158,11 +173,15 @@ void Server::startAttempt(Slot& s, bool cold) {
   if (e->initialise(u.data(), v.data(), p.data(), rho.data(), mu.data()) != CFDGPU_OK)
     throw std::runtime_error("the solver could not start");
   Inst& h = e->hostInst();
-  h.stopTol = opt_.rule.tolerance;
+  // with coefficient settling the device stops only on divergence and at the
+  // limit; the residual test and the settling are checked between chunks
+  h.stopTol = opt_.settleWindow > 0 ? 0.0 : opt_.rule.tolerance;
   h.stopMaxIter = opt_.rule.max_iterations > 0 ? opt_.rule.max_iterations : 0;
   hipCheck(hipStreamSynchronize(e->stream()), "solver set-up");
   s.bytes = e->deviceBytes();
   s.lastIteration = 0;
+  s.nextSample = opt_.settleWindow;
+  s.samples.clear();
   s.engine = std::move(e);
 }
 
@@ -319,6 +338,13 @@ void Server::retire() {
     memberIterations_ += fl[fIteration] - s.lastIteration;
     s.lastIteration = fl[fIteration];
     if (!fl[fStopped]) {
+      if (opt_.settleWindow > 0 && settled(s, fl[fIteration])) {
+        rosterDirty_ = true;
+        cfdgpu_monitor mon{};
+        s.engine->readMonitorOn(s.engine->stream(), &mon);
+        finish(s, 0, fl[fIteration], mon.continuity);
+        continue;
+      }
       keep.push_back(std::move(active_[q]));
       continue;
     }
@@ -346,6 +372,31 @@ void Server::retire() {
     finish(s, status, fl[fIteration], mon.continuity);
   }
   active_ = std::move(keep);
+}
+
+// Coefficient settling, checked when a solve has run another settle window:
+// the residual test of the worker's rule on the last monitor, then Cl and Cd
+// of the current field (integrated on the device) against the last samples.
+bool Server::settled(Slot& s, int iteration) {
+  if (iteration < s.nextSample) return false;
+  while (s.nextSample <= iteration) s.nextSample += opt_.settleWindow;
+  const ServeRequest& r = requests_[static_cast<size_t>(s.ticket)];
+  cfdgpu_forces f{};
+  int count = 0;
+  if (s.engine->surface(r.speed, r.alphaDeg, r.density, r.referencePressure, r.chord, nullptr, 0, &count, &f) !=
+      CFDGPU_OK)
+    return false;
+  s.samples.emplace_back(f.cl, f.cd);
+  if (s.samples.size() > 3) s.samples.pop_front();
+  if (s.samples.size() < 3) return false;
+  cfdgpu_monitor mon{};
+  s.engine->readMonitorOn(s.engine->stream(), &mon);
+  const double worst = std::max(std::max(std::fabs(mon.continuity), std::fabs(mon.momentum_x)),
+                                std::fabs(mon.momentum_y));
+  if (!(worst < opt_.rule.tolerance)) return false;
+  const double rl = remainingChange(s.samples[0].first, s.samples[1].first, s.samples[2].first);
+  const double rd = remainingChange(s.samples[0].second, s.samples[1].second, s.samples[2].second);
+  return rl <= opt_.settleCl && rd <= opt_.settleCd;
 }
 
 void Server::step() {

@@ -49,7 +49,15 @@ struct ServerOptions {
   int model{CFDGPU_MODEL_SST};
   bool warmStart{true};
   bool keepFields{false};
+  int settleWindow{0};  // > 0: converged only once Cl and Cd have settled (see cfdgpu_server_options)
+  double settleCl{5e-4}, settleCd{5e-5};
 };
+
+// The change still to come of a converging sequence from its last three
+// samples spaced evenly in iterations (Aitken's geometric tail): |d2| rho /
+// (1 - rho) with rho = d2 / d1 for a monotone, shrinking change; |d2| for an
+// oscillation; infinity for a change that does not shrink.
+double remainingChange(double c0, double c1, double c2);
 
 class Server {
  public:
@@ -85,6 +93,7 @@ class Server {
   bool admitOne();
   void startAttempt(Slot& s, bool cold);
   void finish(Slot& s, int status, int iterations, double continuity);
+  bool settled(Slot& s, int iteration);
   void launchChunk();
   void retire();
 

@@ -335,6 +335,16 @@ typedef struct {
   int model;               /* CFDGPU_MODEL_*, default constants */
   int warm_start;          /* 1: warm start from converged neighbours in alpha */
   int keep_fields;         /* 1: keep each result's u, v, p, mu for cfdgpu_server_fields */
+  /* Coefficient settling (0 = off, the worker's residual rule alone). With settle_window > 0 a
+     solve converges when its residuals are below rule.tolerance and its lift and drag have
+     settled: every settle_window outer iterations Cl and Cd are integrated on the device, and
+     from the last three samples the change still to come is estimated (geometric tail, Aitken:
+     |d2| rho / (1 - rho), rho = d2 / d1; an oscillation counts |d2|; a growing change never
+     settles). Both estimates must be at most settle_cl and settle_cd. No reference
+     counterpart: the residual rule alone leaves the lift of fine near-wall grids far from its
+     converged value. */
+  int settle_window;
+  double settle_cl, settle_cd;
 } cfdgpu_server_options;
 
 typedef struct {
