@@ -37,7 +37,8 @@ EXPORTS = [
     "cfdgpu_iterate", "cfdgpu_iterate_timed", "cfdgpu_solve", "cfdgpu_batch_create", "cfdgpu_batch_destroy",
     "cfdgpu_batch_iterate", "cfdgpu_batch_iterate_timed", "cfdgpu_batch_solve", "cfdgpu_polar_sweep", "cfdgpu_polar_csv", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
     "cfdgpu_set_field", "cfdgpu_get_massflux", "cfdgpu_set_massflux", "cfdgpu_get_turbulence",
-    "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_surface",
+    "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_divergence_noalloc",
+    "cfdgpu_surface",
     "cfdgpu_sweep_angles", "cfdgpu_debug_array", "cfdgpu_server_defaults", "cfdgpu_server_create",
     "cfdgpu_server_destroy", "cfdgpu_server_submit", "cfdgpu_server_step", "cfdgpu_server_run",
     "cfdgpu_server_collect", "cfdgpu_server_fields", "cfdgpu_solver_surface",
@@ -210,6 +211,7 @@ def load(path: str = LIB_PATH):
         "cfdgpu_set_turbulence_fields": (C.c_int, [vp, D, D, D]),
         "cfdgpu_turbulence_info": (C.c_int, [vp, D, D, D]),
         "cfdgpu_divergence": (C.c_int, [vp, _dp]),
+        "cfdgpu_divergence_noalloc": (C.c_int, [vp, _dp]),
         "cfdgpu_surface": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(Forces)]),
         "cfdgpu_sweep_angles": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, C.c_int]),

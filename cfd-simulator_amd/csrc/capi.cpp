@@ -477,6 +477,12 @@ int cfdgpu_divergence(const cfdgpu_solver* s, double* out) {
   return guarded([&]() -> int { return s->engine->divergence(out); });
 }
 
+int cfdgpu_divergence_noalloc(const cfdgpu_solver* s, double* out) {
+  SOLVER_GUARD(s)
+  if (!out) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  return guarded([&]() -> int { return s->engine->divergenceNoAlloc(out); });
+}
+
 int cfdgpu_surface(const cfdgpu_mesh* m, const double* u, const double* v, const double* p, const double* mu,
                    double speed, double alpha, double density, double pref, double chord, double* rows, int capacity,
                    int* count, cfdgpu_forces* forces) {

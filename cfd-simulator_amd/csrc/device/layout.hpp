@@ -123,7 +123,8 @@ __host__ __device__ constexpr int skewLength(int nI, int nJ, int R) {
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
-constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)
+constexpr int kColThreads = 64;
+constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)      // columns per block of a column-parallel launch (one wave)
 
 struct Inst {
   int nI, nJ, N;

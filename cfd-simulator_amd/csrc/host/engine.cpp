@@ -962,6 +962,11 @@ int Engine::divergence(double* out) const {
   return CFDGPU_OK;
 }
 
+int Engine::divergenceNoAlloc(double* out) const {
+  launchDivergenceNoAlloc(dInst_, cfg_, grid_.cells(), out, h_.flags + fStopped, stream_);
+  return CFDGPU_OK;
+}
+
 int Engine::debugArray(const std::string& name, double* out, int capacity) const {
   const size_t N = static_cast<size_t>(grid_.cells());
   const double* src = nullptr;

@@ -44,6 +44,10 @@ LaunchCfg launchCfgFor(const Inst* insts, size_t count);
 // Members of one batch share everything that shapes the launch schedule.
 bool sameSchedule(const Inst& a, const Inst& b);
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
+// launchDivergence staged through the library's static device array (noalloc.hip): no allocation;
+// clears the instance's stop flag first, downloads `cells` values into the host buffer `out`
+void launchDivergenceNoAlloc(const Inst* dInst, const LaunchCfg& cfg, int cells, double* out, int* flagStopped,
+                             hipStream_t s);
 // distance from each of n centroids to the nearest of nw wall-face centres
 void launchWallDistance(const double2* centroid, int n, const double2* walls, int nw, double* out, hipStream_t s);
 struct WallStation;
@@ -126,6 +130,7 @@ class Engine {
   int setTurbulence(const double* k, const double* omega, const double* mut);
   int turbulenceInfo(double* kIn, double* wIn, double* yplus) const;
   int divergence(double* out) const;
+  int divergenceNoAlloc(double* out) const;  // the same values, staged without allocating
   // surface distribution and forces of the current field, on the device; the
   // same numbers as surfaceAndForces on the downloaded field
   int surface(double speed, double alphaDeg, double density, double pref, double chord, double* rows, int capacity,

@@ -275,6 +275,12 @@ int cfdgpu_turbulence_info(const cfdgpu_solver* solver, double* k_inflow, double
                            double* max_wall_yplus);
 /* SimpleSolver::divergence() (SimpleSolver.cpp:796-817) */
 int cfdgpu_divergence(const cfdgpu_solver* solver, double* out);
+/* cfdgpu_divergence without allocating device memory: no counterpart in the reference; for callers
+   that cannot allocate (code under HIP graph capture, tight serving loops). The field is staged in
+   a static device array of 1,048,576 doubles (8 MiB) that each GPU the library runs on holds until
+   the process ends; larger meshes are rejected with CFDGPU_INVALID_ARGUMENT. Calls that overlap on
+   one GPU share that array and race. Same values as cfdgpu_divergence. */
+int cfdgpu_divergence_noalloc(const cfdgpu_solver* solver, double* out);
 
 /* ---- post-processing, FP64 (SurfaceData.hpp:159, Forces.hpp:127-133) ----
  * rows (12 doubles per station, upper surface then lower): side(1 upper/0 lower), face, x, y,

@@ -205,6 +205,11 @@ constexpr unsigned hipStreamNonBlocking = 1;
 inline const char* hipGetErrorString(hipError_t) { return "emulation error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
+template <class T>
+inline hipError_t hipGetSymbolAddress(void** p, T* sym) { *p = static_cast<void*>(sym); return hipSuccess; }
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t bytes) {
   *p = static_cast<T*>(std::malloc(bytes));
