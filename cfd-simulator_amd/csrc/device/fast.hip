@@ -28,21 +28,30 @@
 
 namespace cfdgpu {
 
+// The line CG's arrays in its storage type T, as global-address-space
+// pointers: a generic (FLAT) access would make every wait for LDS traffic
+// also wait for all loads in flight.
 template <typename T>
 struct LineStore {
-  const T *pP, *pN, *pE, *pWk;
-  T *inv, *r, *z, *pd0, *pd1, *Ap;
+  const CFD_GLOBAL T *pP, *pN, *pE, *pWk;
+  CFD_GLOBAL T *inv, *r, *z, *pd0, *pd1, *Ap;
 };
 
+template <typename T>
+__device__ __forceinline__ LineStore<T> lineStoreOf(const T* pP, const T* pN, const T* pE, const T* pWk, T* inv, T* r,
+                                                  T* z, T* pd0, T* pd1, T* Ap) {
+  return LineStore<T>{gmem(pP), gmem(pN), gmem(pE), gmem(pWk), gmem(inv), gmem(r), gmem(z), gmem(pd0), gmem(pd1),
+                      gmem(Ap)};
+}
 template <typename T>
 __device__ __forceinline__ LineStore<T> lineStore(const Inst& I);
 template <>
 __device__ __forceinline__ LineStore<double> lineStore<double>(const Inst& I) {
-  return LineStore<double>{I.pP, I.pN, I.pE, I.pWk, I.lInv, I.r, I.z, I.pd0, I.pd1, I.Ap};
+  return lineStoreOf<double>(I.pP, I.pN, I.pE, I.pWk, I.lInv, I.r, I.z, I.pd0, I.pd1, I.Ap);
 }
 template <>
 __device__ __forceinline__ LineStore<float> lineStore<float>(const Inst& I) {
-  return LineStore<float>{I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp};
+  return lineStoreOf<float>(I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp);
 }
 
 __device__ __forceinline__ double waveSum(double v) {
@@ -147,7 +156,9 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
   }
   __syncthreads();
   const double mean = demean ? bc[0] / static_cast<double>(I.N) : 0.0;
-  const double* src = (pass == 0) ? I.pb : I.rb;
+  const CFD_GLOBAL double* src = gmem(static_cast<const double*>((pass == 0) ? I.pb : I.rb));
+  CFD_GLOBAL double* pb = gmem(I.pb);
+  CFD_GLOBAL double* xg = gmem(I.x);
   const int ic = col ? i : nI - 1;  // idle lanes read a real column and store nothing
   double bA[U], bB[U] = {};
   T dA[U], nA[U], dB[U] = {}, nB[U] = {};
@@ -173,7 +184,7 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
         double b = bA[u];
         if (demean) {
           b -= mean;
-          if (col) I.pb[c] = b;
+          if (col) pb[c] = b;
         }
         const T r = static_cast<T>(b);
         const T lo = nPrev;  // the coupling with the row below (zero on row 0)
@@ -184,7 +195,7 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
           S.inv[c] = inv;
           S.r[c] = r;
           S.z[c] = y;
-          I.x[c] = 0.0;
+          xg[c] = 0.0;
         }
         l1 += fabs(static_cast<double>(r));
         cPrev = nA[u] * inv;  // zero on the top row
@@ -215,16 +226,58 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
 // Iteration `it`, launch A (cell-parallel): the test of iteration it - 1
 // from the column partials (|r|_1 <= tol |r0|_1, then the r.z guard), beta,
 // p = z + beta p_old (neighbours recomputed locally), Ap and p.Ap.
+// Every operand is read through the global address space, without a branch
+// around its load (a missing neighbour is read at the cell itself and its
+// coefficient masked to zero), and issued before the stop flags are tested
+// and the partials summed, so the stencil's loads and the partial sums are
+// one memory round trip; only the lanes of row 0 wait again for their wake
+// partner.
 template <typename T>
 __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.nBlocks) return;
-  if (I.flags[fStopped] || I.flags[fCgDone]) return;
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.nBlocks);
-  const int c = tile * kBlock + static_cast<int>(threadIdx.x);
-  const bool live = c < I.N;
+  const int c0 = tile * kBlock + static_cast<int>(threadIdx.x);
+  const bool live = c0 < I.N;
   const int nI = I.nI, nJ = I.nJ;
+  const int c = live ? c0 : 0;
+  const int i = c % nI, j = c / nI;
   const LineStore<T> S = lineStore<T>(I);
+  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
+  const CFD_GLOBAL double* red = gmem(static_cast<const double*>(I.red));
+  const CFD_GLOBAL T* z = S.z;
+  const CFD_GLOBAL T* pOld = (it & 1) ? S.pd0 : S.pd1;
+  const CFD_GLOBAL T* aPg = S.pP;
+  const CFD_GLOBAL T* aNg = S.pN;
+  const CFD_GLOBAL T* aEg = S.pE;
+  const int cS = (j > 0) ? c - nI : c;
+  const int cN = (j < nJ - 1) ? c + nI : c;
+  const int cW = (i > 0) ? c - 1 : c;
+  const int cE = (i < nI - 1) ? c + 1 : c;
+  const int pr = gmem(static_cast<const int*>(I.partner))[i];
+  const int stopped = flagsG[fStopped];
+  const int cgDone = flagsG[fCgDone];
+  const double norm0 = red[sCgNorm0];
+  const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
+  const T zc = z[c], zS = z[cS], zN = z[cN], zW = z[cW], zE = z[cE];
+  T oc = T(0), oS = T(0), oN = T(0), oW = T(0), oE = T(0);
+  if (it > 0) {
+    oc = pOld[c];
+    oS = pOld[cS];
+    oN = pOld[cN];
+    oW = pOld[cW];
+    oE = pOld[cE];
+  }
+  const T dP = aPg[c], nBelow = aNg[cS], nAbove = aNg[c], eLeft = aEg[cW], eRight = aEg[c];
+  const T wake = S.pWk[c];
+  if (stopped || cgDone) return;  // the whole block: the flags are uniform
+  // row 0: the wake partner's values (a second round trip for those lanes only)
+  T zPr = T(0), oPr = T(0);
+  const bool hasWake = live && j == 0 && pr >= 0;
+  if (hasWake) {
+    zPr = z[pr];
+    if (it > 0) oPr = pOld[pr];
+  }
   double sums[2];
   {
     const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
@@ -236,38 +289,36 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   if (it == 0) {
     if (!(nrm > 0.0)) done = true;  // already solved (LinearSystem.cpp:144-149)
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-      I.red[sCgNorm0] = nrm;
-      I.red[sCgRz0] = rzNew;
+      gmem(I.red)[sCgNorm0] = nrm;
+      gmem(I.red)[sCgRz0] = rzNew;
     }
   } else {
-    const double rzOld = I.red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
-    if (nrm <= I.cgTol * I.red[sCgNorm0]) done = true;
+    if (nrm <= I.cgTol * norm0) done = true;
     else if (!(fabs(rzOld) > 0.0)) done = true;
     else beta = rzNew / rzOld;
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) I.red[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) gmem(I.red)[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
   }
   if (done) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    if (threadIdx.x == 0 && blockIdx.x == 0) gmem(I.flags)[fCgDone] = 1;
     return;
   }
-  const T betaT = static_cast<T>(beta);
-  const T* pOld = (it & 1) ? S.pd0 : S.pd1;
-  auto pAt = [&](int q) -> T { return (it == 0) ? S.z[q] : static_cast<T>(S.z[q] + betaT * pOld[q]); };
+  const T b = static_cast<T>(beta);
+  auto pv = [&](T zq, T oq) -> T { return (it == 0) ? zq : static_cast<T>(zq + b * oq); };
+  // masked couplings: zero where the neighbour is missing
+  const T aS = (j > 0) ? nBelow : (hasWake ? wake : T(0));
+  const T aN = (j < nJ - 1) ? nAbove : T(0);
+  const T aW = (i > 0) ? eLeft : T(0);
+  const T aE = (i < nI - 1) ? eRight : T(0);
+  const T pc = pv(zc, oc);
+  const T pS = (j > 0) ? pv(zS, oS) : pv(zPr, oPr);
+  // diagonal, then below (or the wake partner), above, left, right
+  T y = dP * pc;
+  y += aS * pS;
+  y += aN * pv(zN, oN);
+  y += aW * pv(zW, oW);
+  y += aE * pv(zE, oE);
   double part[1] = {0.0};
   if (live) {
-    const int i = c % nI, j = c / nI;
-    const T pc = pAt(c);
-    // diagonal, then below (or the wake partner), above, left, right
-    T y = S.pP[c] * pc;
-    if (j > 0) {
-      y += S.pN[c - nI] * pAt(c - nI);
-    } else {
-      const int pr = I.partner[i];
-      if (pr >= 0) y += S.pWk[c] * pAt(pr);
-    }
-    if (j < nJ - 1) y += S.pN[c] * pAt(c + nI);
-    if (i > 0) y += S.pE[c - 1] * pAt(c - 1);
-    if (i < nI - 1) y += S.pE[c] * pAt(c + 1);
     ((it & 1) ? S.pd1 : S.pd0)[c] = pc;
     S.Ap[c] = y;
     part[0] = static_cast<double>(pc) * static_cast<double>(y);
@@ -296,7 +347,8 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
   }
   const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
   const T alphaT = static_cast<T>(alpha);
-  const T* p = (it & 1) ? S.pd1 : S.pd0;
+  const CFD_GLOBAL T* p = (it & 1) ? S.pd1 : S.pd0;
+  CFD_GLOBAL double* xg = gmem(I.x);
   const int ic = col ? i : nI - 1;
   T pA[U], aA[U], rA[U], iA[U], nA[U];
   T pB[U] = {}, aB[U] = {}, rB[U] = {}, iB[U] = {}, nB[U] = {};
@@ -310,7 +362,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
       rv[u] = S.r[c];
       iv[u] = S.inv[c];
       nv[u] = S.pN[c];
-      xv[u] = I.x[c];
+      xv[u] = xg[c];
     }
   };
   double l1 = 0.0;
@@ -326,7 +378,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
         const T r = rA[u] - alphaT * aA[u];
         const T y = (r - nPrev * yPrev) * iA[u];
         if (col) {
-          I.x[c] = xA[u] + alpha * static_cast<double>(pA[u]);
+          xg[c] = xA[u] + alpha * static_cast<double>(pA[u]);
           S.r[c] = r;
           S.z[c] = y;
         }
