@@ -126,3 +126,25 @@ def test_sst_512x128_polar_point(G, gpu_available, name):
         return
     first, f, rows = run_matched(G, s, fx)
     compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("name", fixtures("c4_"))
+def test_mixed_batch_coarse_instance(G, gpu_available, name):
+    """The 256 x 64 instances of the mixed batch: the oracle's status
+    (converged, iteration-limit or diverged; a diverged cold solve repeats
+    itself exactly, so the oracle ran it once) and, for converged ones, the
+    coefficients at the oracle's stop."""
+    fx = load(name)
+    print(f"case: {name}: NACA {fx['section']} alpha {fx['alpha']:.3f} Re {fx['reynolds']:.4g}, oracle status "
+          f"{fx['status']} after {fx['iterations']}", flush=True)
+    mesh, s = build(G, fx)
+    if fx["status"] != 0:
+        rep = s.solve(fx["tolerance"], fx["limit"], check_every=100)
+        print(f"device status {rep.status} after {rep.iterations}", flush=True)
+        assert rep.status == fx["status"]
+        if fx["status"] == 2:
+            assert abs(rep.iterations - fx["iterations"]) <= 2
+        return
+    first, f, rows = run_matched(G, s, fx)
+    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5)

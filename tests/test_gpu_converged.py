@@ -65,3 +65,15 @@ def test_converged_sst_forces_match_the_oracle(G, O, gpu_available, section, alp
     for name in ("upper_separation", "lower_separation", "stagnation_xc"):
         a, b = getattr(fg, name), getattr(fo, name)
         assert (a < 0.0) == (b < 0.0) and abs(a - b) <= 1e-4, name
+    # the same separation wall face (the first shear sign change, + to -, away from
+    # stagnation, SurfaceData.cpp:179-204, found in the oracle's station rows)
+    for side, name in ((1, "upper_separation_face"), (0, "lower_separation_face")):
+        st = ro[ro[:, 0] == side]
+        face = -1
+        for q in range(1, len(st)):
+            if st[q, 10] or st[q - 1, 10]:
+                continue
+            if st[q - 1, 8] > 0.0 and st[q, 8] < 0.0:
+                face = int(st[q, 1])
+                break
+        assert getattr(fg, name) == face, name

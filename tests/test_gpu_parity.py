@@ -98,3 +98,24 @@ def test_sst_iterations_track_the_oracle(G, O, gpu_available):
     assert rel_err(ug, uo) < 1e-9
     assert rel_err(pg, po) < 1e-9
     assert rel_err(mug, muo) < 1e-8
+
+
+def test_non_orthogonal_correctors_track_the_oracle(G, O, gpu_available):
+    """SimpleSolver.cpp:716-748 with nonOrthogonalCorrectors = 2: two pressure
+    assemblies, solves and corrections per outer iteration."""
+    gm, om = cgrid_pair(G, O, **SMALL)
+    kind, vel, pres = G.face_conditions(gm, 1.0, 3.0)
+    field = G.uniform_field(gm.cells, speed=1.0, alpha_deg=3.0, density=1.0, reynolds=500.0)
+    gset = G.default_settings(non_orthogonal_correctors=2)
+    oset = O.default_settings()
+    oset.non_orthogonal_correctors = 2
+    gs, os_ = solver_pair(G, O, gm, om, kind, vel, pres, field, gsettings=gset, osettings=oset)
+    print("case: NACA0012 118x27 laminar Re=500 alpha=3, two correctors, five iterations", flush=True)
+    for it in range(5):
+        mg = gs.iterate()
+        mo = os_.iterate()
+        assert mg.pressure_iterations == mo.pressure_iterations, it
+        assert abs(mg.continuity - mo.continuity) <= 1e-8 * abs(mo.continuity), it
+    for a, b in zip(gs.field()[:3], os_.field()[:3]):
+        assert rel_err(a, b) < 1e-9
+    assert rel_err(gs.mass_flux(), os_.flux()) < 1e-9

@@ -76,3 +76,18 @@ def test_laminar_closure_is_indistinguishable_from_no_closure():
     b.iterate(40)
     for x, y in zip(a.field(), b.field()):
         assert np.array_equal(x, y)
+
+
+def test_wall_distance_is_the_nearest_wall_face_centre():
+    """TurbulenceModel.cpp:28-65: each cell's distance to the nearest wall-face
+    centre, by brute force over all pairs (NumPy, independent of the oracle's
+    loop): equal to the last bit, since both take the minimum of squared
+    distances and the square root last."""
+    mesh, _, _, s = sst_small_cgrid(O)
+    t = s.turbulence()
+    walls = mesh.fcentre[mesh.boundary == O.WALL]
+    assert len(walls) > 10
+    dx = mesh.centroid[:, None, 0] - walls[None, :, 0]
+    dy = mesh.centroid[:, None, 1] - walls[None, :, 1]
+    brute = np.sqrt(np.min(dx * dx + dy * dy, axis=1))
+    assert np.array_equal(t["wall_distance"], brute)
