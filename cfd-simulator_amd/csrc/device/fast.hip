@@ -90,7 +90,7 @@ __device__ __forceinline__ double waveSum(double v) {
 // the next block are loaded before the current block is computed (rows past
 // the end are clamped to the last row and not used), so the recurrence waits
 // for memory once per block instead of once per row.
-constexpr int kColUnroll = 8;
+constexpr int kColUnroll = 16;
 
 // Back substitution z_j = y_j - (pN_j inv_j) z_{j+1}, j = nJ-1 .. 0, in place
 // over z (which holds y); returns the lane's sum of r_j z_j in FP64.
@@ -226,57 +226,68 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
 // Iteration `it`, launch A (cell-parallel): the test of iteration it - 1
 // from the column partials (|r|_1 <= tol |r0|_1, then the r.z guard), beta,
 // p = z + beta p_old (neighbours recomputed locally), Ap and p.Ap.
-// Every operand is read through the global address space, without a branch
-// around its load (a missing neighbour is read at the cell itself and its
-// coefficient masked to zero), and issued before the stop flags are tested
-// and the partials summed, so the stencil's loads and the partial sums are
-// one memory round trip; only the lanes of row 0 wait again for their wake
-// partner.
+// Each thread takes kCgCells cells (c = tile * kBlock * kCgCells + q * kBlock
+// + threadIdx.x). Every operand is read through the global address space,
+// without a branch around its load (a missing neighbour is read at the cell
+// itself and its coefficient masked to zero), and issued before the stop
+// flags are tested and the partials summed, so the stencil's loads and the
+// partial sums are one memory round trip; only the lanes of row 0 wait again
+// for their wake partner.
 template <typename T>
 __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
+  constexpr int CELLS = kCgCells;
   const Inst& I = insts[blockIdx.y];
-  if (static_cast<int>(blockIdx.x) >= I.nBlocks) return;
-  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.nBlocks);
-  const int c0 = tile * kBlock + static_cast<int>(threadIdx.x);
-  const bool live = c0 < I.N;
+  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  if (it >= I.cgMax) return;  // a batch member with a smaller cap than the batch's
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
+  const int base = tile * kBlock * CELLS + static_cast<int>(threadIdx.x);
   const int nI = I.nI, nJ = I.nJ;
-  const int c = live ? c0 : 0;
-  const int i = c % nI, j = c / nI;
   const LineStore<T> S = lineStore<T>(I);
   const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
   const CFD_GLOBAL double* red = gmem(static_cast<const double*>(I.red));
+  const CFD_GLOBAL int* partnerG = gmem(static_cast<const int*>(I.partner));
   const CFD_GLOBAL T* z = S.z;
   const CFD_GLOBAL T* pOld = (it & 1) ? S.pd0 : S.pd1;
-  const CFD_GLOBAL T* aPg = S.pP;
-  const CFD_GLOBAL T* aNg = S.pN;
-  const CFD_GLOBAL T* aEg = S.pE;
-  const int cS = (j > 0) ? c - nI : c;
-  const int cN = (j < nJ - 1) ? c + nI : c;
-  const int cW = (i > 0) ? c - 1 : c;
-  const int cE = (i < nI - 1) ? c + 1 : c;
-  const int pr = gmem(static_cast<const int*>(I.partner))[i];
+  int cq[CELLS], iq[CELLS], jq[CELLS], prq[CELLS];
+  bool live[CELLS];
+  T zc[CELLS], zS[CELLS], zN[CELLS], zW[CELLS], zE[CELLS];
+  T oc[CELLS], oS[CELLS], oN[CELLS], oW[CELLS], oE[CELLS];
+  T dP[CELLS], nB[CELLS], nA[CELLS], eL[CELLS], eR[CELLS], wk[CELLS];
+#pragma unroll
+  for (int q = 0; q < CELLS; ++q) {
+    const int c0 = base + q * kBlock;
+    live[q] = c0 < I.N;
+    const int c = live[q] ? c0 : 0;
+    const int i = c % nI, j = c / nI;
+    cq[q] = c;
+    iq[q] = i;
+    jq[q] = j;
+    prq[q] = partnerG[i];
+    const int cS = (j > 0) ? c - nI : c;
+    const int cN = (j < nJ - 1) ? c + nI : c;
+    const int cW = (i > 0) ? c - 1 : c;
+    const int cE = (i < nI - 1) ? c + 1 : c;
+    zc[q] = z[c]; zS[q] = z[cS]; zN[q] = z[cN]; zW[q] = z[cW]; zE[q] = z[cE];
+    oc[q] = oS[q] = oN[q] = oW[q] = oE[q] = T(0);
+    if (it > 0) { oc[q] = pOld[c]; oS[q] = pOld[cS]; oN[q] = pOld[cN]; oW[q] = pOld[cW]; oE[q] = pOld[cE]; }
+    dP[q] = S.pP[c]; nB[q] = S.pN[cS]; nA[q] = S.pN[c]; eL[q] = S.pE[cW]; eR[q] = S.pE[c]; wk[q] = S.pWk[c];
+  }
   const int stopped = flagsG[fStopped];
   const int cgDone = flagsG[fCgDone];
   const double norm0 = red[sCgNorm0];
   const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
-  const T zc = z[c], zS = z[cS], zN = z[cN], zW = z[cW], zE = z[cE];
-  T oc = T(0), oS = T(0), oN = T(0), oW = T(0), oE = T(0);
-  if (it > 0) {
-    oc = pOld[c];
-    oS = pOld[cS];
-    oN = pOld[cN];
-    oW = pOld[cW];
-    oE = pOld[cE];
-  }
-  const T dP = aPg[c], nBelow = aNg[cS], nAbove = aNg[c], eLeft = aEg[cW], eRight = aEg[c];
-  const T wake = S.pWk[c];
   if (stopped || cgDone) return;  // the whole block: the flags are uniform
   // row 0: the wake partner's values (a second round trip for those lanes only)
-  T zPr = T(0), oPr = T(0);
-  const bool hasWake = live && j == 0 && pr >= 0;
-  if (hasWake) {
-    zPr = z[pr];
-    if (it > 0) oPr = pOld[pr];
+  T zPr[CELLS], oPr[CELLS];
+  bool wake[CELLS];
+#pragma unroll
+  for (int q = 0; q < CELLS; ++q) {
+    wake[q] = live[q] && jq[q] == 0 && prq[q] >= 0;
+    zPr[q] = oPr[q] = T(0);
+    if (wake[q]) {
+      zPr[q] = z[prq[q]];
+      if (it > 0) oPr[q] = pOld[prq[q]];
+    }
   }
   double sums[2];
   {
@@ -303,25 +314,29 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
     return;
   }
   const T b = static_cast<T>(beta);
-  auto pv = [&](T zq, T oq) -> T { return (it == 0) ? zq : static_cast<T>(zq + b * oq); };
-  // masked couplings: zero where the neighbour is missing
-  const T aS = (j > 0) ? nBelow : (hasWake ? wake : T(0));
-  const T aN = (j < nJ - 1) ? nAbove : T(0);
-  const T aW = (i > 0) ? eLeft : T(0);
-  const T aE = (i < nI - 1) ? eRight : T(0);
-  const T pc = pv(zc, oc);
-  const T pS = (j > 0) ? pv(zS, oS) : pv(zPr, oPr);
-  // diagonal, then below (or the wake partner), above, left, right
-  T y = dP * pc;
-  y += aS * pS;
-  y += aN * pv(zN, oN);
-  y += aW * pv(zW, oW);
-  y += aE * pv(zE, oE);
+  auto pv = [&](T zv, T ov) -> T { return (it == 0) ? zv : static_cast<T>(zv + b * ov); };
   double part[1] = {0.0};
-  if (live) {
-    ((it & 1) ? S.pd1 : S.pd0)[c] = pc;
-    S.Ap[c] = y;
-    part[0] = static_cast<double>(pc) * static_cast<double>(y);
+#pragma unroll
+  for (int q = 0; q < CELLS; ++q) {
+    const int i = iq[q], j = jq[q];
+    // masked couplings: zero where the neighbour is missing
+    const T aS = (j > 0) ? nB[q] : (wake[q] ? wk[q] : T(0));
+    const T aN = (j < nJ - 1) ? nA[q] : T(0);
+    const T aW = (i > 0) ? eL[q] : T(0);
+    const T aE = (i < nI - 1) ? eR[q] : T(0);
+    const T pc = pv(zc[q], oc[q]);
+    const T pS = (j > 0) ? pv(zS[q], oS[q]) : pv(zPr[q], oPr[q]);
+    // diagonal, then below (or the wake partner), above, left, right
+    T y = dP[q] * pc;
+    y += aS * pS;
+    y += aN * pv(zN[q], oN[q]);
+    y += aW * pv(zW[q], oW[q]);
+    y += aE * pv(zE[q], oE[q]);
+    if (live[q]) {
+      ((it & 1) ? S.pd1 : S.pd0)[cq[q]] = pc;
+      S.Ap[cq[q]] = y;
+      part[0] += static_cast<double>(pc) * static_cast<double>(y);
+    }
   }
   blockSum<1>(part, sm);
   if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
@@ -335,9 +350,10 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
   LCG_COLUMN_PROLOGUE
   constexpr int U = kColUnroll;
   __shared__ double bc[1];
+  if (it >= I.cgMax) return;
   if (I.flags[fCgDone]) return;
   const LineStore<T> S = lineStore<T>(I);
-  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), I.nBlocks);
+  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), I.cgBlocks);
   if (threadIdx.x == 0) bc[0] = papLane;
   __syncthreads();
   const double pap = bc[0];

@@ -120,25 +120,25 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     for (int rp = 0; rp < refine; ++rp) {
       if (line && mixed) {
         hipLaunchKernelGGL(k_lcg_init_f32, colGrid, col, 0, s, d, rp);
-        for (int it = 0; it < h.cgMax; ++it) {
-          hipLaunchKernelGGL(k_lcg_a_f32, grid, block, 0, s, d, it);
+        for (int it = 0; it < cfg.cgIters; ++it) {
+          hipLaunchKernelGGL(k_lcg_a_f32, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_lcg_b_f32, colGrid, col, 0, s, d, it);
         }
       } else if (line) {
         hipLaunchKernelGGL(k_lcg_init, colGrid, col, 0, s, d, rp);
-        for (int it = 0; it < h.cgMax; ++it) {
-          hipLaunchKernelGGL(k_lcg_a, grid, block, 0, s, d, it);
+        for (int it = 0; it < cfg.cgIters; ++it) {
+          hipLaunchKernelGGL(k_lcg_a, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_lcg_b, colGrid, col, 0, s, d, it);
         }
       } else if (mixed) {
         hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d, rp);
-        for (int it = 0; it < h.cgMax; ++it) {
+        for (int it = 0; it < cfg.cgIters; ++it) {
           hipLaunchKernelGGL(k_cg_a_f32, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_cg_b_f32, cgGrid, block, 0, s, d, it);
         }
       } else {
         hipLaunchKernelGGL(k_cg_init, cgGrid, block, 0, s, d, rp);
-        for (int it = 0; it < h.cgMax; ++it) {
+        for (int it = 0; it < cfg.cgIters; ++it) {
           hipLaunchKernelGGL(k_cg_a, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_cg_b, cgGrid, block, 0, s, d, it);
         }

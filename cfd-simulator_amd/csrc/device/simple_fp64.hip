@@ -1012,6 +1012,7 @@ __device__ __forceinline__ void cgA(const Inst* insts, int it, double* sm) {
   constexpr int THREADS = kBlock;
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  if (it >= I.cgMax) return;  // a batch member with a smaller cap than the batch's
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
   const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
   const int nI = I.nI;
@@ -1141,6 +1142,7 @@ template <typename T>
 __device__ __forceinline__ void cgB(const Inst* insts, int it, double* sm) {
   const Inst& I = insts[blockIdx.y];
   if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  if (it >= I.cgMax) return;
   const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
   const int base = tile * kBlock * kCgCells + static_cast<int>(threadIdx.x);
   const CgStore<T> S = cgStore<T>(I);

@@ -342,6 +342,7 @@ void Engine::fillInstScalars() {
   h_.alphaT = settings_.turbulence_relaxation;
   h_.momSweeps = settings_.momentum_sweeps;
   h_.cgMax = settings_.pressure_iterations;
+  cfg_.cgIters = h_.cgMax;
   h_.cgTol = settings_.pressure_tolerance;
   h_.nonOrthPasses = std::max(1, settings_.non_orthogonal_correctors);
   h_.gsOrder = settings_.gs_ordering;
@@ -659,31 +660,32 @@ bool IterationGraph::Key::operator==(const Key& o) const {
   return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && maxColBlocks == o.maxColBlocks &&
          batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
          precision == o.precision && pSolver == o.pSolver && refinePasses == o.refinePasses &&
-         momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgMax == o.cgMax && hasPRef == o.hasPRef &&
+         momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgIters == o.cgIters && hasPRef == o.hasPRef &&
          model == o.model;
 }
 
 IterationGraph::Key IterationGraph::keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h) {
   return Key{d,           cfg.maxBlocks,   cfg.maxCgBlocks, cfg.maxColBlocks, cfg.batch, cfg.gsThreads,
              cfg.gsLds,   h.gsOrder,       h.precision,     h.pSolver,        h.refinePasses, h.momSweeps,
-             h.nonOrthPasses, h.cgMax, h.hasPRef, h.model};
+             h.nonOrthPasses, cfg.cgIters, h.hasPRef, h.model};
 }
 
 LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
-  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, maxNI = 0;
+  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, maxNI = 0, cgIters = 0;
   for (size_t q = 0; q < count; ++q) {
     const Inst& b = insts[q];
+    cgIters = std::max(cgIters, b.cgMax);
     lanes = std::max(lanes, b.gsR);
     blocks = std::max(blocks, b.nBlocks);
     cgBlocks = std::max(cgBlocks, b.cgBlocks);
     colBlocks = std::max(colBlocks, b.colBlocks);
     maxNI = std::max(maxNI, b.nI);
   }
-  return LaunchCfg{blocks, cgBlocks, colBlocks, static_cast<int>(count), lanes, gsLdsFor(0, maxNI, lanes)};
+  return LaunchCfg{blocks, cgBlocks, colBlocks, cgIters, static_cast<int>(count), lanes, gsLdsFor(0, maxNI, lanes)};
 }
 
 bool sameSchedule(const Inst& b, const Inst& a) {
-  return b.model == a.model && b.momSweeps == a.momSweeps && b.cgMax == a.cgMax &&
+  return b.model == a.model && b.momSweeps == a.momSweeps &&
          b.nonOrthPasses == a.nonOrthPasses && b.gsOrder == a.gsOrder && b.precision == a.precision &&
          b.pSolver == a.pSolver && b.refinePasses == a.refinePasses && b.hasPRef == a.hasPRef;
 }

@@ -25,6 +25,8 @@ struct LaunchCfg {
   int maxBlocks;   // grid.x for cell-parallel kernels
   int maxCgBlocks; // grid.x for the CG kernels
   int maxColBlocks; // grid.x for the column-parallel kernels (line-preconditioned CG)
+  int cgIters;      // CG iterations launched per solve: the largest cap of the batch (a member
+                    // with a smaller cap stops its CG at its own)
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
@@ -88,7 +90,7 @@ class IterationGraph {
     const Inst* d;
     int maxBlocks, maxCgBlocks, maxColBlocks, batch, gsThreads;
     size_t gsLds;
-    int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgMax, hasPRef, model;
+    int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgIters, hasPRef, model;
     bool operator==(const Key& o) const;
   };
   static Key keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h);

@@ -87,12 +87,12 @@ int Server::submit(const ServeRequest& r) {
   if (r.ownSettings) {
     const cfdgpu_settings& a = r.settings;
     const cfdgpu_settings& b = opt_.settings;
-    if (a.momentum_sweeps != b.momentum_sweeps || a.pressure_iterations != b.pressure_iterations ||
+    if (a.momentum_sweeps != b.momentum_sweeps ||
         a.non_orthogonal_correctors != b.non_orthogonal_correctors || a.gs_ordering != b.gs_ordering ||
         a.precision != b.precision || a.pressure_solver != b.pressure_solver ||
         a.refinement_passes != b.refinement_passes)
-      throw std::invalid_argument("a request's sweeps, CG cap, correctors, ordering, precision and pressure solver "
-                                  "must be the server's (they shape the shared launches)");
+      throw std::invalid_argument("a request's sweeps, correctors, ordering, precision, pressure solver and "
+                                  "refinement passes must be the server's (they shape the shared launches)");
   }
   const int ticket = static_cast<int>(requests_.size());
   requests_.push_back(r);
