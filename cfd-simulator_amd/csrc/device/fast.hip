@@ -34,24 +34,24 @@ namespace cfdgpu {
 template <typename T>
 struct LineStore {
   const CFD_GLOBAL T *pP, *pN, *pE, *pWk;
-  CFD_GLOBAL T *inv, *r, *z, *pd0, *pd1, *Ap;
+  CFD_GLOBAL T *inv, *up, *r, *z, *pd0, *pd1, *Ap;
 };
 
 template <typename T>
-__device__ __forceinline__ LineStore<T> lineStoreOf(const T* pP, const T* pN, const T* pE, const T* pWk, T* inv, T* r,
-                                                  T* z, T* pd0, T* pd1, T* Ap) {
-  return LineStore<T>{gmem(pP), gmem(pN), gmem(pE), gmem(pWk), gmem(inv), gmem(r), gmem(z), gmem(pd0), gmem(pd1),
-                      gmem(Ap)};
+__device__ __forceinline__ LineStore<T> lineStoreOf(const T* pP, const T* pN, const T* pE, const T* pWk, T* inv, T* up,
+                                                  T* r, T* z, T* pd0, T* pd1, T* Ap) {
+  return LineStore<T>{gmem(pP), gmem(pN), gmem(pE), gmem(pWk), gmem(inv), gmem(up), gmem(r), gmem(z), gmem(pd0),
+                      gmem(pd1), gmem(Ap)};
 }
 template <typename T>
 __device__ __forceinline__ LineStore<T> lineStore(const Inst& I);
 template <>
 __device__ __forceinline__ LineStore<double> lineStore<double>(const Inst& I) {
-  return lineStoreOf<double>(I.pP, I.pN, I.pE, I.pWk, I.lInv, I.r, I.z, I.pd0, I.pd1, I.Ap);
+  return lineStoreOf<double>(I.pP, I.pN, I.pE, I.pWk, I.lInv, I.lUp, I.r, I.z, I.pd0, I.pd1, I.Ap);
 }
 template <>
 __device__ __forceinline__ LineStore<float> lineStore<float>(const Inst& I) {
-  return lineStoreOf<float>(I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp);
+  return lineStoreOf<float>(I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.flUp, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp);
 }
 
 __device__ __forceinline__ double waveSum(double v) {
@@ -92,47 +92,40 @@ __device__ __forceinline__ double waveSum(double v) {
 // for memory once per block instead of once per row.
 constexpr int kColUnroll = 16;
 
-// Back substitution z_j = y_j - (pN_j inv_j) z_{j+1}, j = nJ-1 .. 0, in place
-// over z (which holds y); returns the lane's sum of r_j z_j in FP64.
+// Back substitution z_j = y_j - up_j z_{j+1}, j = nJ-1 .. 0, in place over z
+// (which holds y), with up_j = pN_j / pivot_j stored by the factorisation.
 template <typename T>
-__device__ __forceinline__ double lineBack(const LineStore<T>& S, int ic, bool col, int nI, int nJ) {
+__device__ __forceinline__ void lineBack(const LineStore<T>& S, int ic, bool col, int nI, int nJ) {
   constexpr int U = kColUnroll;
-  T yA[U], nA[U], iA[U], rA[U];
-  T yB[U] = {}, nB[U] = {}, iB[U] = {}, rB[U] = {};
-  auto load = [&](int top, T (&y)[U], T (&n)[U], T (&iv)[U], T (&r)[U]) {
+  T yA[U], uA[U];
+  T yB[U] = {}, uB[U] = {};
+  auto load = [&](int top, T (&y)[U], T (&up)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int c = max(top - u, 0) * nI + ic;
       y[u] = S.z[c];
-      n[u] = S.pN[c];
-      iv[u] = S.inv[c];
-      r[u] = S.r[c];
+      up[u] = S.up[c];
     }
   };
-  double rz = 0.0;
   T zNext = T(0);
-  load(nJ - 1, yA, nA, iA, rA);
+  load(nJ - 1, yA, uA);
   for (int top = nJ - 1; top >= 0; top -= U) {
-    if (top - U >= 0) load(top - U, yB, nB, iB, rB);
+    if (top - U >= 0) load(top - U, yB, uB);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int j = top - u;
       if (j >= 0) {
-        const T z = yA[u] - (nA[u] * iA[u]) * zNext;
+        const T z = yA[u] - uA[u] * zNext;
         if (col) S.z[j * nI + ic] = z;
-        rz += static_cast<double>(rA[u]) * static_cast<double>(z);
         zNext = z;
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       yA[u] = yB[u];
-      nA[u] = nB[u];
-      iA[u] = iB[u];
-      rA[u] = rB[u];
+      uA[u] = uB[u];
     }
   }
-  return col ? rz : 0.0;
 }
 
 // Start of pass `pass`: r = b (pass 0: the assembled source, its mean removed
@@ -171,7 +164,10 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
       n[u] = S.pN[c];
     }
   };
-  double l1 = 0.0;
+  // r.z of z = M^-1 r accumulated in the forward substitution: with
+  // w_j = r_j - l_j y_{j-1} (the unit lower factor's solve) and y_j = w_j /
+  // pivot_j, r^T M^-1 r = sum_j w_j^2 / pivot_j = sum_j y_j w_j
+  double l1 = 0.0, rz = 0.0;
   T cPrev = T(0), yPrev = T(0), nPrev = T(0);
   load(0, bA, dA, nA);
   for (int j0 = 0; j0 < nJ; j0 += U) {
@@ -190,15 +186,18 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
         const T lo = nPrev;  // the coupling with the row below (zero on row 0)
         const T den = dA[u] - lo * cPrev;
         const T inv = (fabs(static_cast<double>(den)) > 0.0) ? T(1) / den : T(1);
-        const T y = (r - lo * yPrev) * inv;
+        const T w = r - lo * yPrev;
+        const T y = w * inv;
+        cPrev = nA[u] * inv;  // zero on the top row
         if (col) {
           S.inv[c] = inv;
+          S.up[c] = cPrev;
           S.r[c] = r;
           S.z[c] = y;
           xg[c] = 0.0;
         }
         l1 += fabs(static_cast<double>(r));
-        cPrev = nA[u] * inv;  // zero on the top row
+        rz += static_cast<double>(y) * static_cast<double>(w);
         yPrev = y;
         nPrev = nA[u];
       }
@@ -210,9 +209,9 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
       nA[u] = nB[u];
     }
   }
-  double rz = lineBack<T>(S, ic, col, nI, nJ);
+  lineBack<T>(S, ic, col, nI, nJ);
   l1 = waveSum(col ? l1 : 0.0);
-  rz = waveSum(rz);
+  rz = waveSum(col ? rz : 0.0);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = l1;
     partialSlot(I, sCgRz)[blockIdx.x] = rz;
@@ -381,7 +380,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
       xv[u] = xg[c];
     }
   };
-  double l1 = 0.0;
+  double l1 = 0.0, rz = 0.0;  // r.z as in the start of the pass
   T yPrev = T(0), nPrev = T(0);
   load(0, pA, aA, rA, iA, nA, xA);
   for (int j0 = 0; j0 < nJ; j0 += U) {
@@ -392,13 +391,15 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
       if (j < nJ) {
         const int c = j * nI + ic;
         const T r = rA[u] - alphaT * aA[u];
-        const T y = (r - nPrev * yPrev) * iA[u];
+        const T w = r - nPrev * yPrev;
+        const T y = w * iA[u];
         if (col) {
           xg[c] = xA[u] + alpha * static_cast<double>(pA[u]);
           S.r[c] = r;
           S.z[c] = y;
         }
         l1 += fabs(static_cast<double>(r));
+        rz += static_cast<double>(y) * static_cast<double>(w);
         yPrev = y;
         nPrev = nA[u];
       }
@@ -413,9 +414,9 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
       xA[u] = xB[u];
     }
   }
-  double rz = lineBack<T>(S, ic, col, nI, nJ);
+  lineBack<T>(S, ic, col, nI, nJ);
   l1 = waveSum(col ? l1 : 0.0);
-  rz = waveSum(rz);
+  rz = waveSum(col ? rz : 0.0);
   if (threadIdx.x == 0) {
     partialSlot(I, sCgNorm)[blockIdx.x] = l1;
     partialSlot(I, sCgRz)[blockIdx.x] = rz;
@@ -497,26 +498,26 @@ __global__ void __launch_bounds__(kBlock) k_refine_final(const Inst* insts) {
 // 1e-10, KOmegaSST.cpp:468-471). Off-diagonal terms in cellFaces order
 // (below or the wake partner, right, above, left; Mesh.cpp:206-213); a row
 // with a zero diagonal is skipped (LinearSystem.cpp:88-90): its correction is
-// zero. The C-grid has an even nI, so a wake cell and its partner differ in
-// colour.
+// zero. The arrays are colour-packed (rbIndex): a launch reads its colour's
+// rows contiguously and the other colour's corrections at the neighbours.
+// The C-grid has an even nI, so a wake cell and its partner differ in colour.
 template <int NS>
 __device__ __forceinline__ void rbDefect(const float* const (&A)[NS][5], const float* const (&R)[NS],
-                                         float* const (&D)[NS], int c, int i, int j, int nI, int nJ, int pr,
+                                         float* const (&D)[NS], int pk, const int (&nb)[4], const bool (&has)[4],
                                          bool first, float (&out)[NS]) {
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const float d = A[s][0][c];
+    const float d = A[s][0][pk];
     float delta = 0.0f;
     if (fabsf(d) > 0.0f) {
       float off = 0.0f;
       if (!first) {
-        if (j > 0) off += A[s][1][c] * D[s][c - nI];
-        else if (pr >= 0) off += A[s][1][c] * D[s][pr];
-        if (i < nI - 1) off += A[s][4][c] * D[s][c + 1];
-        if (j < nJ - 1) off += A[s][2][c] * D[s][c + nI];
-        if (i > 0) off += A[s][3][c] * D[s][c - 1];
+        if (has[0]) off += A[s][1][pk] * D[s][nb[0]];
+        if (has[1]) off += A[s][4][pk] * D[s][nb[1]];
+        if (has[2]) off += A[s][2][pk] * D[s][nb[2]];
+        if (has[3]) off += A[s][3][pk] * D[s][nb[3]];
       }
-      delta = (R[s][c] - off) / d;
+      delta = (R[s][pk] - off) / d;
     }
     out[s] = delta;
   }
@@ -526,7 +527,26 @@ __global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int
   FAST_CELL_PROLOGUE
   if (!live) return;
   const bool mine = ((i + j) & 1) == color;
-  const int pr = (j == 0) ? I.partner[i] : -1;
+  if (!mine && !last) return;
+  const int pk = rbIndex(i, j, nI, nJ);
+  // the four neighbours in cellFaces order: below (or the wake partner), right, above, left
+  int nb[4] = {0, 0, 0, 0};
+  bool has[4] = {false, false, false, false};
+  if (mine && !first) {
+    if (j > 0) {
+      nb[0] = rbIndex(i, j - 1, nI, nJ);
+      has[0] = true;
+    } else {
+      const int pr = I.partner[i];
+      if (pr >= 0) {
+        nb[0] = rbIndex(pr, 0, nI, nJ);
+        has[0] = true;
+      }
+    }
+    if (i < nI - 1) { nb[1] = rbIndex(i + 1, j, nI, nJ); has[1] = true; }
+    if (j < nJ - 1) { nb[2] = rbIndex(i, j + 1, nI, nJ); has[2] = true; }
+    if (i > 0) { nb[3] = rbIndex(i - 1, j, nI, nJ); has[3] = true; }
+  }
   if (sys == 0) {
     // u and v share the coefficients
     const float* const A[2][5] = {{I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]},
@@ -535,12 +555,12 @@ __global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int
     float* const D[2] = {I.fMomDelta[0], I.fMomDelta[1]};
     float du[2];
     if (mine) {
-      rbDefect<2>(A, R, D, c, i, j, nI, nJ, pr, first != 0, du);
-      D[0][c] = du[0];
-      D[1][c] = du[1];
-    } else if (last) {
-      du[0] = D[0][c];
-      du[1] = D[1][c];
+      rbDefect<2>(A, R, D, pk, nb, has, first != 0, du);
+      D[0][pk] = du[0];
+      D[1][pk] = du[1];
+    } else {
+      du[0] = D[0][pk];
+      du[1] = D[1][pk];
     }
     if (last) {
       I.u[c] += static_cast<double>(du[0]);
@@ -553,12 +573,12 @@ __global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int
     float* const D[2] = {I.fTurbDelta[0], I.fTurbDelta[1]};
     float dt[2];
     if (mine) {
-      rbDefect<2>(A, R, D, c, i, j, nI, nJ, pr, first != 0, dt);
-      D[0][c] = dt[0];
-      D[1][c] = dt[1];
-    } else if (last) {
-      dt[0] = D[0][c];
-      dt[1] = D[1][c];
+      rbDefect<2>(A, R, D, pk, nb, has, first != 0, dt);
+      D[0][pk] = dt[0];
+      D[1][pk] = dt[1];
+    } else {
+      dt[0] = D[0][pk];
+      dt[1] = D[1][pk];
     }
     if (last) {
       I.k[c] = mx(I.k[c] + static_cast<double>(dt[0]), 0.0);

@@ -121,6 +121,23 @@ __host__ __device__ constexpr int skewLength(int nI, int nJ, int R) {
   return (kSkewFront + skewSteps(nI, nJ) + kSkewTail) * R + kSkewPark;
 }
 
+// The mixed path's two-colour sweep arrays hold each colour contiguously, so
+// a launch of one colour reads only that colour's rows: colour C = (i + j) & 1
+// (colour 0 first), rows in j order, a row's cells of that colour in i order
+// (i = i0, i0 + 2, ... with i0 = (C + j) & 1, so the cell's place in its row is
+// i >> 1). rbRowBase counts the colour-C cells of rows 0 .. j - 1 (two
+// consecutive rows hold nI of them).
+__host__ __device__ inline int rbRowBase(int j, int C, int nI) {
+  const int pairs = (j >> 1) * nI;
+  if (!(j & 1)) return pairs;
+  const int i0 = (C + j - 1) & 1;  // first colour-C column of row j - 1
+  return pairs + (nI - i0 + 1) / 2;
+}
+__host__ __device__ inline int rbIndex(int i, int j, int nI, int nJ) {
+  const int C = (i + j) & 1;
+  return (C ? rbRowBase(nJ, 0, nI) : 0) + rbRowBase(j, C, nI) + (i >> 1);
+}
+
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
 constexpr int kColThreads = 64;
@@ -205,16 +222,18 @@ struct Inst {
   float *fr, *fz, *fpd0, *fpd1, *fAp;
 
   // Line-preconditioned CG: 1 / pivot of each column's tridiagonal factor
-  // (Thomas), in the CG's storage type; the refinement's FP64 residual
-  // right-hand side and accumulated correction.
-  double* lInv;
-  float* flInv;
+  // (Thomas) and the eliminated upper coefficient (pN / pivot), in the CG's
+  // storage type; the refinement's FP64 residual right-hand side and
+  // accumulated correction.
+  double *lInv, *lUp;
+  float *flInv, *flUp;
   double *rb, *xacc;
 
   // Mixed path with two-colour sweeps: every swept system in FP32 as
   // coefficients (aP, aS, aN, aW, aE), the FP64 defect b - A phi rounded to
   // FP32, and the FP32 correction the sweeps solve for (phi += delta after
   // the last sweep, in FP64). Momentum: u and v share the coefficients.
+  // Cell (i, j) of these arrays is at rbIndex(i, j, nI, nJ).
   float* fMom[5];
   float* fMomRes[2];
   float* fMomDelta[2];

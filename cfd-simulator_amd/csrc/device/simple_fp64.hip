@@ -264,12 +264,13 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
       I.bx[c] = row.bx;
       I.by[c] = row.by;
     } else if (I.gsOrder == 1) {
-      // the mixed path's FP32 rows (the defects follow below)
-      I.fMom[0][c] = static_cast<float>(d);
-      I.fMom[1][c] = static_cast<float>(aS);
-      I.fMom[2][c] = static_cast<float>(aN);
-      I.fMom[3][c] = static_cast<float>(aW);
-      I.fMom[4][c] = static_cast<float>(aE);
+      // the mixed path's FP32 rows, colour-packed (the defects follow below)
+      const int pk = rbIndex(i, j, nI, nJ);
+      I.fMom[0][pk] = static_cast<float>(d);
+      I.fMom[1][pk] = static_cast<float>(aS);
+      I.fMom[2][pk] = static_cast<float>(aN);
+      I.fMom[3][pk] = static_cast<float>(aW);
+      I.fMom[4][pk] = static_cast<float>(aE);
     } else {
       // the lexicographic sweep's copy, anti-diagonal layout, in pairs
       const int sk = skewIndex(i, j, I.gsR);
@@ -290,8 +291,9 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     part[1] = fabs(row.by - axv);
     part[2] = fabs(d);
     if (I.gsOrder == 1 && I.precision == 1) {
-      I.fMomRes[0][c] = static_cast<float>(row.bx - axu);
-      I.fMomRes[1][c] = static_cast<float>(row.by - axv);
+      const int pk = rbIndex(i, j, nI, nJ);
+      I.fMomRes[0][pk] = static_cast<float>(row.bx - axu);
+      I.fMomRes[1][pk] = static_cast<float>(row.by - axv);
     }
   }
   blockSum<3>(part, sm);

@@ -233,12 +233,12 @@ __device__ __forceinline__ void skewRow(const Inst& I, int sys, int at, const Ss
 }
 
 // The mixed path's FP32 copy of one row (two-colour defect sweeps).
-__device__ __forceinline__ void mixedTurbRow(const Inst& I, int sys, int c, const SstRow& r) {
-  I.fTurb[sys][0][c] = static_cast<float>(r.d);
-  I.fTurb[sys][1][c] = static_cast<float>(r.aS);
-  I.fTurb[sys][2][c] = static_cast<float>(r.aN);
-  I.fTurb[sys][3][c] = static_cast<float>(r.aW);
-  I.fTurb[sys][4][c] = static_cast<float>(r.aE);
+__device__ __forceinline__ void mixedTurbRow(const Inst& I, int sys, int pk, const SstRow& r) {
+  I.fTurb[sys][0][pk] = static_cast<float>(r.d);
+  I.fTurb[sys][1][pk] = static_cast<float>(r.aS);
+  I.fTurb[sys][2][pk] = static_cast<float>(r.aN);
+  I.fTurb[sys][3][pk] = static_cast<float>(r.aW);
+  I.fTurb[sys][4][pk] = static_cast<float>(r.aE);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
@@ -269,10 +269,10 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
       } else if (I.precision == 0) {
         I.kP[c] = r.d; I.kS[c] = r.aS; I.kN[c] = r.aN; I.kW[c] = r.aW; I.kE[c] = r.aE; I.kB[c] = r.b;
       } else {
-        mixedTurbRow(I, 0, c, r);
+        mixedTurbRow(I, 0, rbIndex(i, j, nI, nJ), r);
       }
       const double defect = sstDefect(I, r, I.k, c, i, j);
-      if (I.gsOrder == 1 && I.precision == 1) I.fTurbRes[0][c] = static_cast<float>(defect);
+      if (I.gsOrder == 1 && I.precision == 1) I.fTurbRes[0][rbIndex(i, j, nI, nJ)] = static_cast<float>(defect);
       part[0] = fabs(defect);
       part[1] = fabs(r.d);
     }
@@ -297,10 +297,10 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
       } else if (I.precision == 0) {
         I.wP[c] = r.d; I.wS[c] = r.aS; I.wN[c] = r.aN; I.wW[c] = r.aW; I.wE[c] = r.aE; I.wB[c] = r.b;
       } else {
-        mixedTurbRow(I, 1, c, r);
+        mixedTurbRow(I, 1, rbIndex(i, j, nI, nJ), r);
       }
       const double defect = sstDefect(I, r, I.w, c, i, j);
-      if (I.gsOrder == 1 && I.precision == 1) I.fTurbRes[1][c] = static_cast<float>(defect);
+      if (I.gsOrder == 1 && I.precision == 1) I.fTurbRes[1][rbIndex(i, j, nI, nJ)] = static_cast<float>(defect);
       part[2] = fabs(defect);
       part[3] = fabs(r.d);
     }

@@ -248,9 +248,9 @@ void carveInst(Carver& cv, Inst& I, const Grid& grid) {
     for (double2** a : vecArrays) *a = cv.take<double2>(N);
     I.Fj = cv.take<double>(nJf);
     I.Fi = cv.take<double>(nIf);
-    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.flInv};
+    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.flInv, &I.flUp};
     for (float** a : fArrays) *a = cv.take<float>(N);
-    double** lineArrays[] = {&I.lInv, &I.rb, &I.xacc};
+    double** lineArrays[] = {&I.lInv, &I.lUp, &I.rb, &I.xacc};
     for (double** a : lineArrays) *a = cv.take<double>(N);
     for (float*& a : I.fMom) a = cv.take<float>(N);
     for (float*& a : I.fMomRes) a = cv.take<float>(N);
@@ -1005,7 +1005,9 @@ int Engine::debugArray(const std::string& name, double* out, int capacity) const
     std::vector<float> f(N);
     check(hipMemcpyAsync(f.data(), h_.fMom[momIdx + 1], N * sizeof(float), hipMemcpyDeviceToHost, stream_), "debug");
     check(hipStreamSynchronize(stream_), "debug sync");
-    for (size_t c = 0; c < N; ++c) out[c] = f[c];
+    for (int j = 0; j < grid_.nJ; ++j)
+      for (int i = 0; i < grid_.nI; ++i)
+        out[static_cast<size_t>(j) * grid_.nI + i] = f[static_cast<size_t>(rbIndex(i, j, grid_.nI, grid_.nJ))];
     return static_cast<int>(N);
   }
   if (momIdx >= 4 && settings_.precision == CFDGPU_PRECISION_MIXED) return -CFDGPU_NOT_FOUND;

@@ -38,6 +38,7 @@ struct Server::Slot {
   // coefficient settling: Cl and Cd every settle window, the last three kept
   int nextSample{0};
   std::deque<std::pair<double, double>> samples;
+  int settledChecks{0};  // consecutive checks that passed
 };
 
 double remainingChange(double c0, double c1, double c2) {
@@ -182,6 +183,7 @@ void Server::startAttempt(Slot& s, bool cold) {
   s.lastIteration = 0;
   s.nextSample = opt_.settleWindow;
   s.samples.clear();
+  s.settledChecks = 0;
   s.engine = std::move(e);
 }
 
@@ -393,10 +395,13 @@ bool Server::settled(Slot& s, int iteration) {
   s.engine->readMonitorOn(s.engine->stream(), &mon);
   const double worst = std::max(std::max(std::fabs(mon.continuity), std::fabs(mon.momentum_x)),
                                 std::fabs(mon.momentum_y));
-  if (!(worst < opt_.rule.tolerance)) return false;
   const double rl = remainingChange(s.samples[0].first, s.samples[1].first, s.samples[2].first);
   const double rd = remainingChange(s.samples[0].second, s.samples[1].second, s.samples[2].second);
-  return rl <= opt_.settleCl && rd <= opt_.settleCd;
+  const bool pass = worst < opt_.rule.tolerance && rl <= opt_.settleCl && rd <= opt_.settleCd;
+  s.settledChecks = pass ? s.settledChecks + 1 : 0;
+  // two consecutive checks, so an early transient that happens to look
+  // geometric for three samples does not end the solve
+  return s.settledChecks >= 2;
 }
 
 void Server::step() {

@@ -245,8 +245,9 @@ int cfdgpu_solve(cfdgpu_solver* solver, const cfdgpu_stop_rule* rule, cfdgpu_sol
  *      thread); it serves polar sweeps and mixed-size batches. The batch refers
  *      to the solvers, which must be initialised, outlive it, and not be used
  *      on their own while a batch call runs. Members must share the closure,
- *      sweep count, CG cap, corrector passes, ordering and pressure anchoring;
- *      meshes may differ. ---- */
+ *      sweep count, corrector passes, ordering, precision, pressure solver,
+ *      refinement passes and pressure anchoring; meshes and CG caps may differ
+ *      (each member stops its CG at its own cap). ---- */
 int cfdgpu_batch_create(cfdgpu_batch** out, cfdgpu_solver* const* solvers, int count);
 void cfdgpu_batch_destroy(cfdgpu_batch* batch);
 /* count outer iterations of every member; out[i] = member i's last monitor (may be NULL) */
@@ -346,7 +347,8 @@ typedef struct {
      settled: every settle_window outer iterations Cl and Cd are integrated on the device, and
      from the last three samples the change still to come is estimated (geometric tail, Aitken:
      |d2| rho / (1 - rho), rho = d2 / d1; an oscillation counts |d2|; a growing change never
-     settles). Both estimates must be at most settle_cl and settle_cd. No reference
+     settles). Both estimates must be at most settle_cl and settle_cd, with the residual test
+     passed, at two consecutive samples. No reference
      counterpart: the residual rule alone leaves the lift of fine near-wall grids far from its
      converged value. */
   int settle_window;
@@ -359,9 +361,10 @@ typedef struct {
   cfdgpu_cgrid_options mesh;   /* its C-grid */
   double alpha_deg, reynolds, speed, density, reference_pressure;
   double arrival_s;            /* server-clock arrival time; 0 = at once */
-  const cfdgpu_settings* settings; /* NULL: the server's. Relaxation, scheme, tolerances and inflow
-                                      turbulence may differ; sweeps, CG cap, correctors, ordering,
-                                      precision and pressure solver must be the server's */
+  const cfdgpu_settings* settings; /* NULL: the server's. Relaxation, scheme, tolerances, CG cap and
+                                      inflow turbulence may differ; sweeps, correctors, ordering,
+                                      precision, pressure solver and refinement passes must be the
+                                      server's */
 } cfdgpu_serve_request;
 
 typedef struct {

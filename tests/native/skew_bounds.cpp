@@ -6,7 +6,8 @@
 // step counters). It also checks that every fetch of a lane on the wavefront
 // reads the cell the step needs, that a store off the wavefront lands in a
 // slot of no cell or a parking slot, and that no fetch reads a parking slot.
-// Mirrors the index expressions of gsLexWave. Prints "ok" or the first
+// Mirrors the index expressions of gsLexWave. Also checks the colour-packed
+// layout of the mixed path's two-colour sweeps. Prints "ok" or the first
 // violation.
 #include <algorithm>
 #include <cstdio>
@@ -95,7 +96,27 @@ static int check(int nI, int nJ, int D, int B) {
   return 0;
 }
 
+// The colour-packed layout of the mixed path's two-colour sweeps (rbIndex):
+// a bijection of the mesh's cells onto [0, N), colour 0 first.
+static int checkColours(int nI, int nJ) {
+  const int N = nI * nJ;
+  const int split = rbRowBase(nJ, 0, nI);
+  std::set<int> seen;
+  for (int j = 0; j < nJ; ++j)
+    for (int i = 0; i < nI; ++i) {
+      const int k = rbIndex(i, j, nI, nJ);
+      if (k < 0 || k >= N) return fail("colour index", nI, nJ, 0, k, N);
+      if (((i + j) & 1) == 0 ? k >= split : k < split) return fail("colour plane", nI, nJ, 0, k, split);
+      if (!seen.insert(k).second) return fail("colour index repeated", nI, nJ, 0, k, N);
+    }
+  return 0;
+}
+
 int main() {
+  for (int nI = 1; nI <= 41; ++nI)
+    for (int nJ = 1; nJ <= 41; ++nJ)
+      if (checkColours(nI, nJ)) return 1;
+  if (checkColours(1024, 256) || checkColours(512, 128) || checkColours(256, 64) || checkColours(118, 27)) return 1;
   const int shapes[][2] = {{1, 1}, {2, 1}, {1, 5}, {26, 5}, {26, 79}, {118, 27}, {64, 64}, {65, 63},
                            {256, 64}, {512, 128}, {1024, 256}, {300, 257}, {40, 1000}, {7, 1024}};
   for (const auto& s : shapes) {
