@@ -109,7 +109,7 @@ def path_settings(G, args, **kw):
     inflow turbulence, and the path options of the command line."""
     st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=precision_of(G, args), **kw)
     if getattr(args, "fast", False):
-        st.precision, st.gs_ordering, st.pressure_solver, st.pressure_iterations = G.MIXED, 1, 1, 20
+        st.precision, st.gs_ordering, st.pressure_solver, st.pressure_iterations = G.MIXED, 1, 1, 15
     if getattr(args, "gs", None) is not None:
         st.gs_ordering = {"lex": 0, "rb": 1}[args.gs]
     if getattr(args, "psolver", None) is not None:
@@ -822,7 +822,7 @@ def main():
                     help="fp64: the plain path; mixed: FP32 storage in the pressure CG")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-path comparison of the default run")
     ap.add_argument("--fast", action="store_true",
-                    help="the fast path: mixed precision, two-colour FP32 sweeps, line-preconditioned CG of 20")
+                    help="the fast path: mixed precision, two-colour FP32 sweeps, line-preconditioned CG of 15")
     ap.add_argument("--gs", choices=("lex", "rb"), help="Gauss-Seidel ordering")
     ap.add_argument("--psolver", choices=("jacobi", "line"), help="pressure solver")
     ap.add_argument("--cg-max", type=int, help="pressure CG iteration cap")
@@ -830,7 +830,7 @@ def main():
     ap.add_argument("--only", help="with --mixed-batch / --serve: comma-separated manifest rows to run")
     ap.add_argument("--single", action="store_true",
                     help="the single 1024x256 SST solve (plain FP64 step time, roofline, mixed comparison)")
-    ap.add_argument("--fast-cg", type=int, default=20, help="line-CG cap of the fast path in the headline run")
+    ap.add_argument("--fast-cg", type=int, default=15, help="line-CG cap of the fast path in the headline run")
     ap.add_argument("--fast-cg-medium", type=int, default=None, help="line-CG cap on the 512x128 instances")
     ap.add_argument("--fast-cg-fine", type=int, default=None, help="line-CG cap on the 1024x256 instances")
     ap.add_argument("--plain-results", help="headline: take the plain FP64 run from this --results-file")
