@@ -90,13 +90,14 @@ __device__ __forceinline__ double waveSum(double v) {
 // the next block are loaded before the current block is computed (rows past
 // the end are clamped to the last row and not used), so the recurrence waits
 // for memory once per block instead of once per row.
-constexpr int kColUnroll = 16;
+constexpr int kColUnroll = 16;   // start of a pass
+constexpr int kColUnrollB = 32;  // the CG iteration's sweeps (four operand arrays forward, two back)
 
 // Back substitution z_j = y_j - up_j z_{j+1}, j = nJ-1 .. 0, in place over z
 // (which holds y), with up_j = pN_j / pivot_j stored by the factorisation.
 template <typename T>
 __device__ __forceinline__ void lineBack(const LineStore<T>& S, int ic, bool col, int nI, int nJ) {
-  constexpr int U = kColUnroll;
+  constexpr int U = kColUnrollB;
   T yA[U], uA[U];
   T yB[U] = {}, uB[U] = {};
   auto load = [&](int top, T (&y)[U], T (&up)[U]) {
@@ -273,9 +274,26 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   }
   const int stopped = flagsG[fStopped];
   const int cgDone = flagsG[fCgDone];
+  const int itersDone = flagsG[fCgIters];
   const double norm0 = red[sCgNorm0];
   const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
-  if (stopped || cgDone) return;  // the whole block: the flags are uniform
+  const double alphaPrev = red[sCgAlpha];
+  double xq[CELLS];
+#pragma unroll
+  for (int q = 0; q < CELLS; ++q) xq[q] = gmem(static_cast<const double*>(I.x))[cq[q]];
+  if (stopped) return;  // the whole block: the flags are uniform
+  // x += alpha p of iteration it - 1 (launch B leaves it to this launch, whose
+  // cells hold p_{it-1}), whatever this launch's test decides. This comes
+  // before the test of cgDone: block 0 of this launch may already have set
+  // it for a converged solve, and every block must still add its cells'
+  // share of the last step (itersDone == it only when launch B of it - 1
+  // made one).
+  if (it > 0 && itersDone == it) {
+#pragma unroll
+    for (int q = 0; q < CELLS; ++q)
+      if (live[q]) gmem(I.x)[cq[q]] = xq[q] + alphaPrev * static_cast<double>(oc[q]);
+  }
+  if (cgDone) return;
   // row 0: the wake partner's values (a second round trip for those lanes only)
   T zPr[CELLS], oPr[CELLS];
   bool wake[CELLS];
@@ -341,13 +359,13 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
 }
 
-// Iteration `it`, launch B (column-parallel): alpha = r.z / p.Ap, x += alpha p
-// (FP64), r -= alpha Ap, |r|_1, z = M^-1 r (forward and back substitution
-// along the column), r.z.
+// Iteration `it`, launch B (column-parallel): alpha = r.z / p.Ap (kept for
+// the next launch A, which adds alpha p to x), r -= alpha Ap, |r|_1,
+// z = M^-1 r (forward and back substitution along the column), r.z.
 template <typename T>
 __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
   LCG_COLUMN_PROLOGUE
-  constexpr int U = kColUnroll;
+  constexpr int U = kColUnrollB;
   __shared__ double bc[1];
   if (it >= I.cgMax) return;
   if (I.flags[fCgDone]) return;
@@ -362,29 +380,24 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
   }
   const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
   const T alphaT = static_cast<T>(alpha);
-  const CFD_GLOBAL T* p = (it & 1) ? S.pd1 : S.pd0;
-  CFD_GLOBAL double* xg = gmem(I.x);
   const int ic = col ? i : nI - 1;
-  T pA[U], aA[U], rA[U], iA[U], nA[U];
-  T pB[U] = {}, aB[U] = {}, rB[U] = {}, iB[U] = {}, nB[U] = {};
-  double xA[U], xB[U] = {};
-  auto load = [&](int j0, T (&pv)[U], T (&av)[U], T (&rv)[U], T (&iv)[U], T (&nv)[U], double (&xv)[U]) {
+  T aA[U], rA[U], iA[U], nA[U];
+  T aB[U] = {}, rB[U] = {}, iB[U] = {}, nB[U] = {};
+  auto load = [&](int j0, T (&av)[U], T (&rv)[U], T (&iv)[U], T (&nv)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int c = min(j0 + u, nJ - 1) * nI + ic;
-      pv[u] = p[c];
       av[u] = S.Ap[c];
       rv[u] = S.r[c];
       iv[u] = S.inv[c];
       nv[u] = S.pN[c];
-      xv[u] = xg[c];
     }
   };
   double l1 = 0.0, rz = 0.0;  // r.z as in the start of the pass
   T yPrev = T(0), nPrev = T(0);
-  load(0, pA, aA, rA, iA, nA, xA);
+  load(0, aA, rA, iA, nA);
   for (int j0 = 0; j0 < nJ; j0 += U) {
-    if (j0 + U < nJ) load(j0 + U, pB, aB, rB, iB, nB, xB);
+    if (j0 + U < nJ) load(j0 + U, aB, rB, iB, nB);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int j = j0 + u;
@@ -394,7 +407,6 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
         const T w = r - nPrev * yPrev;
         const T y = w * iA[u];
         if (col) {
-          xg[c] = xA[u] + alpha * static_cast<double>(pA[u]);
           S.r[c] = r;
           S.z[c] = y;
         }
@@ -406,12 +418,10 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      pA[u] = pB[u];
       aA[u] = aB[u];
       rA[u] = rB[u];
       iA[u] = iB[u];
       nA[u] = nB[u];
-      xA[u] = xB[u];
     }
   }
   lineBack<T>(S, ic, col, nI, nJ);
@@ -421,10 +431,24 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
     partialSlot(I, sCgNorm)[blockIdx.x] = l1;
     partialSlot(I, sCgRz)[blockIdx.x] = rz;
     if (blockIdx.x == 0) {
+      I.red[sCgAlpha] = alpha;
       I.flags[fCgIters] = it + 1;
       I.flags[fCgTotal] += 1;
     }
   }
+}
+
+// After the CG iterations of a pass: the x update of the last launch B, when
+// no launch A followed it (the cap was reached).
+template <typename T>
+__device__ __forceinline__ void lcgFinal(const Inst* insts) {
+  FAST_CELL_PROLOGUE
+  (void)i; (void)j; (void)nI; (void)nJ;
+  const int n = I.flags[fCgIters];
+  if (!live || n != I.cgMax || n == 0) return;
+  const LineStore<T> S = lineStore<T>(I);
+  const CFD_GLOBAL T* p = ((n - 1) & 1) ? S.pd1 : S.pd0;
+  gmem(I.x)[c] += I.red[sCgAlpha] * static_cast<double>(p[c]);
 }
 
 __global__ void __launch_bounds__(kColThreads) k_lcg_init(const Inst* insts, int pass) { lcgInit<double>(insts, pass); }
@@ -433,6 +457,7 @@ __global__ void __launch_bounds__(kBlock) k_lcg_a(const Inst* insts, int it) {
   lcgA<double>(insts, it, sm);
 }
 __global__ void __launch_bounds__(kColThreads) k_lcg_b(const Inst* insts, int it) { lcgB<double>(insts, it); }
+__global__ void __launch_bounds__(kBlock) k_lcg_final(const Inst* insts) { lcgFinal<double>(insts); }
 
 __global__ void __launch_bounds__(kColThreads) k_lcg_init_f32(const Inst* insts, int pass) {
   lcgInit<float>(insts, pass);
@@ -442,6 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_lcg_a_f32(const Inst* insts, int it)
   lcgA<float>(insts, it, sm);
 }
 __global__ void __launch_bounds__(kColThreads) k_lcg_b_f32(const Inst* insts, int it) { lcgB<float>(insts, it); }
+__global__ void __launch_bounds__(kBlock) k_lcg_final_f32(const Inst* insts) { lcgFinal<float>(insts); }
 
 // ---------------------------------------------------------------------------
 // FP64-residual refinement of the pressure correction

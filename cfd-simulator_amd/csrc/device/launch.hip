@@ -39,6 +39,8 @@ __global__ void k_lcg_b(const Inst*, int);
 __global__ void k_lcg_init_f32(const Inst*, int);
 __global__ void k_lcg_a_f32(const Inst*, int);
 __global__ void k_lcg_b_f32(const Inst*, int);
+__global__ void k_lcg_final(const Inst*);
+__global__ void k_lcg_final_f32(const Inst*);
 __global__ void k_refine_acc(const Inst*, int);
 __global__ void k_refine_res(const Inst*);
 __global__ void k_refine_final(const Inst*);
@@ -124,12 +126,14 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
           hipLaunchKernelGGL(k_lcg_a_f32, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_lcg_b_f32, colGrid, col, 0, s, d, it);
         }
+        hipLaunchKernelGGL(k_lcg_final_f32, grid, block, 0, s, d);
       } else if (line) {
         hipLaunchKernelGGL(k_lcg_init, colGrid, col, 0, s, d, rp);
         for (int it = 0; it < cfg.cgIters; ++it) {
           hipLaunchKernelGGL(k_lcg_a, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_lcg_b, colGrid, col, 0, s, d, it);
         }
+        hipLaunchKernelGGL(k_lcg_final, grid, block, 0, s, d);
       } else if (mixed) {
         hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d, rp);
         for (int it = 0; it < cfg.cgIters; ++it) {

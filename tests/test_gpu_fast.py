@@ -34,12 +34,14 @@ def _system(mesh, s):
     return L.matrix(dP, pN, pE, pW, ni, nj, partner), dP, pN, b
 
 
-@pytest.mark.parametrize("iterations", [1, 30])
+@pytest.mark.parametrize("iterations,cap", [(1, 20), (30, 20), (2, 500)], ids=["1it", "30it", "early-exit"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
-def test_line_cg_matches_its_definition(G, gpu_available, precision, iterations):
-    print(f"case: 118x27 SST, line CG ({'fp64' if precision == 0 else 'fp32'}), {iterations} iterations",
-          flush=True)
-    mesh, s, st, mon = _cgrid_solver(G, precision, iterations)
+def test_line_cg_matches_its_definition(G, gpu_available, precision, iterations, cap):
+    """With a cap of 500 the CG reaches the reference's 0.01 L1 reduction
+    first (about 150 iterations): the exit path."""
+    print(f"case: 118x27 SST, line CG ({'fp64' if precision == 0 else 'fp32'}), {iterations} iterations, "
+          f"cap {cap}", flush=True)
+    mesh, s, st, mon = _cgrid_solver(G, precision, iterations, pressure_iterations=cap)
     A, dP, pN, b = _system(mesh, s)
     x, its = L.line_cg(A, dP, pN, b, mesh.ni, mesh.nj, st.pressure_tolerance, st.pressure_iterations)
     got = s.debug_array("correction")
@@ -48,6 +50,8 @@ def test_line_cg_matches_its_definition(G, gpu_available, precision, iterations)
     if precision == 0:
         assert mon.pressure_iterations == its
         assert err < 1e-10
+        if cap == 500:
+            assert its < cap
     else:
         # FP32 storage and arithmetic, FP64 inner products: the iterate is
         # the FP64 one to single-precision rounding accumulated over 20 steps
