@@ -22,11 +22,14 @@ iterations), the speedups, p50/p99 latency, the roofline of the dominant unit
 (one line-CG iteration over the whole batch, from HIP events on the batch's
 stream inside the timed region: K outer iterations after W untimed), the
 per-stage times, and the CPU baseline (the FP64 oracle, a restatement of the
-reference solver, timed on this host's cores).
+reference solver, timed on this host's cores). The line CG runs with the
+launch shapes of cfd-simulator_amd/workloads/launch_tuning.json (written by
+cfd-simulator_amd/tune.py) when that file exists.
 
-Other modes: --single (one 1024x256 SST solve: per-step times, roofline,
-mixed comparison), --solve, --converge, --batch-polar, --mixed-batch,
---serve, --calibrate-copy.
+Other modes: --roofline-only (the timed region and its roofline alone, the
+command kernel traces are taken of), --single (one 1024x256 SST solve:
+per-step times, roofline, mixed comparison), --solve, --converge,
+--batch-polar, --mixed-batch, --serve, --calibrate-copy.
 """
 from __future__ import annotations
 
@@ -66,10 +69,11 @@ def traffic_model():
         "correction": 23,
         "sst_update": 130,
         # bytes per cell of one line-CG iteration on the mixed path (FP32 storage, FP64 x):
-        # launch A reads z, p_old, aP, aN, aE (4 B each) and writes p, Ap = 28; launch B reads
-        # p, Ap, r, 1/pivot, aN (20) and x (8), writes x (8), r, z (8), then the back
-        # substitution reads z, aN, 1/pivot, r (16) and writes z (4) = 64
-        "line_cg_iteration_mixed": 92,
+        # launch A reads z, p_old, aP, aN, aE (4 B each) and x (8), writes x (8), p, Ap (8) = 44
+        # (x += alpha p of the previous iteration rides in launch A); launch B reads Ap, r,
+        # 1/pivot, aN (16) and writes r, z (8), then the back substitution reads z and the
+        # eliminated coupling (8) and writes z (4) = 36
+        "line_cg_iteration_mixed": 80,
     }
 
 
@@ -77,7 +81,7 @@ def traffic_model():
 # the traffic model, launches per stage call and CG iteration).
 DOMINANT_CANDIDATES = [("momentum_sweeps", "k_gs_wave_mom", "momentum_sweeps"),
                        ("sst_sweeps", "k_gs_wave_turb", "sst_sweeps"),
-                       ("pressure_solve", "k_cg_a + k_cg_b", "cg_iteration")]
+                       ("cg_iterations", "k_cg_a + k_cg_b", "cg_iteration")]
 
 
 def build_case(G):
@@ -251,6 +255,66 @@ def cpu_baseline_batch(rows, results, sample_iterations=(12, 4, 2), parallel_ite
                        f"for the parallel efficiency")}
 
 
+LAUNCH_TUNING = {"source": "built-in"}
+
+
+def apply_tuning(args):
+    """Line-CG launch shapes of cfd-simulator_amd/tune.py's file, if any."""
+    import cfdgpu as G
+    import tune
+
+    path = args.tuning or tune.TUNING_FILE
+    shape = None if args.tuning == "none" else tune.load_tuning(path)
+    if shape is not None:
+        G.set_launch_tuning(*shape)
+        LAUNCH_TUNING["source"] = os.path.relpath(os.path.abspath(path), _HERE)
+    cells, rows = G.launch_tuning()
+    LAUNCH_TUNING.update(cells_per_thread=cells, column_rows=rows)
+
+
+def batch_roofline(fast_st, cells, stage_ms, calls, cg_cells):
+    """The roofline object of the fast path's line-CG iteration on the batch."""
+    tm = traffic_model()
+    # one line-CG iteration of the whole batch (launch A + launch B): the CG loop's HIP-event time
+    # over its launch pairs; bytes of the member-iterations those launches ran (a member whose CG
+    # has met its tolerance returns at once)
+    pairs = calls["cg_iterations"] * fast_st.pressure_iterations
+    unit_ms = stage_ms["cg_iterations"] / max(pairs, 1)
+    active = cg_cells / max(pairs, 1)  # cells per launch pair that iterate
+    algo = tm["line_cg_iteration_mixed"] * active
+    achieved = algo / (unit_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get("line_cg_iteration_mixed"),
+                "kernel": f"k_lcg_a_f32_c{LAUNCH_TUNING['cells_per_thread']} + "
+                          f"k_lcg_b_f32_u{LAUNCH_TUNING['column_rows']}", "stage": "cg_iterations",
+                "algorithmic_bytes_per_launch": algo, "launch_ms": unit_ms, "launch_pairs": pairs,
+                "bytes_per_cell": tm["line_cg_iteration_mixed"], "cells": cells, "iterating_cells": active,
+                "note": "one line-CG iteration of the whole batch (launch A + launch B), HIP events on the "
+                        "batch's stream around the CG loop of every timed outer iteration"}
+    return roofline
+
+
+def run_roofline(args):
+    """Only the timed region of the headline: K outer iterations of the fast path
+    on the 64-instance batch after W untimed, with the roofline of its line-CG
+    iteration (a diagnostic line, the command a kernel trace is taken of)."""
+    import cfdgpu as G
+    import workloads as W
+
+    if G.device_count() < 1:
+        raise SystemExit("bench.py needs a GPU: no HIP device visible")
+    rows = W.load(W.MIXED_BATCH_FILE)
+    fast_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=G.MIXED, gs_ordering=1,
+                                 pressure_solver=1, pressure_iterations=args.fast_cg)
+    cells, ms_per_step, stage_ms, calls, cg_cells = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
+    print(json.dumps({"diagnostic": "fast path on the 64-instance mixed-size batch: step time and the line-CG "
+                                    "roofline (not the metric)",
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "cells": cells,
+                      "fast_path": settings_summary(fast_st), "launch_tuning": dict(LAUNCH_TUNING),
+                      "roofline": batch_roofline(fast_st, cells, stage_ms, calls, cg_cells),
+                      "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()}}), flush=True)
+
+
 def run_headline(args):
     """The benchmark line: the metric on the 64-instance mixed-size batch."""
     import cfdgpu as G
@@ -266,7 +330,8 @@ def run_headline(args):
     common = dict(tolerance=H["tolerance"], max_iterations=H["max_iterations"], chunk=H["chunk"], slots=H["slots"],
                   settle_window=H["settle_window"], settle_cl=H["settle_cl"], settle_cd=H["settle_cd"])
     # the timed unit of the roofline: K outer iterations of the whole batch after W untimed
-    cells, ms_per_step, stage_ms, calls, cg_its = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
+    cells, ms_per_step, stage_ms, calls, cg_cells = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
+    roofline = batch_roofline(fast_st, cells, stage_ms, calls, cg_cells)
     caps = {"coarse": args.fast_cg, "medium": args.fast_cg_medium or args.fast_cg,
             "fine": args.fast_cg_fine or args.fast_cg}
     fast_st.pressure_iterations = max(caps.values())
@@ -288,17 +353,6 @@ def run_headline(args):
     alone_rate = plain_conv / alone_s if alone_s > 0 else None
     fast_alone_ms = alone_rates(G, W, rows, fast_st)
 
-    tm = traffic_model()
-    unit_ms = stage_ms["pressure_solve"] / max(cg_its, 1)
-    algo = tm["line_cg_iteration_mixed"] * cells
-    achieved = algo / (unit_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get("line_cg_iteration_mixed"),
-                "kernel": "k_lcg_a_f32 + k_lcg_b_f32", "stage": "pressure_solve",
-                "algorithmic_bytes_per_launch": algo, "launch_ms": unit_ms,
-                "bytes_per_cell": tm["line_cg_iteration_mixed"], "cells": cells,
-                "note": "one line-CG iteration of the whole batch (launch A + launch B) from HIP events on the "
-                        "batch's stream; the init launch of each solve is included in the stage time"}
     line = {"metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
@@ -310,6 +364,7 @@ def run_headline(args):
                                     f"settled (estimated remaining change <= {H['settle_cl']:g}, {H['settle_cd']:g} "
                                     f"from samples every {H['settle_window']} iterations)",
                        "fast_path": dict(settings_summary(fast_st), cg_cap_by_level=caps),
+                       "launch_tuning": dict(LAUNCH_TUNING),
                        "plain_path": settings_summary(plain_st),
                        "step": "one outer iteration of all 64 resident solves (fast path), timed by HIP events"},
             "accepted": accepted, "converged_both": len(acc),
@@ -731,8 +786,14 @@ def roofline_batch(G, W, rows, st, steps, warmup):
     t0 = time.perf_counter()
     mons, stage_ms, calls = batch.iterate_timed(steps)
     wall = time.perf_counter() - t0
-    cg_its = max(m.pressure_iterations for m in mons) * steps
-    return cells, wall * 1e3 / steps, stage_ms, calls, cg_its
+    caps = [mon.pressure_iterations for mon in mons]
+    if min(caps) != max(caps):
+        print(f"[roofline] CG iterations per member in the last step range {min(caps)}-{max(caps)}",
+              file=sys.stderr, flush=True)
+    # cells x CG iterations of the timed region (the last step's counts, the same in every step
+    # once every member runs to its cap)
+    cg_cells = sum(m.cells * mon.pressure_iterations for (m, _), mon in zip(solvers, mons)) * steps
+    return cells, wall * 1e3 / steps, stage_ms, calls, cg_cells
 
 
 def compare_results(mine, path):
@@ -830,17 +891,23 @@ def main():
     ap.add_argument("--only", help="with --mixed-batch / --serve: comma-separated manifest rows to run")
     ap.add_argument("--single", action="store_true",
                     help="the single 1024x256 SST solve (plain FP64 step time, roofline, mixed comparison)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the headline's timed region and its line-CG roofline (for kernel traces)")
     ap.add_argument("--fast-cg", type=int, default=15, help="line-CG cap of the fast path in the headline run")
     ap.add_argument("--fast-cg-medium", type=int, default=None, help="line-CG cap on the 512x128 instances")
     ap.add_argument("--fast-cg-fine", type=int, default=None, help="line-CG cap on the 1024x256 instances")
     ap.add_argument("--plain-results", help="headline: take the plain FP64 run from this --results-file")
     ap.add_argument("--results-file", help="with --serve / --mixed-batch: write every request's result here")
     ap.add_argument("--compare-to", help="with --serve / --mixed-batch: a --results-file to compare coefficients with")
+    ap.add_argument("--tuning", default=None,
+                    help="line-CG launch shapes from this tuner file (default: the package's "
+                         "workloads/launch_tuning.json when present); 'none' keeps the built-in shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent solves shard "
               "by instance, one solve per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    apply_tuning(args)
     if args.calibrate_copy:
         run_calibrate_copy(args)
     elif args.batch_polar:
@@ -853,6 +920,8 @@ def main():
         run_converge(args)
     elif args.single:
         run_bench(args)
+    elif args.roofline_only:
+        run_roofline(args)
     else:
         run_headline(args)
 

@@ -38,7 +38,7 @@ EXPORTS = [
     "cfdgpu_batch_iterate", "cfdgpu_batch_iterate_timed", "cfdgpu_batch_solve", "cfdgpu_polar_sweep", "cfdgpu_polar_csv", "cfdgpu_get_settings", "cfdgpu_set_settings", "cfdgpu_get_field",
     "cfdgpu_set_field", "cfdgpu_get_massflux", "cfdgpu_set_massflux", "cfdgpu_get_turbulence",
     "cfdgpu_set_turbulence_fields", "cfdgpu_turbulence_info", "cfdgpu_divergence", "cfdgpu_divergence_noalloc",
-    "cfdgpu_surface",
+    "cfdgpu_set_launch_tuning", "cfdgpu_get_launch_tuning", "cfdgpu_surface",
     "cfdgpu_sweep_angles", "cfdgpu_debug_array", "cfdgpu_server_defaults", "cfdgpu_server_create",
     "cfdgpu_server_destroy", "cfdgpu_server_submit", "cfdgpu_server_step", "cfdgpu_server_run",
     "cfdgpu_server_collect", "cfdgpu_server_fields", "cfdgpu_solver_surface",
@@ -84,7 +84,8 @@ class SstConstants(C.Structure):
 # CFDGPU_STAGE_* in order
 STAGES = ("gradients", "momentum_assembly", "momentum_sweeps", "rhie_chow", "pressure_assembly",
           "pressure_solve", "correction", "sst_blending", "sst_assembly", "sst_sweeps",
-          "sst_wall_viscosity", "monitor", "sweep_copy")
+          "sst_wall_viscosity", "monitor", "sweep_copy",
+          "cg_iterations")  # the last is inside pressure_solve, not added to it
 
 
 class Monitor(C.Structure):
@@ -212,6 +213,8 @@ def load(path: str = LIB_PATH):
         "cfdgpu_turbulence_info": (C.c_int, [vp, D, D, D]),
         "cfdgpu_divergence": (C.c_int, [vp, _dp]),
         "cfdgpu_divergence_noalloc": (C.c_int, [vp, _dp]),
+        "cfdgpu_set_launch_tuning": (C.c_int, [C.c_int, C.c_int]),
+        "cfdgpu_get_launch_tuning": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "cfdgpu_surface": (C.c_int, [vp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(Forces)]),
         "cfdgpu_sweep_angles": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, C.c_int]),
@@ -257,6 +260,17 @@ def device_count() -> int:
     n = C.c_int(0)
     lib().cfdgpu_device_count(C.byref(n))
     return n.value
+
+
+def set_launch_tuning(cells_per_thread: int, column_rows: int) -> None:
+    """Launch shapes of the line-preconditioned CG (process-wide; results agree to rounding)."""
+    _check(lib().cfdgpu_set_launch_tuning(int(cells_per_thread), int(column_rows)))
+
+
+def launch_tuning() -> tuple[int, int]:
+    cells, rows = C.c_int(), C.c_int()
+    _check(lib().cfdgpu_get_launch_tuning(C.byref(cells), C.byref(rows)))
+    return cells.value, rows.value
 
 
 def default_settings(**overrides) -> Settings:

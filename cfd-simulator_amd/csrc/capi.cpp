@@ -483,6 +483,25 @@ int cfdgpu_divergence_noalloc(const cfdgpu_solver* s, double* out) {
   return guarded([&]() -> int { return s->engine->divergenceNoAlloc(out); });
 }
 
+int cfdgpu_set_launch_tuning(int cells, int rows) {
+  if (cells != 1 && cells != 2 && cells != 4)
+    return fail(CFDGPU_INVALID_ARGUMENT, "cells per thread must be 1, 2 or 4");
+  if (rows != 16 && rows != 32) return fail(CFDGPU_INVALID_ARGUMENT, "column rows must be 16 or 32");
+  cfdgpu::LaunchTuning t;
+  t.lcgCells = cells;
+  t.colUnroll = rows;
+  cfdgpu::setLaunchTuning(t);
+  return CFDGPU_OK;
+}
+
+int cfdgpu_get_launch_tuning(int* cells, int* rows) {
+  if (!cells || !rows) return fail(CFDGPU_INVALID_ARGUMENT, "null argument");
+  const cfdgpu::LaunchTuning t = cfdgpu::launchTuning();
+  *cells = t.lcgCells;
+  *rows = t.colUnroll;
+  return CFDGPU_OK;
+}
+
 int cfdgpu_surface(const cfdgpu_mesh* m, const double* u, const double* v, const double* p, const double* mu,
                    double speed, double alpha, double density, double pref, double chord, double* rows, int capacity,
                    int* count, cfdgpu_forces* forces) {

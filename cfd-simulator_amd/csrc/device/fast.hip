@@ -90,14 +90,12 @@ __device__ __forceinline__ double waveSum(double v) {
 // the next block are loaded before the current block is computed (rows past
 // the end are clamped to the last row and not used), so the recurrence waits
 // for memory once per block instead of once per row.
-constexpr int kColUnroll = 16;   // start of a pass
-constexpr int kColUnrollB = 32;  // the CG iteration's sweeps (four operand arrays forward, two back)
+constexpr int kColUnroll = 16;  // start of a pass (the CG iteration's depth is the launch tuning's)
 
 // Back substitution z_j = y_j - up_j z_{j+1}, j = nJ-1 .. 0, in place over z
 // (which holds y), with up_j = pN_j / pivot_j stored by the factorisation.
-template <typename T>
+template <typename T, int U>
 __device__ __forceinline__ void lineBack(const LineStore<T>& S, int ic, bool col, int nI, int nJ) {
-  constexpr int U = kColUnrollB;
   T yA[U], uA[U];
   T yB[U] = {}, uB[U] = {};
   auto load = [&](int top, T (&y)[U], T (&up)[U]) {
@@ -210,7 +208,7 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
       nA[u] = nB[u];
     }
   }
-  lineBack<T>(S, ic, col, nI, nJ);
+  lineBack<T, kColUnroll>(S, ic, col, nI, nJ);
   l1 = waveSum(col ? l1 : 0.0);
   rz = waveSum(col ? rz : 0.0);
   if (threadIdx.x == 0) {
@@ -226,20 +224,20 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
 // Iteration `it`, launch A (cell-parallel): the test of iteration it - 1
 // from the column partials (|r|_1 <= tol |r0|_1, then the r.z guard), beta,
 // p = z + beta p_old (neighbours recomputed locally), Ap and p.Ap.
-// Each thread takes kCgCells cells (c = tile * kBlock * kCgCells + q * kBlock
-// + threadIdx.x). Every operand is read through the global address space,
+// Each thread takes CELLS cells (c = tile * kBlock * CELLS + q * kBlock +
+// threadIdx.x; 1, 2 or 4, the launch tuning's choice). Every operand is read through the global address space,
 // without a branch around its load (a missing neighbour is read at the cell
 // itself and its coefficient masked to zero), and issued before the stop
 // flags are tested and the partials summed, so the stencil's loads and the
 // partial sums are one memory round trip; only the lanes of row 0 wait again
 // for their wake partner.
-template <typename T>
+template <typename T, int CELLS>
 __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
-  constexpr int CELLS = kCgCells;
   const Inst& I = insts[blockIdx.y];
-  if (static_cast<int>(blockIdx.x) >= I.cgBlocks) return;
+  const int nb = lineBlocksA(I.N, CELLS);
+  if (static_cast<int>(blockIdx.x) >= nb) return;
   if (it >= I.cgMax) return;  // a batch member with a smaller cap than the batch's
-  const int tile = xcdTile(static_cast<int>(blockIdx.x), I.cgBlocks);
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), nb);
   const int base = tile * kBlock * CELLS + static_cast<int>(threadIdx.x);
   const int nI = I.nI, nJ = I.nJ;
   const LineStore<T> S = lineStore<T>(I);
@@ -270,7 +268,8 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
     zc[q] = z[c]; zS[q] = z[cS]; zN[q] = z[cN]; zW[q] = z[cW]; zE[q] = z[cE];
     oc[q] = oS[q] = oN[q] = oW[q] = oE[q] = T(0);
     if (it > 0) { oc[q] = pOld[c]; oS[q] = pOld[cS]; oN[q] = pOld[cN]; oW[q] = pOld[cW]; oE[q] = pOld[cE]; }
-    dP[q] = S.pP[c]; nB[q] = S.pN[cS]; nA[q] = S.pN[c]; eL[q] = S.pE[cW]; eR[q] = S.pE[c]; wk[q] = S.pWk[c];
+    dP[q] = S.pP[c]; nB[q] = S.pN[cS]; nA[q] = S.pN[c]; eL[q] = S.pE[cW]; eR[q] = S.pE[c];
+    wk[q] = (j == 0) ? S.pWk[c] : T(0);  // the wake coupling exists on row 0 only
   }
   const int stopped = flagsG[fStopped];
   const int cgDone = flagsG[fCgDone];
@@ -362,15 +361,14 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
 // Iteration `it`, launch B (column-parallel): alpha = r.z / p.Ap (kept for
 // the next launch A, which adds alpha p to x), r -= alpha Ap, |r|_1,
 // z = M^-1 r (forward and back substitution along the column), r.z.
-template <typename T>
-__device__ __forceinline__ void lcgB(const Inst* insts, int it) {
+template <typename T, int U>
+__device__ __forceinline__ void lcgB(const Inst* insts, int it, int cellsA) {
   LCG_COLUMN_PROLOGUE
-  constexpr int U = kColUnrollB;
   __shared__ double bc[1];
   if (it >= I.cgMax) return;
   if (I.flags[fCgDone]) return;
   const LineStore<T> S = lineStore<T>(I);
-  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), I.cgBlocks);
+  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), lineBlocksA(I.N, cellsA));
   if (threadIdx.x == 0) bc[0] = papLane;
   __syncthreads();
   const double pap = bc[0];
@@ -424,7 +422,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it) {
       nA[u] = nB[u];
     }
   }
-  lineBack<T>(S, ic, col, nI, nJ);
+  lineBack<T, U>(S, ic, col, nI, nJ);
   l1 = waveSum(col ? l1 : 0.0);
   rz = waveSum(col ? rz : 0.0);
   if (threadIdx.x == 0) {
@@ -451,23 +449,35 @@ __device__ __forceinline__ void lcgFinal(const Inst* insts) {
   gmem(I.x)[c] += I.red[sCgAlpha] * static_cast<double>(p[c]);
 }
 
-__global__ void __launch_bounds__(kColThreads) k_lcg_init(const Inst* insts, int pass) { lcgInit<double>(insts, pass); }
-__global__ void __launch_bounds__(kBlock) k_lcg_a(const Inst* insts, int it) {
-  __shared__ double sm[16];
-  lcgA<double>(insts, it, sm);
-}
-__global__ void __launch_bounds__(kColThreads) k_lcg_b(const Inst* insts, int it) { lcgB<double>(insts, it); }
-__global__ void __launch_bounds__(kBlock) k_lcg_final(const Inst* insts) { lcgFinal<double>(insts); }
+// The launches in the storage type T, with the shapes the launch tuning picks
+// from: launch A with 1, 2 or 4 cells per thread, launch B walking its columns
+// 16 or 32 rows ahead.
+#define CFD_LCG_KERNELS(SUFFIX, T)                                                                     \
+  __global__ void __launch_bounds__(kColThreads) k_lcg_init##SUFFIX(const Inst* insts, int pass) {      \
+    lcgInit<T>(insts, pass);                                                                           \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kBlock) k_lcg_a##SUFFIX##_c1(const Inst* insts, int it) {           \
+    __shared__ double sm[16];                                                                          \
+    lcgA<T, 1>(insts, it, sm);                                                                         \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kBlock) k_lcg_a##SUFFIX##_c2(const Inst* insts, int it) {           \
+    __shared__ double sm[16];                                                                          \
+    lcgA<T, 2>(insts, it, sm);                                                                         \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kBlock) k_lcg_a##SUFFIX##_c4(const Inst* insts, int it) {           \
+    __shared__ double sm[16];                                                                          \
+    lcgA<T, 4>(insts, it, sm);                                                                         \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kColThreads) k_lcg_b##SUFFIX##_u16(const Inst* insts, int it, int cellsA) { \
+    lcgB<T, 16>(insts, it, cellsA);                                                                    \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kColThreads) k_lcg_b##SUFFIX##_u32(const Inst* insts, int it, int cellsA) { \
+    lcgB<T, 32>(insts, it, cellsA);                                                                    \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kBlock) k_lcg_final##SUFFIX(const Inst* insts) { lcgFinal<T>(insts); }
 
-__global__ void __launch_bounds__(kColThreads) k_lcg_init_f32(const Inst* insts, int pass) {
-  lcgInit<float>(insts, pass);
-}
-__global__ void __launch_bounds__(kBlock) k_lcg_a_f32(const Inst* insts, int it) {
-  __shared__ double sm[16];
-  lcgA<float>(insts, it, sm);
-}
-__global__ void __launch_bounds__(kColThreads) k_lcg_b_f32(const Inst* insts, int it) { lcgB<float>(insts, it); }
-__global__ void __launch_bounds__(kBlock) k_lcg_final_f32(const Inst* insts) { lcgFinal<float>(insts); }
+CFD_LCG_KERNELS(, double)
+CFD_LCG_KERNELS(_f32, float)
 
 // ---------------------------------------------------------------------------
 // FP64-residual refinement of the pressure correction

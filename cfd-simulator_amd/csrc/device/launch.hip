@@ -34,11 +34,17 @@ __global__ void k_sst_wall(const Inst*);
 __global__ void k_sst_mut(const Inst*);
 __global__ void k_cg_init_f32(const Inst*, int);
 __global__ void k_lcg_init(const Inst*, int);
-__global__ void k_lcg_a(const Inst*, int);
-__global__ void k_lcg_b(const Inst*, int);
+__global__ void k_lcg_a_c1(const Inst*, int);
+__global__ void k_lcg_a_c2(const Inst*, int);
+__global__ void k_lcg_a_c4(const Inst*, int);
+__global__ void k_lcg_b_u16(const Inst*, int, int);
+__global__ void k_lcg_b_u32(const Inst*, int, int);
 __global__ void k_lcg_init_f32(const Inst*, int);
-__global__ void k_lcg_a_f32(const Inst*, int);
-__global__ void k_lcg_b_f32(const Inst*, int);
+__global__ void k_lcg_a_f32_c1(const Inst*, int);
+__global__ void k_lcg_a_f32_c2(const Inst*, int);
+__global__ void k_lcg_a_f32_c4(const Inst*, int);
+__global__ void k_lcg_b_f32_u16(const Inst*, int, int);
+__global__ void k_lcg_b_f32_u32(const Inst*, int, int);
 __global__ void k_lcg_final(const Inst*);
 __global__ void k_lcg_final_f32(const Inst*);
 __global__ void k_refine_acc(const Inst*, int);
@@ -120,32 +126,40 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     const dim3 cgGrid(cfg.maxCgBlocks, cfg.batch);
     const dim3 col(kColThreads);
     for (int rp = 0; rp < refine; ++rp) {
-      if (line && mixed) {
-        hipLaunchKernelGGL(k_lcg_init_f32, colGrid, col, 0, s, d, rp);
+      if (line) {
+        // the launch tuning's shapes (cfdgpu_set_launch_tuning)
+        using KA = void (*)(const Inst*, int);
+        using KB = void (*)(const Inst*, int, int);
+        const int cells = cfg.lcgCells;
+        const KA ka = mixed ? (cells == 1 ? k_lcg_a_f32_c1 : cells == 4 ? k_lcg_a_f32_c4 : k_lcg_a_f32_c2)
+                            : (cells == 1 ? k_lcg_a_c1 : cells == 4 ? k_lcg_a_c4 : k_lcg_a_c2);
+        const KB kb = mixed ? (cfg.colUnroll == 16 ? k_lcg_b_f32_u16 : k_lcg_b_f32_u32)
+                            : (cfg.colUnroll == 16 ? k_lcg_b_u16 : k_lcg_b_u32);
+        const dim3 aGrid(static_cast<unsigned>(cfg.maxLcgBlocks), static_cast<unsigned>(cfg.batch));
+        hipLaunchKernelGGL(mixed ? k_lcg_init_f32 : k_lcg_init, colGrid, col, 0, s, d, rp);
+        begin(CFDGPU_STAGE_CG_ITERATIONS);
         for (int it = 0; it < cfg.cgIters; ++it) {
-          hipLaunchKernelGGL(k_lcg_a_f32, cgGrid, block, 0, s, d, it);
-          hipLaunchKernelGGL(k_lcg_b_f32, colGrid, col, 0, s, d, it);
+          hipLaunchKernelGGL(ka, aGrid, block, 0, s, d, it);
+          hipLaunchKernelGGL(kb, colGrid, col, 0, s, d, it, cells);
         }
-        hipLaunchKernelGGL(k_lcg_final_f32, grid, block, 0, s, d);
-      } else if (line) {
-        hipLaunchKernelGGL(k_lcg_init, colGrid, col, 0, s, d, rp);
-        for (int it = 0; it < cfg.cgIters; ++it) {
-          hipLaunchKernelGGL(k_lcg_a, cgGrid, block, 0, s, d, it);
-          hipLaunchKernelGGL(k_lcg_b, colGrid, col, 0, s, d, it);
-        }
-        hipLaunchKernelGGL(k_lcg_final, grid, block, 0, s, d);
+        end(CFDGPU_STAGE_CG_ITERATIONS);
+        hipLaunchKernelGGL(mixed ? k_lcg_final_f32 : k_lcg_final, grid, block, 0, s, d);
       } else if (mixed) {
         hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d, rp);
+        begin(CFDGPU_STAGE_CG_ITERATIONS);
         for (int it = 0; it < cfg.cgIters; ++it) {
           hipLaunchKernelGGL(k_cg_a_f32, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_cg_b_f32, cgGrid, block, 0, s, d, it);
         }
+        end(CFDGPU_STAGE_CG_ITERATIONS);
       } else {
         hipLaunchKernelGGL(k_cg_init, cgGrid, block, 0, s, d, rp);
+        begin(CFDGPU_STAGE_CG_ITERATIONS);
         for (int it = 0; it < cfg.cgIters; ++it) {
           hipLaunchKernelGGL(k_cg_a, cgGrid, block, 0, s, d, it);
           hipLaunchKernelGGL(k_cg_b, cgGrid, block, 0, s, d, it);
         }
+        end(CFDGPU_STAGE_CG_ITERATIONS);
       }
       if (rp + 1 < refine) {
         // FP64-residual refinement: accumulate the correction, then the

@@ -140,8 +140,10 @@ __host__ __device__ inline int rbIndex(int i, int j, int nI, int nJ) {
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
-constexpr int kColThreads = 64;
-constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)      // columns per block of a column-parallel launch (one wave)
+constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)
+constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)
+// blocks of the line CG's launch A for n cells at `cells` cells per thread
+__host__ __device__ constexpr int lineBlocksA(int n, int cells) { return (n + kBlock * cells - 1) / (kBlock * cells); }
 
 struct Inst {
   int nI, nJ, N;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -47,23 +48,30 @@ class StageClock final : public StageRecorder {
   void reset() {
     used_ = 0;
     marks_.clear();
+    open_.clear();
   }
+  // stages may nest (the CG iterations inside the pressure solve)
   void begin(int stage, hipStream_t s) override {
-    marks_.push_back({stage, used_});
+    open_.push_back(marks_.size());
+    marks_.push_back({stage, used_, used_});
     record(s);
   }
-  void end(int, hipStream_t s) override { record(s); }
+  void end(int, hipStream_t s) override {
+    marks_[open_.back()].end = used_;
+    open_.pop_back();
+    record(s);
+  }
   // after the stream has drained
   void collect(double* ms, int* calls) const {
     for (int q = 0; q < CFDGPU_STAGE_COUNT; ++q) {
       if (ms) ms[q] = 0.0;
       if (calls) calls[q] = 0;
     }
-    for (const auto& m : marks_) {
+    for (const Mark& m : marks_) {
       float t = 0.0f;
-      check(hipEventElapsedTime(&t, pool_[m.second], pool_[m.second + 1]), "stage timing");
-      if (ms) ms[m.first] += static_cast<double>(t);
-      if (calls) calls[m.first] += 1;
+      check(hipEventElapsedTime(&t, pool_[m.begin], pool_[m.end]), "stage timing");
+      if (ms) ms[m.stage] += static_cast<double>(t);
+      if (calls) calls[m.stage] += 1;
     }
   }
 
@@ -77,9 +85,14 @@ class StageClock final : public StageRecorder {
     check(hipEventRecord(pool_[used_], s), "hipEventRecord");
     ++used_;
   }
+  struct Mark {
+    int stage;
+    size_t begin, end;  // event indices
+  };
   std::vector<hipEvent_t> pool_;
   size_t used_{0};
-  std::vector<std::pair<int, size_t>> marks_;
+  std::vector<Mark> marks_;
+  std::vector<size_t> open_;  // marks begun and not yet ended
 };
 
 DeviceArena::~DeviceArena() {
@@ -343,6 +356,7 @@ void Engine::fillInstScalars() {
   h_.momSweeps = settings_.momentum_sweeps;
   h_.cgMax = settings_.pressure_iterations;
   cfg_.cgIters = h_.cgMax;
+  applyTuning(cfg_, grid_.cells());
   h_.cgTol = settings_.pressure_tolerance;
   h_.nonOrthPasses = std::max(1, settings_.non_orthogonal_correctors);
   h_.gsOrder = settings_.gs_ordering;
@@ -658,6 +672,7 @@ bool graphsEnabled() {
 
 bool IterationGraph::Key::operator==(const Key& o) const {
   return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && maxColBlocks == o.maxColBlocks &&
+         lcgCells == o.lcgCells && colUnroll == o.colUnroll && maxLcgBlocks == o.maxLcgBlocks &&
          batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
          precision == o.precision && pSolver == o.pSolver && refinePasses == o.refinePasses &&
          momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgIters == o.cgIters && hasPRef == o.hasPRef &&
@@ -665,23 +680,49 @@ bool IterationGraph::Key::operator==(const Key& o) const {
 }
 
 IterationGraph::Key IterationGraph::keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h) {
-  return Key{d,           cfg.maxBlocks,   cfg.maxCgBlocks, cfg.maxColBlocks, cfg.batch, cfg.gsThreads,
+  return Key{d,           cfg.maxBlocks,   cfg.maxCgBlocks, cfg.maxColBlocks, cfg.lcgCells, cfg.colUnroll,
+             cfg.maxLcgBlocks, cfg.batch, cfg.gsThreads,
              cfg.gsLds,   h.gsOrder,       h.precision,     h.pSolver,        h.refinePasses, h.momSweeps,
              h.nonOrthPasses, cfg.cgIters, h.hasPRef, h.model};
 }
 
+namespace {
+std::mutex gTuningMutex;
+LaunchTuning gTuning;
+}  // namespace
+
+LaunchTuning launchTuning() {
+  std::lock_guard<std::mutex> lock(gTuningMutex);
+  return gTuning;
+}
+
+void setLaunchTuning(const LaunchTuning& t) {
+  std::lock_guard<std::mutex> lock(gTuningMutex);
+  gTuning = t;
+}
+
+void applyTuning(LaunchCfg& cfg, int maxCells) {
+  const LaunchTuning t = launchTuning();
+  cfg.lcgCells = t.lcgCells;
+  cfg.colUnroll = t.colUnroll;
+  cfg.maxLcgBlocks = lineBlocksA(maxCells, t.lcgCells);
+}
+
 LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
-  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, maxNI = 0, cgIters = 0;
+  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, maxNI = 0, cgIters = 0, maxN = 1;
   for (size_t q = 0; q < count; ++q) {
     const Inst& b = insts[q];
     cgIters = std::max(cgIters, b.cgMax);
+    maxN = std::max(maxN, b.N);
     lanes = std::max(lanes, b.gsR);
     blocks = std::max(blocks, b.nBlocks);
     cgBlocks = std::max(cgBlocks, b.cgBlocks);
     colBlocks = std::max(colBlocks, b.colBlocks);
     maxNI = std::max(maxNI, b.nI);
   }
-  return LaunchCfg{blocks, cgBlocks, colBlocks, cgIters, static_cast<int>(count), lanes, gsLdsFor(0, maxNI, lanes)};
+  LaunchCfg cfg{blocks, cgBlocks, colBlocks, cgIters, 2, 32, 1, static_cast<int>(count), lanes, gsLdsFor(0, maxNI, lanes)};
+  applyTuning(cfg, maxN);
+  return cfg;
 }
 
 bool sameSchedule(const Inst& b, const Inst& a) {
@@ -727,6 +768,7 @@ void Engine::checkStall() const {
 
 int Engine::iterate(int count, cfdgpu_monitor* out) {
   if (!initialised_) return CFDGPU_INITIALIZATION_FAILURE;
+  applyTuning(cfg_, grid_.cells());  // (the launch tuning may have changed)
   // a plain iterate() never stops by itself (SimpleSolver::iterate never fails)
   int zero[2] = {0, 0};
   check(hipMemcpyAsync(h_.flags + fStopped, zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
@@ -743,6 +785,7 @@ int Engine::iterateTimed(int count, cfdgpu_monitor* out, double* stageMs, int* s
   if (!clock_) clock_ = std::make_unique<StageClock>();
   StageClock& clock = *clock_;
   clock.reset();
+  applyTuning(cfg_, grid_.cells());
   int zero[2] = {0, 0};
   check(hipMemcpyAsync(h_.flags + fStopped, zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
   for (int n = 0; n < count; ++n) launchIteration(dInst_, cfg_, h_, stream_, &clock);
@@ -765,6 +808,7 @@ int Engine::solve(const cfdgpu_stop_rule& rule, cfdgpu_solve_report* out) {
   h_.stopTol = rule.tolerance;
   h_.stopMaxIter = (rule.max_iterations > 0) ? startIter + rule.max_iterations : 0;
   uploadInst();
+  applyTuning(cfg_, grid_.cells());
   int zero = 0;
   check(hipMemcpyAsync(h_.flags + fStopped, &zero, sizeof(int), hipMemcpyHostToDevice, stream_), "flags");
   const int chunk = std::max(1, rule.check_every);
@@ -1086,6 +1130,7 @@ void Batch::gather() {
   }
   check(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),
         "upload batch descriptors");
+  cfg_ = launchCfgFor(host_.data(), host_.size());  // (the launch tuning may have changed)
 }
 
 int Batch::iterate(int count, cfdgpu_monitor* out) {

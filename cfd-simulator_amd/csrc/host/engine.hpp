@@ -27,6 +27,9 @@ struct LaunchCfg {
   int maxColBlocks; // grid.x for the column-parallel kernels (line-preconditioned CG)
   int cgIters;      // CG iterations launched per solve: the largest cap of the batch (a member
                     // with a smaller cap stops its CG at its own)
+  int lcgCells;     // line CG launch A: cells per thread (1, 2, 4)
+  int colUnroll;    // line CG launch B: rows of a column loaded ahead (16, 32)
+  int maxLcgBlocks; // grid.x of the line CG's launch A
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
@@ -43,6 +46,16 @@ void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hip
                      StageRecorder* rec = nullptr);
 // The launch shape of a batch of instances (the widest member's grid of each kind).
 LaunchCfg launchCfgFor(const Inst* insts, size_t count);
+// Process-wide launch tuning of the line CG (cfdgpu_set_launch_tuning); read
+// whenever a launch shape is formed, so a change takes effect at the next call
+// (a captured iteration graph is captured again).
+struct LaunchTuning {
+  int lcgCells{4};
+  int colUnroll{32};
+};
+LaunchTuning launchTuning();
+void setLaunchTuning(const LaunchTuning& t);
+void applyTuning(LaunchCfg& cfg, int maxCells);
 // Members of one batch share everything that shapes the launch schedule.
 bool sameSchedule(const Inst& a, const Inst& b);
 void launchDivergence(const Inst* dInst, const LaunchCfg& cfg, double* out, hipStream_t s);
@@ -88,7 +101,7 @@ class IterationGraph {
  private:
   struct Key {
     const Inst* d;
-    int maxBlocks, maxCgBlocks, maxColBlocks, batch, gsThreads;
+    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, batch, gsThreads;
     size_t gsLds;
     int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgIters, hasPRef, model;
     bool operator==(const Key& o) const;

@@ -315,6 +315,8 @@ void Server::launchChunk() {
     hipCheck(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),
              "upload roster");
     rosterDirty_ = false;
+  } else {
+    cfg_ = launchCfgFor(host_.data(), host_.size());  // (the launch tuning may have changed)
   }
   graph_.run(dInsts_, cfg_, host_.front(), stream_, opt_.chunk);
   hipCheck(hipGetLastError(), "launch");

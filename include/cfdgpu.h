@@ -80,7 +80,8 @@ enum {
   CFDGPU_STAGE_SST_WALL_VISCOSITY = 10,/* wall omega + eddy viscosity + effective viscosity */
   CFDGPU_STAGE_MONITOR = 11,           /* reductions, SolverMonitor, stop rule */
   CFDGPU_STAGE_SWEEP_COPY = 12,        /* skewed sweep results copied to the fields (+ k, omega floors) */
-  CFDGPU_STAGE_COUNT = 13
+  CFDGPU_STAGE_CG_ITERATIONS = 13,     /* the CG iterations alone: part of PRESSURE_SOLVE, not added to it */
+  CFDGPU_STAGE_COUNT = 14
 };
 
 typedef struct cfdgpu_mesh cfdgpu_mesh;
@@ -282,6 +283,16 @@ int cfdgpu_divergence(const cfdgpu_solver* solver, double* out);
    the process ends; larger meshes are rejected with CFDGPU_INVALID_ARGUMENT. Calls that overlap on
    one GPU share that array and race. Same values as cfdgpu_divergence. */
 int cfdgpu_divergence_noalloc(const cfdgpu_solver* solver, double* out);
+
+/* Launch shapes of the line-preconditioned CG (CFDGPU_PSOLVE_LINE_CG), process-wide: cells per
+   thread of the cell-parallel launch (1, 2 or 4) and rows of a column loaded ahead by the
+   column sweep (16 or 32). Every shape computes the same iterates; the block partial sums of
+   the cell-parallel launch split differently, so results agree to rounding (each shape is
+   deterministic). The next iterate, solve, batch call or server chunk uses the new shapes. No
+   reference counterpart: the hook of the launch tuner (cfd-simulator_amd/tune.py).
+   INVALID_ARGUMENT for other values. Defaults 4 and 32. */
+int cfdgpu_set_launch_tuning(int cells_per_thread, int column_rows);
+int cfdgpu_get_launch_tuning(int* cells_per_thread, int* column_rows);
 
 /* ---- post-processing, FP64 (SurfaceData.hpp:159, Forces.hpp:127-133) ----
  * rows (12 doubles per station, upper surface then lower): side(1 upper/0 lower), face, x, y,

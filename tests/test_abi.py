@@ -46,6 +46,21 @@ def test_host_side_validation_needs_no_gpu(G):
     assert lib.cfdgpu_server_create(C.byref(bad), C.byref(h)) == 1  # InvalidArgument
 
 
+def test_launch_tuning_is_validated_and_kept(G):
+    before = G.launch_tuning()
+    assert before in {(c, r) for c in (1, 2, 4) for r in (16, 32)}
+    try:
+        for c, r in ((1, 16), (4, 32), (2, 32)):
+            G.set_launch_tuning(c, r)
+            assert G.launch_tuning() == (c, r)
+        for c, r in ((3, 32), (0, 16), (2, 8), (2, 64)):
+            with pytest.raises(G.CfdError):
+                G.set_launch_tuning(c, r)
+            assert G.launch_tuning() == (2, 32)  # a rejected call changes nothing
+    finally:
+        G.set_launch_tuning(*before)
+
+
 def test_server_create_fails_loudly_without_a_device(G):
     if G.device_count() > 0:
         pytest.skip("a GPU is visible")

@@ -59,6 +59,57 @@ def test_line_cg_matches_its_definition(G, gpu_available, precision, iterations,
         assert err < 1e-4
 
 
+@pytest.fixture
+def restore_tuning(G):
+    before = G.launch_tuning()
+    yield
+    G.set_launch_tuning(*before)
+
+
+@pytest.mark.parametrize("shape", [(1, 16), (1, 32), (2, 16), (4, 16), (4, 32)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_every_launch_shape_matches_the_definition(G, gpu_available, restore_tuning, precision, shape):
+    """The tuner's launch shapes (cells per thread of launch A, rows loaded
+    ahead by launch B) against the NumPy line CG, with the exit path (cap 500)."""
+    print(f"case: 118x27 SST, line CG ({'fp64' if precision == 0 else 'fp32'}), launch shape {shape}, cap 500",
+          flush=True)
+    G.set_launch_tuning(*shape)
+    mesh, s, st, mon = _cgrid_solver(G, precision, 2, pressure_iterations=500)
+    A, dP, pN, b = _system(mesh, s)
+    x, its = L.line_cg(A, dP, pN, b, mesh.ni, mesh.nj, st.pressure_tolerance, st.pressure_iterations)
+    err = np.max(np.abs(s.debug_array("correction") - x)) / np.max(np.abs(x))
+    print(f"CG iterations {mon.pressure_iterations} (NumPy {its}), relative difference {err:.3e}", flush=True)
+    if precision == 0:
+        assert mon.pressure_iterations == its and its < 500
+        assert err < 1e-10
+    else:
+        assert abs(mon.pressure_iterations - its) <= 2
+        assert err < 1e-3
+
+
+def test_launch_shape_changes_between_calls(G, gpu_available, restore_tuning):
+    """A shape set between two calls takes effect at the next (the captured
+    iteration graph is captured again): 10 + 10 iterations under two shapes
+    agree with 20 under one to rounding, and a batch member under a shape is
+    bit-identical to its solve alone under the same shape."""
+    print("case: 118x27 SST fast path, shapes (1, 16) then (4, 32), batch of two under (4, 16)", flush=True)
+    G.set_launch_tuning(1, 16)
+    _, one, _, _ = _cgrid_solver(G, 1, 20)
+    _, two, _, _ = _cgrid_solver(G, 1, 10)
+    G.set_launch_tuning(4, 32)
+    two.iterate(10)
+    for a, b in zip(one.field(), two.field()):
+        assert np.max(np.abs(a - b)) <= 1e-4 * max(np.max(np.abs(a)), 1e-30)
+    G.set_launch_tuning(4, 16)
+    _, alone, _, _ = _cgrid_solver(G, 1, 0)
+    _, member, _, _ = _cgrid_solver(G, 1, 0)
+    _, other, _, _ = _cgrid_solver(G, 1, 0, alpha=-3.0)
+    alone.iterate(15)
+    G.Batch([member, other]).iterate(15)
+    for a, b in zip(alone.field(), member.field()):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("passes", [2, 3])
 def test_refinement_matches_its_definition(G, gpu_available, passes):
     print(f"case: 118x27 SST, line CG FP64, {passes} refinement passes", flush=True)
