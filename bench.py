@@ -701,7 +701,7 @@ def serve_batch(G, W, rows, st, label, tolerance, max_iterations, chunk, slots, 
         if stats.clock_s - last >= progress:
             last = stats.clock_s
             print(f"[{label}] {stats.clock_s:7.1f}s resident {stats.active} finished {stats.finished} "
-                  f"converged {stats.converged}", flush=True)
+                  f"converged {stats.converged}", file=sys.stderr, flush=True)
         if stats.queued == 0 and stats.active == 0:
             break
         if wall_limit > 0 and stats.clock_s >= wall_limit:
@@ -831,7 +831,8 @@ def alone_comparison(G, W, rows, results, batched_seconds):
         t0 = time.perf_counter()
         s.iterate(100)
         per_level[lv] = (time.perf_counter() - t0) / 100
-        print(f"alone: {lv} {mesh.cells} cells {per_level[lv] * 1e3:.3f} ms per outer iteration", flush=True)
+        print(f"alone: {lv} {mesh.cells} cells {per_level[lv] * 1e3:.3f} ms per outer iteration", file=sys.stderr,
+              flush=True)
     level_of = {q: rows[q]["level"] for q in range(len(rows))}
     seconds = sum(r.iterations * per_level[level_of[r.ticket]] for r in results)
     converged = sum(1 for r in results if r.status == 0)

@@ -559,16 +559,19 @@ __device__ __forceinline__ void rbDefect(const float* const (&A)[NS][5], const f
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int color, int first, int last) {
-  FAST_CELL_PROLOGUE
-  if (!live) return;
-  const bool mine = ((i + j) & 1) == color;
-  if (!mine && !last) return;
-  const int pk = rbIndex(i, j, nI, nJ);
+// One colour of one sweep, one lane per cell of that colour (the colour-packed
+// order, so every lane of a wave has a row to solve and its loads are
+// contiguous). The launch that ends the last sweep also applies the other
+// colour's corrections, lane q taking the other colour's cell q.
+template <int NS>
+__device__ __forceinline__ void rbSolveCell(const float* const (&A)[NS][5], const float* const (&R)[NS],
+                                            float* const (&D)[NS], const Inst& I, int i, int j, int pk, bool first,
+                                            float (&out)[NS]) {
+  const int nI = I.nI, nJ = I.nJ;
   // the four neighbours in cellFaces order: below (or the wake partner), right, above, left
   int nb[4] = {0, 0, 0, 0};
   bool has[4] = {false, false, false, false};
-  if (mine && !first) {
+  if (!first) {
     if (j > 0) {
       nb[0] = rbIndex(i, j - 1, nI, nJ);
       has[0] = true;
@@ -583,43 +586,53 @@ __global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int
     if (j < nJ - 1) { nb[2] = rbIndex(i, j + 1, nI, nJ); has[2] = true; }
     if (i > 0) { nb[3] = rbIndex(i - 1, j, nI, nJ); has[3] = true; }
   }
+  rbDefect<NS>(A, R, D, pk, nb, has, first, out);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) D[s][pk] = out[s];
+}
+
+__device__ __forceinline__ void rbApply(const Inst& I, int sys, int c, const float (&d)[2]) {
   if (sys == 0) {
-    // u and v share the coefficients
-    const float* const A[2][5] = {{I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]},
-                                  {I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]}};
-    const float* const R[2] = {I.fMomRes[0], I.fMomRes[1]};
-    float* const D[2] = {I.fMomDelta[0], I.fMomDelta[1]};
-    float du[2];
-    if (mine) {
-      rbDefect<2>(A, R, D, pk, nb, has, first != 0, du);
-      D[0][pk] = du[0];
-      D[1][pk] = du[1];
-    } else {
-      du[0] = D[0][pk];
-      du[1] = D[1][pk];
-    }
-    if (last) {
-      I.u[c] += static_cast<double>(du[0]);
-      I.v[c] += static_cast<double>(du[1]);
-    }
+    I.u[c] += static_cast<double>(d[0]);
+    I.v[c] += static_cast<double>(d[1]);
   } else {
-    const float* const A[2][5] = {{I.fTurb[0][0], I.fTurb[0][1], I.fTurb[0][2], I.fTurb[0][3], I.fTurb[0][4]},
-                                  {I.fTurb[1][0], I.fTurb[1][1], I.fTurb[1][2], I.fTurb[1][3], I.fTurb[1][4]}};
-    const float* const R[2] = {I.fTurbRes[0], I.fTurbRes[1]};
-    float* const D[2] = {I.fTurbDelta[0], I.fTurbDelta[1]};
-    float dt[2];
-    if (mine) {
-      rbDefect<2>(A, R, D, pk, nb, has, first != 0, dt);
-      D[0][pk] = dt[0];
-      D[1][pk] = dt[1];
-    } else {
-      dt[0] = D[0][pk];
-      dt[1] = D[1][pk];
-    }
-    if (last) {
-      I.k[c] = mx(I.k[c] + static_cast<double>(dt[0]), 0.0);
-      I.w[c] = mx(I.w[c] + static_cast<double>(dt[1]), 1.0e-10);
-    }
+    I.k[c] = mx(I.k[c] + static_cast<double>(d[0]), 0.0);
+    I.w[c] = mx(I.w[c] + static_cast<double>(d[1]), 1.0e-10);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int color, int first, int last) {
+  const Inst& I = insts[blockIdx.y];
+  const int nI = I.nI, nJ = I.nJ, N = I.N;
+  const int count0 = rbRowBase(nJ, 0, nI);  // cells of colour 0
+  const int nb = rbBlocks(nI, nJ);
+  if (static_cast<int>(blockIdx.x) >= nb) return;
+  if (I.flags[fStopped]) return;
+  const int q = xcdTile(static_cast<int>(blockIdx.x), nb) * kBlock + static_cast<int>(threadIdx.x);
+  const int countC = color ? N - count0 : count0;
+  const int countO = color ? count0 : N - count0;
+  const float* const Am[2][5] = {{I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]},
+                                 {I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]}};
+  const float* const At[2][5] = {{I.fTurb[0][0], I.fTurb[0][1], I.fTurb[0][2], I.fTurb[0][3], I.fTurb[0][4]},
+                                 {I.fTurb[1][0], I.fTurb[1][1], I.fTurb[1][2], I.fTurb[1][3], I.fTurb[1][4]}};
+  const float* const R[2] = {sys == 0 ? I.fMomRes[0] : I.fTurbRes[0], sys == 0 ? I.fMomRes[1] : I.fTurbRes[1]};
+  float* const D[2] = {sys == 0 ? I.fMomDelta[0] : I.fTurbDelta[0], sys == 0 ? I.fMomDelta[1] : I.fTurbDelta[1]};
+  if (q < countC) {
+    int i, j;
+    rbCell(q, color, nI, i, j);
+    const int pk = (color ? count0 : 0) + q;
+    float d[2];
+    // u and v share their coefficients (sys 0); k and omega have their own
+    if (sys == 0) rbSolveCell<2>(Am, R, D, I, i, j, pk, first != 0, d);
+    else rbSolveCell<2>(At, R, D, I, i, j, pk, first != 0, d);
+    if (last) rbApply(I, sys, j * nI + i, d);
+  }
+  if (last && q < countO) {
+    int i, j;
+    rbCell(q, 1 - color, nI, i, j);
+    const int pk = (color ? 0 : count0) + q;
+    const float d[2] = {D[0][pk], D[1][pk]};
+    rbApply(I, sys, j * nI + i, d);
   }
 }
 

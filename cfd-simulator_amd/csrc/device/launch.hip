@@ -78,10 +78,11 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
   const dim3 colGrid(static_cast<unsigned>(cfg.maxColBlocks), static_cast<unsigned>(cfg.batch));
   // n sweeps of one system in two colours; the first colour of the first
   // sweep reads no neighbour, the last colour of the last applies the result
+  const dim3 rbGrid(static_cast<unsigned>(cfg.maxRbBlocks), static_cast<unsigned>(cfg.batch));
   auto rbSweeps = [&](int sys, int n) {
     for (int sw = 0; sw < n; ++sw)
       for (int color = 0; color < 2; ++color)
-        hipLaunchKernelGGL(k_rb32, grid, block, 0, s, d, sys, color, (sw == 0 && color == 0) ? 1 : 0,
+        hipLaunchKernelGGL(k_rb32, rbGrid, block, 0, s, d, sys, color, (sw == 0 && color == 0) ? 1 : 0,
                            (sw == n - 1 && color == 1) ? 1 : 0);
   };
   auto begin = [&](int stage) { if (rec) rec->begin(stage, s); };

@@ -137,8 +137,30 @@ __host__ __device__ inline int rbIndex(int i, int j, int nI, int nJ) {
   const int C = (i + j) & 1;
   return (C ? rbRowBase(nJ, 0, nI) : 0) + rbRowBase(j, C, nI) + (i >> 1);
 }
+// The cell (i, j) of colour C at place q of that colour's packed order
+// (q < its count; the inverse of rbIndex within one colour): two consecutive
+// rows 2r, 2r + 1 hold nI cells of colour C, the first (nI - C + 1) / 2 of
+// them in row 2r from column C, the rest in row 2r + 1 from column 1 - C.
+__host__ __device__ inline void rbCell(int q, int C, int nI, int& i, int& j) {
+  const int r = q / nI;
+  const int rem = q - r * nI;
+  const int n0 = (nI - C + 1) / 2;
+  if (rem < n0) {
+    j = 2 * r;
+    i = C + 2 * rem;
+  } else {
+    j = 2 * r + 1;
+    i = (1 - C) + 2 * (rem - n0);
+  }
+}
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
+// Blocks of a one-lane-per-cell launch over the larger colour.
+__host__ __device__ inline int rbBlocks(int nI, int nJ) {
+  const int c0 = rbRowBase(nJ, 0, nI), c1 = nI * nJ - c0;
+  const int m = c0 > c1 ? c0 : c1;
+  return (m + kBlock - 1) / kBlock;
+}
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
 constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)
 constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)

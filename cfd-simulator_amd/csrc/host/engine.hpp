@@ -30,6 +30,7 @@ struct LaunchCfg {
   int lcgCells;     // line CG launch A: cells per thread (1, 2, 4)
   int colUnroll;    // line CG launch B: rows of a column loaded ahead (16, 32)
   int maxLcgBlocks; // grid.x of the line CG's launch A
+  int maxRbBlocks;  // grid.x of the two-colour FP32 sweep (one lane per cell of a colour)
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
@@ -101,7 +102,7 @@ class IterationGraph {
  private:
   struct Key {
     const Inst* d;
-    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, batch, gsThreads;
+    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, maxRbBlocks, batch, gsThreads;
     size_t gsLds;
     int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgIters, hasPRef, model;
     bool operator==(const Key& o) const;
