@@ -612,18 +612,68 @@ void lduMultiply(const Mesh& mesh, const double* diag, const double* upper, cons
   }
 }
 
-double residualL1(const Mesh& mesh, const Ldu& s, const std::vector<double>& x) {  // :45-54
-  std::vector<double> ax(s.diag.size());
-  lduMultiply(mesh, s.diag.data(), s.upper.data(), s.lower.data(), x.data(), ax.data());
+// Order of the global sums below (L1 norms, CG inner products, means). 0, the
+// default, is the reference's: one sequential pass from cell 0. A positive
+// value B sums consecutive blocks of B cells sequentially and then the block
+// totals in order; -1 sums pairwise (a balanced tree). Both are the same
+// terms in another association, used only to measure how far a solve's
+// answer moves with the rounding of its global sums (a device sums in blocks
+// and trees).
+int gSumBlock = 0;
+
+// Evaluation of the SST blending tanh: 0, the default, is std::tanh as in the
+// reference; 1 evaluates (1 - e^-2x) / (1 + e^-2x), which differs from it by
+// a few units in the last place. Like gSumBlock, only for measuring how far
+// a solve's answer moves with the rounding of a device's own tanh.
+int gTanhVariant = 0;
+
+double blendTanh(double x) {
+  if (gTanhVariant == 0) return std::tanh(x);
+  const double e = std::exp(-2.0 * x);
+  return (1.0 - e) / (1.0 + e);
+}
+
+template <class Term>
+double orderedSum(int n, Term term) {
+  if (gSumBlock == 0) {
+    double total = 0.0;
+    for (int c = 0; c < n; ++c) total += term(c);
+    return total;
+  }
+  if (gSumBlock < 0) {
+    // pairwise: adjacent terms, then adjacent pair sums, ... (a balanced tree)
+    std::vector<double> t(static_cast<std::size_t>(n));
+    for (int c = 0; c < n; ++c) t[static_cast<std::size_t>(c)] = term(c);
+    std::size_t m = t.size();
+    while (m > 1) {
+      const std::size_t half = m / 2;
+      for (std::size_t q = 0; q < half; ++q) t[q] = t[2 * q] + t[2 * q + 1];
+      if (m & 1) t[half] = t[m - 1];
+      m = half + (m & 1);
+    }
+    return n > 0 ? t[0] : 0.0;
+  }
   double total = 0.0;
-  for (std::size_t c = 0; c < s.diag.size(); ++c) total += std::abs(s.source[c] - ax[c]);
+  for (int b = 0; b < n; b += gSumBlock) {
+    const int e = std::min(n, b + gSumBlock);
+    double part = 0.0;
+    for (int c = b; c < e; ++c) part += term(c);
+    total += part;
+  }
   return total;
 }
 
+double residualL1(const Mesh& mesh, const Ldu& s, const std::vector<double>& x) {  // :45-54
+  std::vector<double> ax(s.diag.size());
+  lduMultiply(mesh, s.diag.data(), s.upper.data(), s.lower.data(), x.data(), ax.data());
+  return orderedSum(static_cast<int>(s.diag.size()), [&](int c) {
+    return std::abs(s.source[static_cast<std::size_t>(c)] - ax[static_cast<std::size_t>(c)]);
+  });
+}
+
 double diagonalL1(const Ldu& s) {  // :56-62
-  double total = 0.0;
-  for (const double v : s.diag) total += std::abs(v);
-  return total;
+  return orderedSum(static_cast<int>(s.diag.size()),
+                    [&](int c) { return std::abs(s.diag[static_cast<std::size_t>(c)]); });
 }
 
 // LinearSystem.cpp:77-116
@@ -667,10 +717,9 @@ CgReport conjugateGradient(const Mesh& mesh, const double* diag, const double* u
       z(static_cast<std::size_t>(n)), ad(static_cast<std::size_t>(n));
   lduMultiply(mesh, diag, upper, lower, x, ad.data());
   for (int c = 0; c < n; ++c) r[static_cast<std::size_t>(c)] = source[c] - ad[static_cast<std::size_t>(c)];
-  auto norm = [&]() {
-    double t = 0.0;
-    for (const double v : r) t += std::abs(v);
-    return t;
+  auto norm = [&]() { return orderedSum(n, [&](int c) { return std::abs(r[static_cast<std::size_t>(c)]); }); };
+  auto rDotZ = [&]() {
+    return orderedSum(n, [&](int c) { return r[static_cast<std::size_t>(c)] * z[static_cast<std::size_t>(c)]; });
   };
   rep.initial = norm();
   rep.final = rep.initial;
@@ -686,12 +735,11 @@ CgReport conjugateGradient(const Mesh& mesh, const double* diag, const double* u
   };
   precondition();
   d = z;
-  double rz = 0.0;
-  for (int c = 0; c < n; ++c) rz += r[static_cast<std::size_t>(c)] * z[static_cast<std::size_t>(c)];
+  double rz = rDotZ();
   for (int it = 0; it < maxIt; ++it) {
     lduMultiply(mesh, diag, upper, lower, d.data(), ad.data());
-    double dAd = 0.0;
-    for (int c = 0; c < n; ++c) dAd += d[static_cast<std::size_t>(c)] * ad[static_cast<std::size_t>(c)];
+    const double dAd =
+        orderedSum(n, [&](int c) { return d[static_cast<std::size_t>(c)] * ad[static_cast<std::size_t>(c)]; });
     if (!(std::abs(dAd) > 0.0)) break;
     const double step = rz / dAd;
     for (int c = 0; c < n; ++c) {
@@ -705,8 +753,7 @@ CgReport conjugateGradient(const Mesh& mesh, const double* diag, const double* u
       break;
     }
     precondition();
-    double rzNext = 0.0;
-    for (int c = 0; c < n; ++c) rzNext += r[static_cast<std::size_t>(c)] * z[static_cast<std::size_t>(c)];
+    const double rzNext = rDotZ();
     if (!(std::abs(rz) > 0.0)) break;
     const double beta = rzNext / rz;
     rz = rzNext;
@@ -1108,8 +1155,8 @@ void assemblePressure(Solver& s) {
   }
   for (std::size_t c = 0; c < s.netFlux.size(); ++c) s.pres.source[c] = -s.netFlux[c];
   if (!s.hasPressureRef) {
-    double mean = 0.0;
-    for (const double v : s.pres.source) mean += v;
+    double mean = orderedSum(static_cast<int>(s.pres.source.size()),
+                             [&](int c) { return s.pres.source[static_cast<std::size_t>(c)]; });
     mean /= static_cast<double>(s.pres.source.size());
     for (double& v : s.pres.source) v -= mean;
   }
@@ -1279,9 +1326,9 @@ void sstBlending(Solver& s) {
     const double term2 = 500.0 * nu / (d * d * omega);
     const double term3 = 4.0 * rho * k.sigma_w2 * energy / (t.cdPlus[c] * d * d);
     const double arg1 = std::min(std::max(term1, term2), term3);
-    t.f1[c] = std::tanh(std::clamp(arg1 * arg1 * arg1 * arg1, 0.0, 50.0));
+    t.f1[c] = blendTanh(std::clamp(arg1 * arg1 * arg1 * arg1, 0.0, 50.0));
     const double arg2 = std::max(2.0 * term1, term2);
-    t.f2[c] = std::tanh(std::clamp(arg2 * arg2, 0.0, 50.0));
+    t.f2[c] = blendTanh(std::clamp(arg2 * arg2, 0.0, 50.0));
   }
 }
 
@@ -1480,8 +1527,8 @@ void iterateOnce(Solver& s, orc_monitor& mon) {
   for (int pass = 0; pass < std::max(1, s.settings.non_orthogonal_correctors); ++pass) {
     assemblePressure(s);
     if (pass == 0) {
-      double imb = 0.0;
-      for (const double v : s.netFlux) imb += std::abs(v);
+      const double imb = orderedSum(static_cast<int>(s.netFlux.size()),
+                                    [&](int c) { return std::abs(s.netFlux[static_cast<std::size_t>(c)]); });
       mon.mass_imbalance = imb;
       mon.continuity = imb / s.refMassFlow;
     }
@@ -1491,8 +1538,8 @@ void iterateOnce(Solver& s, orc_monitor& mon) {
                                            s.settings.pressure_iterations);
     mon.pressure_iterations += rep.iterations;
     if (!s.hasPressureRef) {
-      double mean = 0.0;
-      for (const double v : corr) mean += v;
+      double mean = orderedSum(static_cast<int>(corr.size()),
+                               [&](int q) { return corr[static_cast<std::size_t>(q)]; });
       mean /= static_cast<double>(corr.size());
       for (double& v : corr) v -= mean;
     }
@@ -1732,6 +1779,9 @@ void orc_face_conditions(const orc_mesh* m, double speed, double alpha_deg, doub
     vxy[2 * f + 1] = v.y;
   }
 }
+
+void orc_set_sum_block(int block) { gSumBlock = block > 0 ? block : (block < 0 ? -1 : 0); }
+void orc_set_tanh_variant(int variant) { gTanhVariant = variant == 1 ? 1 : 0; }
 
 // SimpleSolver.hpp:86-132 defaults
 void orc_default_settings(orc_settings* s) {

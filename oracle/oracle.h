@@ -102,6 +102,15 @@ void orc_face_conditions(const orc_mesh* mesh, double speed, double alpha_deg,
                          double* pressure);
 
 /* solver */
+/* Order of the global sums (L1 norms, CG inner products, means): 0, the
+   default, sums sequentially from cell 0 as the reference does; B > 0 sums
+   blocks of B cells and then the block totals; -1 sums pairwise (process-wide;
+   only for measuring a solve's sensitivity to the rounding of its global
+   sums). */
+void orc_set_sum_block(int block);
+/* SST blending tanh: 0 std::tanh (the default, the reference's); 1 (1 - e^-2x)
+   / (1 + e^-2x), a few ulps away (process-wide; sensitivity measurement only). */
+void orc_set_tanh_variant(int variant);
 void orc_default_settings(orc_settings* out);
 void orc_default_sst(orc_sst_constants* out);
 orc_solver* orc_solver_create(const orc_mesh* mesh, const int* kind, const double* velocity_xy,

@@ -92,6 +92,8 @@ def _load():
         "orc_mesh_cells": (None, [vp, _dp, _dp, _ip]),
         "orc_mesh_faces": (None, [vp, _dp, _dp, _dp, _ip, _ip, _ip, _ip]),
         "orc_face_conditions": (None, [vp, C.c_double, C.c_double, C.c_double, _ip, _dp, _dp]),
+        "orc_set_sum_block": (None, [C.c_int]),
+        "orc_set_tanh_variant": (None, [C.c_int]),
         "orc_default_settings": (None, [C.POINTER(Settings)]),
         "orc_default_sst": (None, [C.POINTER(SstConstants)]),
         "orc_solver_create": (vp, [vp, _ip, _dp, _dp, C.POINTER(Settings)]),
@@ -142,6 +144,19 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+def set_sum_block(block: int) -> None:
+    """0: global sums in the reference's sequential order (the default);
+    B > 0: blocks of B cells, then the block totals; -1: pairwise
+    (process-wide)."""
+    lib().orc_set_sum_block(int(block))
+
+
+def set_tanh_variant(variant: int) -> None:
+    """0: std::tanh in the SST blending functions (the default); 1: (1 - e^-2x)
+    / (1 + e^-2x), a few ulps away (process-wide)."""
+    lib().orc_set_tanh_variant(int(variant))
 
 
 def default_settings() -> Settings:

@@ -7,6 +7,7 @@ when residuals.worst() < tolerance, iteration-limit at the cap.
 Run in the development container only (the GPU tests read the JSON files):
 
     python tests/golden/make_fixtures.py [--jobs 6] [--only c1,c3,c4,c2]
+    python tests/golden/make_fixtures.py --probes sum_block_512,tanh_exp [--only c4] [--names ...]
 
 Cases (chord 1, closed trailing edge, C-grid domain 15 / 15 / 20 chords,
 freestream U = rho = 1, mu = rho U c / Re):
@@ -149,19 +150,114 @@ def run_case(case):
     return case["name"], status, it, rec["seconds"]
 
 
+PROBES = {
+    "sum_block_512": "the oracle's global sums (L1 norms, CG inner products, means) in blocks of 512 cells",
+    "sum_block_64": "the oracle's global sums in blocks of 64 cells",
+    "sum_pairwise": "the oracle's global sums pairwise (a balanced tree)",
+    "tanh_exp": "the SST blending tanh evaluated as (1 - e^-2x) / (1 + e^-2x), a few ulps from std::tanh",
+    "pairwise_tanh_exp": "both: pairwise global sums and the (1 - e^-2x) / (1 + e^-2x) tanh",
+}
+SUM_BLOCK = {"sum_block_512": 512, "sum_block_64": 64, "sum_pairwise": -1, "pairwise_tanh_exp": -1}
+TANH_VARIANT = {"tanh_exp": 1, "pairwise_tanh_exp": 1}
+
+
+def run_probe(job):
+    """The same solve with one round-off perturbation of the oracle (PROBES),
+    to the committed fixture's stop iteration: how far the answer at that
+    iteration moves with rounding alone, the measure of the tolerance a
+    second FP64 implementation (another summation order, another tanh) can
+    be held to there. Stored in the fixture under "sensitivity"[probe]:
+    forces, the largest Cp and Cf changes relative to their largest
+    magnitude, the separation faces, and where the probe's own stop rule
+    fires."""
+    case, probe = job
+    import numpy as np
+    import oracle_py as O
+
+    path = os.path.join(OUT, case["name"] + ".json")
+    with open(path) as fh:
+        rec = json.load(fh)
+    if rec["status"] != 0:
+        return case["name"], probe, None
+    t0 = time.time()
+    O.set_sum_block(SUM_BLOCK.get(probe, 0))
+    O.set_tanh_variant(TANH_VARIANT.get(probe, 0))
+    s_pts, w_pts, n_pts = case["points"]
+    opts = O.cgrid_options("coarse", surface_points=s_pts, wake_points=w_pts, normal_points=n_pts,
+                           first_layer=case["first_layer"], **DOMAIN)
+    mesh = O.cgrid(*case["section"], options=opts)
+    kind, vel, pres = O.face_conditions(mesh, 1.0, case["alpha"])
+    settings = O.default_settings()
+    settings.intensity = 0.001
+    settings.viscosity_ratio = 10.0
+    settings.length_scale = 0.0
+    solver = O.Solver(mesh, kind, vel, pres, settings)
+    if case["model"]:
+        solver.set_model(case["model"])
+    solver.initialise(*O.uniform_field(mesh.cells, speed=1.0, alpha_deg=case["alpha"], density=1.0,
+                                       reynolds=case["reynolds"]))
+    n_stop = rec["iterations"]
+    stop, f, rows = None, None, None
+    it = 0
+    while it < n_stop + 3:
+        mon = solver.iterate(1)
+        it += 1
+        if stop is None and mon.worst() < case["tolerance"]:
+            stop = it
+        if it == n_stop:
+            f, rows = O.surface(mesh, *solver.field(), speed=1.0, alpha_deg=case["alpha"], density=1.0,
+                                reference_pressure=0.0)
+        if it >= n_stop and stop is not None:
+            break
+    cp, cf = np.array(rec["cp"]), np.array(rec["cf"])
+    out = {"probe": PROBES[probe], "stop": stop,
+           "forces": {n: getattr(f, n) for n, _ in O.Forces._fields_},
+           "cp_change": float(np.max(np.abs(rows[:, 7] - cp)) / np.max(np.abs(cp))),
+           "cf_change": float(np.max(np.abs(rows[:, 9] - cf)) / np.max(np.abs(cf))),
+           "upper_separation_face": separation_face(rows, 1), "lower_separation_face": separation_face(rows, 0),
+           "seconds": round(time.time() - t0, 1)}
+    return case["name"], probe, out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=6)
     ap.add_argument("--only", default="c1,c3,c4,c2")
+    ap.add_argument("--names", help="comma-separated fixture names (default: every case of --only)")
     ap.add_argument("--skip-existing", action="store_true")
+    ap.add_argument("--probes", default="",
+                    help="instead of writing fixtures, rerun committed converged ones under these round-off "
+                         "probes (comma-separated: " + ", ".join(PROBES) + ") and record the change "
+                         "(\"sensitivity\")")
     args = ap.parse_args()
     groups = set(args.only.split(","))
     todo = [c for c in cases() if c["group"] in groups]
+    if args.names:
+        names = set(args.names.split(","))
+        todo = [c for c in todo if c["name"] in names]
+    import multiprocessing as mp
+
+    if args.probes:
+        probes = args.probes.split(",")
+        todo = [c for c in todo if os.path.exists(os.path.join(OUT, c["name"] + ".json"))]
+        todo.sort(key=lambda c: -c["expect_s"])
+        jobs = [(c, p) for c in todo for p in probes]
+        with mp.Pool(max(1, args.jobs)) as pool:
+            for name, probe, out in pool.imap_unordered(run_probe, jobs):
+                print(f"{name} {probe}: {out}", flush=True)
+                if out is None:
+                    continue
+                path = os.path.join(OUT, name + ".json")
+                with open(path) as fh:
+                    rec = json.load(fh)
+                rec.setdefault("sensitivity", {})[probe] = out
+                with open(path, "w") as fh:
+                    json.dump(rec, fh, indent=0)
+                    fh.write("\n")
+        return
     if args.skip_existing:
         todo = [c for c in todo if not os.path.exists(os.path.join(OUT, c["name"] + ".json"))]
     todo.sort(key=lambda c: -c["expect_s"])
-    import multiprocessing as mp
-
     with mp.Pool(max(1, args.jobs)) as pool:
         for name, status, it, sec in pool.imap_unordered(run_case, todo):
             print(f"{name}: status {status} after {it} iterations ({sec:.0f} s)", flush=True)

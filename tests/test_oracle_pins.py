@@ -154,6 +154,58 @@ def test_cg_reports_an_already_solved_system():
     assert O.lib().orc_ldu_cg(mesh.h, diag, upper, lower, b, x, 1e-10, 100, C.byref(r0), C.byref(r1)) == 0
 
 
+@pytest.mark.parametrize("block", [0, 7, 64, -1])
+def test_cg_norms_are_plain_sums_in_every_probe_order(block):
+    """The round-off probes (orc_set_sum_block) change only the association of
+    the global sums: the initial residual norm |b|_1 of an integer right-hand
+    side is exact in any order, the CG still recovers the solution, and the
+    default order is restored afterwards."""
+    mesh = O.box(cells_x=12, cells_y=12, grading=0)
+    diag, upper, lower, _ = laplacian(mesh, np.zeros(mesh.cells))
+    rng = np.random.default_rng(3)
+    b = rng.integers(-50, 50, mesh.cells).astype(np.float64)
+    x = np.zeros(mesh.cells)
+    r0, r1 = C.c_double(), C.c_double()
+    O.set_sum_block(block)
+    try:
+        its = O.lib().orc_ldu_cg(mesh.h, diag, upper, lower, b, x, 1e-12, 500, C.byref(r0), C.byref(r1))
+    finally:
+        O.set_sum_block(0)
+    assert r0.value == np.sum(np.abs(b))
+    assert 0 < its < 500
+    np.testing.assert_allclose(dense(mesh, diag, upper, lower) @ x, b, rtol=0, atol=1e-8 * np.abs(b).max())
+
+
+def test_probe_orders_move_a_solve_by_rounding_only():
+    """Three laminar outer iterations with pairwise sums and the alternative
+    tanh stay within 1e-10 of the default run but not bit-identical (the
+    probes are live), and a run after resetting them repeats the default run
+    exactly."""
+    def run():
+        mesh = O.cgrid(0, 0, 12, options=O.cgrid_options("coarse", **SMALL))
+        kind, vel, pres = O.face_conditions(mesh, 1.0, 3.0)
+        s = O.Solver(mesh, kind, vel, pres)
+        s.initialise(*O.uniform_field(mesh.cells, speed=1.0, alpha_deg=3.0, density=1.0, reynolds=500.0))
+        s.iterate(3)
+        return s.field()[:3]
+
+    base = run()
+    O.set_sum_block(-1)
+    O.set_tanh_variant(1)
+    try:
+        probed = run()
+    finally:
+        O.set_sum_block(0)
+        O.set_tanh_variant(0)
+    again = run()
+    moved = False
+    for a, b, c in zip(base, probed, again):
+        assert np.array_equal(a, c)
+        assert np.abs(b - a).max() <= 1e-10 * np.abs(a).max()
+        moved |= not np.array_equal(a, b)
+    assert moved
+
+
 # --- distributions and geometry (Distribution.cpp, Naca4.cpp) ---------------
 
 def test_geometric_distribution():
