@@ -436,6 +436,145 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it, int cellsA) {
   }
 }
 
+// Launch B with every column's rows split over the waves of a block: one
+// block per 64-column tile, wave w owning rows [w L, w L + L) of its 64
+// columns, L = ceil(nJ / kSegWaves) <= kSegRows, all of them in registers
+// (loads along a row are contiguous across the wave's lanes). The forward
+// solve y_j = w_j / pivot_j, w_j = r_j - l_j y_{j-1}, is the affine recurrence
+// y_j = a_j y_{j-1} + b_j (a_j = -l_j inv_j, b_j = r_j inv_j): each wave
+// composes its rows' maps, the value entering each segment is composed from
+// the segments below it through LDS, and each wave then runs the recurrence
+// over its rows from that value; the back substitution z_j = y_j - up_j
+// z_{j+1} is done the same way from the top. A column's 2 nJ dependent steps
+// become 2 (L + kSegWaves) short ones, and r and z are written once (y stays
+// in registers). Same arithmetic per row as lcgB, with the entering values
+// composed in another order.
+constexpr int kSegWaves = kLcgSegThreads / 64;            // waves per block
+constexpr int kSegRows = kLcgSegMaxRows / kSegWaves;      // rows per wave at most
+static_assert(kSegWaves * kSegRows == kLcgSegMaxRows, "segmented launch B shape");
+
+template <typename T>
+__device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
+  const Inst& I = insts[blockIdx.y];
+  if (static_cast<int>(blockIdx.x) >= I.colBlocks) return;
+  if (I.flags[fStopped]) return;
+  if (it >= I.cgMax) return;
+  if (I.flags[fCgDone]) return;
+  __shared__ double bc[1];
+  __shared__ T segA[kSegWaves][64];
+  __shared__ T segB[kSegWaves][64];
+  __shared__ double sums[2][kSegWaves];
+  const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
+  const int nI = I.nI, nJ = I.nJ;
+  const int i = static_cast<int>(blockIdx.x) * 64 + lane;
+  const bool col = i < nI;
+  const int ic = col ? i : nI - 1;  // idle lanes read a real column and store nothing
+  const LineStore<T> S = lineStore<T>(I);
+  const int L = (nJ + kSegWaves - 1) / kSegWaves;
+  const int j0 = wv * L;
+  const int n = max(0, min(nJ, j0 + L) - j0);  // this wave's rows
+  // every operand of this wave's rows, issued before the partial sums
+  T ap[kSegRows], rr[kSegRows], lo[kSegRows], iv[kSegRows], up[kSegRows];
+#pragma unroll
+  for (int u = 0; u < kSegRows; ++u) {
+    const int j = min(j0 + u, nJ - 1);
+    const int c = j * nI + ic;
+    ap[u] = S.Ap[c];
+    rr[u] = S.r[c];
+    iv[u] = S.inv[c];
+    up[u] = S.up[c];
+    lo[u] = S.pN[(j > 0 ? j - 1 : 0) * nI + ic];
+  }
+  if (wv == 0) {
+    const double papLane = waveSumPartials(partialSlot(I, sCgPAp), lineBlocksA(I.N, cellsA));
+    if (lane == 0) bc[0] = papLane;
+  }
+  __syncthreads();
+  const double pap = bc[0];
+  if (!(fabs(pap) > 0.0)) {  // the direction has no component the matrix acts on
+    if (threadIdx.x == 0 && blockIdx.x == 0) I.flags[fCgDone] = 1;
+    return;
+  }
+  const double alpha = I.red[(it & 1) ? sCgRz1 : sCgRz0] / pap;
+  const T alphaT = static_cast<T>(alpha);
+  // r -= alpha Ap; the segment's composed forward map y_out = A y_in + B
+  T A = T(1), B = T(0);
+#pragma unroll
+  for (int u = 0; u < kSegRows; ++u) {
+    if (u < n) {
+      rr[u] = rr[u] - alphaT * ap[u];
+      if (j0 + u == 0) lo[u] = T(0);  // row 0 has no row below
+      const T a = -lo[u] * iv[u];
+      const T b = rr[u] * iv[u];
+      A = a * A;
+      B = a * B + b;
+    }
+  }
+  segA[wv][lane] = A;
+  segB[wv][lane] = B;
+  __syncthreads();
+  T y = T(0);  // y of the row below this wave's first row
+  for (int q = 0; q < wv; ++q) y = segA[q][lane] * y + segB[q][lane];
+  double l1 = 0.0, rz = 0.0;
+  T yv[kSegRows];
+#pragma unroll
+  for (int u = 0; u < kSegRows; ++u) {
+    yv[u] = T(0);
+    if (u < n) {
+      const T w = rr[u] - lo[u] * y;
+      y = w * iv[u];
+      yv[u] = y;
+      if (col) S.r[(j0 + u) * nI + ic] = rr[u];
+      l1 += fabs(static_cast<double>(rr[u]));
+      rz += static_cast<double>(y) * static_cast<double>(w);
+    }
+  }
+  // back substitution: the segment's composed map z_bottom = C z_above + D
+  T Cm = T(1), Dm = T(0);
+#pragma unroll
+  for (int u = kSegRows - 1; u >= 0; --u) {
+    if (u < n) {
+      const T cu = -up[u];
+      Cm = cu * Cm;
+      Dm = cu * Dm + yv[u];
+    }
+  }
+  __syncthreads();  // every wave has read the forward maps
+  segA[wv][lane] = Cm;
+  segB[wv][lane] = Dm;
+  __syncthreads();
+  T z = T(0);  // z of the row above this wave's last row
+  for (int q = kSegWaves - 1; q > wv; --q) z = segA[q][lane] * z + segB[q][lane];
+#pragma unroll
+  for (int u = kSegRows - 1; u >= 0; --u) {
+    if (u < n) {
+      z = yv[u] - up[u] * z;
+      if (col) S.z[(j0 + u) * nI + ic] = z;
+    }
+  }
+  l1 = waveSum(col ? l1 : 0.0);
+  rz = waveSum(col ? rz : 0.0);
+  if (lane == 0) {
+    sums[0][wv] = l1;
+    sums[1][wv] = rz;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tl = 0.0, tr = 0.0;
+    for (int q = 0; q < kSegWaves; ++q) {
+      tl += sums[0][q];
+      tr += sums[1][q];
+    }
+    partialSlot(I, sCgNorm)[blockIdx.x] = tl;
+    partialSlot(I, sCgRz)[blockIdx.x] = tr;
+    if (blockIdx.x == 0) {
+      I.red[sCgAlpha] = alpha;
+      I.flags[fCgIters] = it + 1;
+      I.flags[fCgTotal] += 1;
+    }
+  }
+}
+
 // After the CG iterations of a pass: the x update of the last launch B, when
 // no launch A followed it (the cap was reached).
 template <typename T>
@@ -473,6 +612,9 @@ __device__ __forceinline__ void lcgFinal(const Inst* insts) {
   }                                                                                                    \
   __global__ void __launch_bounds__(kColThreads) k_lcg_b##SUFFIX##_u32(const Inst* insts, int it, int cellsA) { \
     lcgB<T, 32>(insts, it, cellsA);                                                                    \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kSegWaves * 64) k_lcg_b##SUFFIX##_seg(const Inst* insts, int it, int cellsA) { \
+    lcgBSeg<T>(insts, it, cellsA);                                                                     \
   }                                                                                                    \
   __global__ void __launch_bounds__(kBlock) k_lcg_final##SUFFIX(const Inst* insts) { lcgFinal<T>(insts); }
 

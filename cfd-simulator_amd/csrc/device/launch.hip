@@ -39,12 +39,14 @@ __global__ void k_lcg_a_c2(const Inst*, int);
 __global__ void k_lcg_a_c4(const Inst*, int);
 __global__ void k_lcg_b_u16(const Inst*, int, int);
 __global__ void k_lcg_b_u32(const Inst*, int, int);
+__global__ void k_lcg_b_seg(const Inst*, int, int);
 __global__ void k_lcg_init_f32(const Inst*, int);
 __global__ void k_lcg_a_f32_c1(const Inst*, int);
 __global__ void k_lcg_a_f32_c2(const Inst*, int);
 __global__ void k_lcg_a_f32_c4(const Inst*, int);
 __global__ void k_lcg_b_f32_u16(const Inst*, int, int);
 __global__ void k_lcg_b_f32_u32(const Inst*, int, int);
+__global__ void k_lcg_b_f32_seg(const Inst*, int, int);
 __global__ void k_lcg_final(const Inst*);
 __global__ void k_lcg_final_f32(const Inst*);
 __global__ void k_refine_acc(const Inst*, int);
@@ -134,14 +136,19 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         const int cells = cfg.lcgCells;
         const KA ka = mixed ? (cells == 1 ? k_lcg_a_f32_c1 : cells == 4 ? k_lcg_a_f32_c4 : k_lcg_a_f32_c2)
                             : (cells == 1 ? k_lcg_a_c1 : cells == 4 ? k_lcg_a_c4 : k_lcg_a_c2);
-        const KB kb = mixed ? (cfg.colUnroll == 16 ? k_lcg_b_f32_u16 : k_lcg_b_f32_u32)
-                            : (cfg.colUnroll == 16 ? k_lcg_b_u16 : k_lcg_b_u32);
+        // columns of at most 256 rows: the segmented launch B (1024 threads per
+        // 64 columns); taller ones: one lane per column, walking the tuned depth ahead
+        const bool seg = cfg.maxNJ <= kLcgSegMaxRows;
+        const KB kb = seg ? (mixed ? k_lcg_b_f32_seg : k_lcg_b_seg)
+                          : mixed ? (cfg.colUnroll == 16 ? k_lcg_b_f32_u16 : k_lcg_b_f32_u32)
+                                  : (cfg.colUnroll == 16 ? k_lcg_b_u16 : k_lcg_b_u32);
+        const dim3 colB(seg ? kLcgSegThreads : kColThreads);
         const dim3 aGrid(static_cast<unsigned>(cfg.maxLcgBlocks), static_cast<unsigned>(cfg.batch));
         hipLaunchKernelGGL(mixed ? k_lcg_init_f32 : k_lcg_init, colGrid, col, 0, s, d, rp);
         begin(CFDGPU_STAGE_CG_ITERATIONS);
         for (int it = 0; it < cfg.cgIters; ++it) {
           hipLaunchKernelGGL(ka, aGrid, block, 0, s, d, it);
-          hipLaunchKernelGGL(kb, colGrid, col, 0, s, d, it, cells);
+          hipLaunchKernelGGL(kb, colGrid, colB, 0, s, d, it, cells);
         }
         end(CFDGPU_STAGE_CG_ITERATIONS);
         hipLaunchKernelGGL(mixed ? k_lcg_final_f32 : k_lcg_final, grid, block, 0, s, d);

@@ -163,6 +163,8 @@ __host__ __device__ inline int rbBlocks(int nI, int nJ) {
 }
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
 constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)
+constexpr int kLcgSegThreads = 1024; // the line CG's segmented launch B: 16 waves over one 64-column tile
+constexpr int kLcgSegMaxRows = 256;  // ... each wave holding at most 16 rows
 constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)
 // blocks of the line CG's launch A for n cells at `cells` cells per thread
 __host__ __device__ constexpr int lineBlocksA(int n, int cells) { return (n + kBlock * cells - 1) / (kBlock * cells); }

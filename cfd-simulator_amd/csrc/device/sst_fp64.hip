@@ -163,13 +163,13 @@ __device__ __forceinline__ void sstBoundaryRow(const Inst& I, const FaceBC& bc, 
                                                double inflow, double& d, double& b) {
   const double diff = sstDiffusivity<isK>(I, o) * bc.gSst;
   const bool imposed = bc.kind == kWall || bc.kind == kInlet || (bc.kind == kFarField && F < 0.0);
-  if (imposed) {
-    const double value = (bc.kind == kWall) ? wallValue : inflow;
-    d += diff + mx(F, 0.0);
-    b += (diff - mn(F, 0.0)) * value;
-  } else {
-    d += mx(F, 0.0);
-  }
+  const double value = (bc.kind == kWall) ? wallValue : inflow;
+  // both sums updated on every path (b gains +0 where nothing is imposed:
+  // b is never -0, so the sum is unchanged), which keeps d and b in registers
+  const double addD = imposed ? diff + mx(F, 0.0) : mx(F, 0.0);
+  const double addB = imposed ? (diff - mn(F, 0.0)) * value : 0.0;
+  d += addD;
+  b += addB;
 }
 
 template <bool isK>

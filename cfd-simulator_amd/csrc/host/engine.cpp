@@ -322,6 +322,7 @@ void Engine::allocate() {
   cfg_.maxCgBlocks = h_.cgBlocks;
   cfg_.maxColBlocks = h_.colBlocks;
   cfg_.maxRbBlocks = rbBlocks(h_.nI, h_.nJ);
+  cfg_.maxNJ = h_.nJ;
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
   cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
@@ -674,7 +675,7 @@ bool graphsEnabled() {
 bool IterationGraph::Key::operator==(const Key& o) const {
   return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && maxColBlocks == o.maxColBlocks &&
          lcgCells == o.lcgCells && colUnroll == o.colUnroll && maxLcgBlocks == o.maxLcgBlocks &&
-         maxRbBlocks == o.maxRbBlocks && batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
+         maxRbBlocks == o.maxRbBlocks && maxNJ == o.maxNJ && batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
          precision == o.precision && pSolver == o.pSolver && refinePasses == o.refinePasses &&
          momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgIters == o.cgIters && hasPRef == o.hasPRef &&
          model == o.model;
@@ -682,7 +683,7 @@ bool IterationGraph::Key::operator==(const Key& o) const {
 
 IterationGraph::Key IterationGraph::keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h) {
   return Key{d,           cfg.maxBlocks,   cfg.maxCgBlocks, cfg.maxColBlocks, cfg.lcgCells, cfg.colUnroll,
-             cfg.maxLcgBlocks, cfg.maxRbBlocks, cfg.batch, cfg.gsThreads,
+             cfg.maxLcgBlocks, cfg.maxRbBlocks, cfg.maxNJ, cfg.batch, cfg.gsThreads,
              cfg.gsLds,   h.gsOrder,       h.precision,     h.pSolver,        h.refinePasses, h.momSweeps,
              h.nonOrthPasses, cfg.cgIters, h.hasPRef, h.model};
 }
@@ -710,7 +711,7 @@ void applyTuning(LaunchCfg& cfg, int maxCells) {
 }
 
 LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
-  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, rbBlk = 1, maxNI = 0, cgIters = 0, maxN = 1;
+  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, rbBlk = 1, maxNI = 0, maxNJ = 1, cgIters = 0, maxN = 1;
   for (size_t q = 0; q < count; ++q) {
     const Inst& b = insts[q];
     cgIters = std::max(cgIters, b.cgMax);
@@ -721,8 +722,9 @@ LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
     cgBlocks = std::max(cgBlocks, b.cgBlocks);
     colBlocks = std::max(colBlocks, b.colBlocks);
     maxNI = std::max(maxNI, b.nI);
+    maxNJ = std::max(maxNJ, b.nJ);
   }
-  LaunchCfg cfg{blocks, cgBlocks, colBlocks, cgIters, 2, 32, 1, rbBlk, static_cast<int>(count), lanes,
+  LaunchCfg cfg{blocks, cgBlocks, colBlocks, cgIters, 2, 32, 1, rbBlk, maxNJ, static_cast<int>(count), lanes,
                 gsLdsFor(0, maxNI, lanes)};
   applyTuning(cfg, maxN);
   return cfg;

@@ -31,6 +31,7 @@ struct LaunchCfg {
   int colUnroll;    // line CG launch B: rows of a column loaded ahead (16, 32)
   int maxLcgBlocks; // grid.x of the line CG's launch A
   int maxRbBlocks;  // grid.x of the two-colour FP32 sweep (one lane per cell of a colour)
+  int maxNJ;        // the tallest member's rows (the line CG's segmented launch B holds up to 256)
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
@@ -102,7 +103,8 @@ class IterationGraph {
  private:
   struct Key {
     const Inst* d;
-    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, maxRbBlocks, batch, gsThreads;
+    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, maxRbBlocks, maxNJ, batch,
+        gsThreads;
     size_t gsLds;
     int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgIters, hasPRef, model;
     bool operator==(const Key& o) const;
