@@ -47,9 +47,13 @@ UNIT = "converged solves/s"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 # HBM bytes per launch of the dominant kernels from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE, scaled by the same counters on a copy of known size
-# (profiles/naca2412_1024x256_sst_traffic.md); None until measured for this build.
-TRAFFIC_BYTES = {"k_gs_wave_mom": 142.32e6, "k_gs_wave_turb": 135.59e6, "k_cg_a + k_cg_b": 32.08e6}
+# and WRITE_SIZE, scaled by the same counters on a copy of known size (FETCH_SIZE
+# x 2.00, WRITE_SIZE x 1.00): the plain path's single 1024x256 solve
+# (profiles/naca2412_1024x256_sst_traffic.md) and the fast path's line-CG
+# iteration over the 64-instance batch with every member iterating (launch A
+# 290.9 MB + launch B 227.3 MB; profiles/mixed_batch_64_fast_kernels.md).
+TRAFFIC_BYTES = {"k_gs_wave_mom": 142.32e6, "k_gs_wave_turb": 135.59e6, "k_cg_a + k_cg_b": 32.08e6,
+                 "line_cg_iteration_mixed": 518.2e6}
 
 WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
                 first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
@@ -68,12 +72,12 @@ def traffic_model():
         "rhie_chow_pressure": 34,
         "correction": 23,
         "sst_update": 130,
-        # bytes per cell of one line-CG iteration on the mixed path (FP32 storage, FP64 x):
-        # launch A reads z, p_old, aP, aN, aE (4 B each) and x (8), writes x (8), p, Ap (8) = 44
-        # (x += alpha p of the previous iteration rides in launch A); launch B reads Ap, r,
-        # 1/pivot, aN (16) and writes r, z (8), then the back substitution reads z and the
-        # eliminated coupling (8) and writes z (4) = 36
-        "line_cg_iteration_mixed": 80,
+        # bytes per cell of one line-CG iteration on the mixed path (FP32 storage throughout,
+        # the correction x included): launch A reads z, p_old, aP, aN, aE and x (4 B each) and
+        # writes x, p, Ap = 36 (x += alpha p of the previous iteration rides in launch A);
+        # launch B (rows in registers) reads Ap, r, 1/pivot, the eliminated coupling and aN
+        # and writes r, z = 28
+        "line_cg_iteration_mixed": 64,
     }
 
 
@@ -304,11 +308,13 @@ def run_roofline(args):
     if G.device_count() < 1:
         raise SystemExit("bench.py needs a GPU: no HIP device visible")
     rows = W.load(W.MIXED_BATCH_FILE)
+    if args.only:
+        rows = [rows[int(q)] for q in args.only.split(",")]
     fast_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=G.MIXED, gs_ordering=1,
                                  pressure_solver=1, pressure_iterations=args.fast_cg)
     cells, ms_per_step, stage_ms, calls, cg_cells = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
-    print(json.dumps({"diagnostic": "fast path on the 64-instance mixed-size batch: step time and the line-CG "
-                                    "roofline (not the metric)",
+    print(json.dumps({"diagnostic": f"fast path on {len(rows)} instances of the 64-instance mixed-size batch: step "
+                                    "time and the line-CG roofline (not the metric)",
                       "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "cells": cells,
                       "fast_path": settings_summary(fast_st), "launch_tuning": dict(LAUNCH_TUNING),
                       "roofline": batch_roofline(fast_st, cells, stage_ms, calls, cg_cells),
@@ -889,7 +895,7 @@ def main():
     ap.add_argument("--psolver", choices=("jacobi", "line"), help="pressure solver")
     ap.add_argument("--cg-max", type=int, help="pressure CG iteration cap")
     ap.add_argument("--refine", type=int, help="pressure refinement passes")
-    ap.add_argument("--only", help="with --mixed-batch / --serve: comma-separated manifest rows to run")
+    ap.add_argument("--only", help="with --mixed-batch / --serve / --roofline-only: comma-separated manifest rows")
     ap.add_argument("--single", action="store_true",
                     help="the single 1024x256 SST solve (plain FP64 step time, roofline, mixed comparison)")
     ap.add_argument("--roofline-only", action="store_true",

@@ -34,24 +34,26 @@ namespace cfdgpu {
 template <typename T>
 struct LineStore {
   const CFD_GLOBAL T *pP, *pN, *pE, *pWk;
-  CFD_GLOBAL T *inv, *up, *r, *z, *pd0, *pd1, *Ap;
+  CFD_GLOBAL T *inv, *up, *r, *z, *pd0, *pd1, *Ap, *x;
 };
 
 template <typename T>
 __device__ __forceinline__ LineStore<T> lineStoreOf(const T* pP, const T* pN, const T* pE, const T* pWk, T* inv, T* up,
-                                                  T* r, T* z, T* pd0, T* pd1, T* Ap) {
+                                                  T* r, T* z, T* pd0, T* pd1, T* Ap, T* x) {
   return LineStore<T>{gmem(pP), gmem(pN), gmem(pE), gmem(pWk), gmem(inv), gmem(up), gmem(r), gmem(z), gmem(pd0),
-                      gmem(pd1), gmem(Ap)};
+                      gmem(pd1), gmem(Ap), gmem(x)};
 }
 template <typename T>
 __device__ __forceinline__ LineStore<T> lineStore(const Inst& I);
 template <>
 __device__ __forceinline__ LineStore<double> lineStore<double>(const Inst& I) {
-  return lineStoreOf<double>(I.pP, I.pN, I.pE, I.pWk, I.lInv, I.lUp, I.r, I.z, I.pd0, I.pd1, I.Ap);
+  return lineStoreOf<double>(I.pP, I.pN, I.pE, I.pWk, I.lInv, I.lUp, I.r, I.z, I.pd0, I.pd1, I.Ap, I.x);
 }
+// FP32 storage: the correction x too (its rounding error scales with the
+// correction, which vanishes at convergence); lcgFinal hands it over in FP64
 template <>
 __device__ __forceinline__ LineStore<float> lineStore<float>(const Inst& I) {
-  return lineStoreOf<float>(I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.flUp, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp);
+  return lineStoreOf<float>(I.fpP, I.fpN, I.fpE, I.fpWk, I.flInv, I.flUp, I.fr, I.fz, I.fpd0, I.fpd1, I.fAp, I.fx);
 }
 
 __device__ __forceinline__ double waveSum(double v) {
@@ -150,7 +152,6 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
   const double mean = demean ? bc[0] / static_cast<double>(I.N) : 0.0;
   const CFD_GLOBAL double* src = gmem(static_cast<const double*>((pass == 0) ? I.pb : I.rb));
   CFD_GLOBAL double* pb = gmem(I.pb);
-  CFD_GLOBAL double* xg = gmem(I.x);
   const int ic = col ? i : nI - 1;  // idle lanes read a real column and store nothing
   double bA[U], bB[U] = {};
   T dA[U], nA[U], dB[U] = {}, nB[U] = {};
@@ -193,7 +194,7 @@ __device__ __forceinline__ void lcgInit(const Inst* insts, int pass) {
           S.up[c] = cPrev;
           S.r[c] = r;
           S.z[c] = y;
-          xg[c] = 0.0;
+          S.x[c] = T(0);
         }
         l1 += fabs(static_cast<double>(r));
         rz += static_cast<double>(y) * static_cast<double>(w);
@@ -277,9 +278,9 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   const double norm0 = red[sCgNorm0];
   const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
   const double alphaPrev = red[sCgAlpha];
-  double xq[CELLS];
+  T xq[CELLS];
 #pragma unroll
-  for (int q = 0; q < CELLS; ++q) xq[q] = gmem(static_cast<const double*>(I.x))[cq[q]];
+  for (int q = 0; q < CELLS; ++q) xq[q] = S.x[cq[q]];
   if (stopped) return;  // the whole block: the flags are uniform
   // x += alpha p of iteration it - 1 (launch B leaves it to this launch, whose
   // cells hold p_{it-1}), whatever this launch's test decides. This comes
@@ -290,7 +291,7 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   if (it > 0 && itersDone == it) {
 #pragma unroll
     for (int q = 0; q < CELLS; ++q)
-      if (live[q]) gmem(I.x)[cq[q]] = xq[q] + alphaPrev * static_cast<double>(oc[q]);
+      if (live[q]) S.x[cq[q]] = xq[q] + static_cast<T>(alphaPrev) * oc[q];
   }
   if (cgDone) return;
   // row 0: the wake partner's values (a second round trip for those lanes only)
@@ -576,16 +577,24 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
 }
 
 // After the CG iterations of a pass: the x update of the last launch B, when
-// no launch A followed it (the cap was reached).
+// no launch A followed it (the cap was reached); with FP32 storage also the
+// hand-over of x to the FP64 correction the velocity and flux updates read.
 template <typename T>
 __device__ __forceinline__ void lcgFinal(const Inst* insts) {
   FAST_CELL_PROLOGUE
   (void)i; (void)j; (void)nI; (void)nJ;
+  if (!live) return;
   const int n = I.flags[fCgIters];
-  if (!live || n != I.cgMax || n == 0) return;
+  const bool capped = n == I.cgMax && n != 0;
   const LineStore<T> S = lineStore<T>(I);
   const CFD_GLOBAL T* p = ((n - 1) & 1) ? S.pd1 : S.pd0;
-  gmem(I.x)[c] += I.red[sCgAlpha] * static_cast<double>(p[c]);
+  if constexpr (sizeof(T) == sizeof(double)) {
+    if (capped) gmem(I.x)[c] += I.red[sCgAlpha] * static_cast<double>(p[c]);
+  } else {
+    // the FP32 accumulation handed over in FP64, with the last step in FP64
+    const double last = capped ? I.red[sCgAlpha] * static_cast<double>(p[c]) : 0.0;
+    gmem(I.x)[c] = static_cast<double>(S.x[c]) + last;
+  }
 }
 
 // The launches in the storage type T, with the shapes the launch tuning picks

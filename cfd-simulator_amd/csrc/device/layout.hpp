@@ -243,9 +243,10 @@ struct Inst {
   double gamma1, gamma2, betaStarQ;  // gamma1(), gamma2(), betaStar^(1/4) (host-computed)
 
   // mixed precision: the pressure system and the CG vectors stored in FP32
-  // (the CG's arithmetic, its reductions and x stay FP64)
+  // (the Jacobi CG's arithmetic, its reductions and x stay FP64; the line CG
+  // accumulates x in FP32, fx, and hands it over in FP64 at the end)
   float *fpP, *fpN, *fpE, *fpWk;
-  float *fr, *fz, *fpd0, *fpd1, *fAp;
+  float *fr, *fz, *fpd0, *fpd1, *fAp, *fx;
 
   // Line-preconditioned CG: 1 / pivot of each column's tridiagonal factor
   // (Thomas) and the eliminated upper coefficient (pN / pivot), in the CG's

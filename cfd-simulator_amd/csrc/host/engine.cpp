@@ -261,7 +261,8 @@ void carveInst(Carver& cv, Inst& I, const Grid& grid) {
     for (double2** a : vecArrays) *a = cv.take<double2>(N);
     I.Fj = cv.take<double>(nJf);
     I.Fi = cv.take<double>(nIf);
-    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.flInv, &I.flUp};
+    float** fArrays[] = {&I.fpP, &I.fpN, &I.fpE, &I.fpWk, &I.fr, &I.fz, &I.fpd0, &I.fpd1, &I.fAp, &I.flInv, &I.flUp,
+                         &I.fx};
     for (float** a : fArrays) *a = cv.take<float>(N);
     double** lineArrays[] = {&I.lInv, &I.lUp, &I.rb, &I.xacc};
     for (double** a : lineArrays) *a = cv.take<double>(N);

@@ -9,10 +9,11 @@ by tests/golden/make_fixtures.py in the development container):
   iteration counts of converged, limited and diverged solves);
 * 1024 x 256 NACA 2412, SST, Re 1e6, alpha 4, to 1e-7.
 
-Comparisons are made at matched iteration counts: the device runs to within
-three iterations of the oracle's stop, then one iteration at a time past it,
-recording where its own stop rule fires (within two iterations of the
-oracle's) and its surface and forces at the oracle's iteration.
+Comparisons are made at matched iteration counts: the device runs to just
+before the oracle's stop, then one iteration at a time past it, recording
+where its own stop rule fires (within two iterations of the oracle's, or of
+the farthest stop of the oracle's round-off probes, below) and its surface
+and forces at the oracle's iteration.
 
 The two differ by rounding alone: the order of global sums (block and tree
 sums on the device, one sequential pass in the oracle) and the last bits of
@@ -64,25 +65,41 @@ def build(G, fx):
     return mesh, s
 
 
+def stop_window(fx):
+    """How far the device's stop may fall from the oracle's: two iterations,
+    or two past the farthest stop of the oracle's own round-off probes (a
+    residual that hovers near the tolerance moves the first iteration below
+    it with rounding alone)."""
+    out = 2
+    for probe in fx.get("sensitivity", {}).values():
+        if probe.get("stop") is None:
+            return None  # a probe did not stop within 3,000 iterations: no window
+        out = max(out, abs(probe["stop"] - fx["iterations"]) + 2)
+    return out
+
+
 def run_matched(G, s, fx):
     """Returns (device stop iteration or None, forces, rows) with forces and
     rows taken at the oracle's stop iteration."""
     n_o, tol = fx["iterations"], fx["tolerance"]
-    pre = max(0, n_o - 3)
+    window = stop_window(fx) or 2
+    pre = max(0, n_o - window - 1)
     first = None
+    it = 0
     if pre:
         rep = s.solve(tol, pre, check_every=100)
+        it = rep.iterations
         if rep.status == 0:
-            return rep.iterations, None, None
-        assert rep.status == 1, f"device status {rep.status} before the oracle's stop"
-    it = pre
+            first = rep.iterations
+        else:
+            assert rep.status == 1, f"device status {rep.status} before the oracle's stop"
     while it < n_o:
         mon = s.iterate(1)
         it += 1
         if first is None and mon.worst() < tol:
             first = it
     f, rows = s.surface(speed=1.0, alpha_deg=fx["alpha"], density=1.0, reference_pressure=0.0)
-    while first is None and it < n_o + 3:
+    while first is None and it < n_o + window:
         mon = s.iterate(1)
         it += 1
         if mon.worst() < tol:
@@ -110,6 +127,7 @@ def probe_change(fx):
 
 def compare(fx, first, f, rows, coef_tol, abs_floor, surf_tol):
     name = fx["name"]
+    window = stop_window(fx)
     assert first is not None and f is not None, f"{name}: no stop near the oracle's {fx['iterations']}"
     ref = fx["forces"]
     sens = probe_change(fx)
@@ -125,9 +143,9 @@ def compare(fx, first, f, rows, coef_tol, abs_floor, surf_tol):
           f"|dCm| {errs['cm']:.3e} (Cl {ref['cl']:.8f} Cd {ref['cd']:.8f}); Cp {errs['cp']:.3e} Cf {errs['cf']:.3e}; "
           f"separation faces {f.upper_separation_face},{f.lower_separation_face} "
           f"(oracle {fx['upper_separation_face']},{fx['lower_separation_face']})", flush=True)
-    print("  bar: " + ", ".join(f"{q} {bars[q]:.2e}" for q in bars) + "; probe change: "
+    print("  bar: " + ", ".join(f"{q} {bars[q]:.2e}" for q in bars) + f"; stop window {window}; probe change: "
           + ", ".join(f"{q} {sens[q]:.1e}" for q in sens), flush=True)
-    assert abs(first - fx["iterations"]) <= 2
+    assert window is not None and abs(first - fx["iterations"]) <= window, (first, window)
     assert np.array_equal(rows[:, 1].astype(int), np.array(fx["station_face"]))
     for q, e in errs.items():
         assert e <= bars[q], (q, e, bars[q])
