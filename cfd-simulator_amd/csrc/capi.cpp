@@ -484,8 +484,8 @@ int cfdgpu_divergence_noalloc(const cfdgpu_solver* s, double* out) {
 }
 
 int cfdgpu_set_launch_tuning(int cells, int rows) {
-  if (cells != 1 && cells != 2 && cells != 4)
-    return fail(CFDGPU_INVALID_ARGUMENT, "cells per thread must be 1, 2 or 4");
+  if (cells != 0 && cells != 1 && cells != 2 && cells != 4)
+    return fail(CFDGPU_INVALID_ARGUMENT, "cells per thread must be 0 (LDS tiles), 1, 2 or 4");
   if (rows != 16 && rows != 32) return fail(CFDGPU_INVALID_ARGUMENT, "column rows must be 16 or 32");
   cfdgpu::LaunchTuning t;
   t.lcgCells = cells;

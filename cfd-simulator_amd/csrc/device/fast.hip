@@ -359,6 +359,157 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
 }
 
+// Launch A on a tile of kTileW columns x kTileH rows per block (4 cells per
+// thread; wave w owns rows w, w + 4, w + 8, w + 12 of the tile, a row's 64
+// cells contiguous across its lanes): z and p_old of the tile and its
+// one-cell halo are loaded once into LDS, p = z + beta p_old is formed there,
+// and A p reads its neighbours' p from LDS, so a cell costs its own z, p_old,
+// coefficients and x from memory instead of five z and five p_old loads.
+// Same per-cell arithmetic as lcgA; the p.Ap partial sums are per tile.
+template <typename T>
+__device__ __forceinline__ void lcgATile(const Inst* insts, int it, double* sm, T (&sp)[kTileH + 2][kTileW + 2]) {
+  const Inst& I = insts[blockIdx.y];
+  const int nI = I.nI, nJ = I.nJ;
+  const int tilesI = (nI + kTileW - 1) / kTileW;
+  const int nb = tilesA(nI, nJ);
+  if (static_cast<int>(blockIdx.x) >= nb) return;
+  if (it >= I.cgMax) return;  // a batch member with a smaller cap than the batch's
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), nb);
+  const int i0 = (tile % tilesI) * kTileW, j0 = (tile / tilesI) * kTileH;
+  const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
+  const int i = i0 + lane;
+  const LineStore<T> S = lineStore<T>(I);
+  const CFD_GLOBAL int* flagsG = gmem(static_cast<const int*>(I.flags));
+  const CFD_GLOBAL double* red = gmem(static_cast<const double*>(I.red));
+  const CFD_GLOBAL T* z = S.z;
+  const CFD_GLOBAL T* pOld = (it & 1) ? S.pd0 : S.pd1;
+  constexpr int Q = kTileH / 4;
+  int cq[Q], jq[Q];
+  bool live[Q];
+  T zc[Q], oc[Q], dP[Q], nB[Q], nA[Q], eL[Q], eR[Q], wk[Q], xq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = j0 + wv + 4 * q;
+    live[q] = i < nI && j < nJ;
+    const int c = live[q] ? j * nI + i : 0;
+    const int jj = live[q] ? j : 0, ii = live[q] ? i : 0;
+    cq[q] = c;
+    jq[q] = jj;
+    const int cS = (jj > 0) ? c - nI : c;
+    const int cW = (ii > 0) ? c - 1 : c;
+    zc[q] = live[q] ? z[c] : T(0);
+    oc[q] = (live[q] && it > 0) ? pOld[c] : T(0);
+    dP[q] = S.pP[c]; nB[q] = S.pN[cS]; nA[q] = S.pN[c]; eL[q] = S.pE[cW]; eR[q] = S.pE[c];
+    wk[q] = (jj == 0) ? S.pWk[c] : T(0);  // the wake coupling exists on row 0 only
+    xq[q] = S.x[c];
+  }
+  // the halo: the rows below and above the tile (waves 0 and 1, a lane per
+  // column) and the columns left and right of it (wave 2, a lane per row);
+  // cells outside the mesh hold zero
+  T hz = T(0), ho = T(0);
+  int hr = -1, hc = -1;
+  if (wv < 2) {
+    const int j = (wv == 0) ? j0 - 1 : j0 + kTileH;
+    hr = (wv == 0) ? 0 : kTileH + 1;
+    hc = lane + 1;
+    if (i < nI && j >= 0 && j < nJ) {
+      hz = z[j * nI + i];
+      if (it > 0) ho = pOld[j * nI + i];
+    }
+  } else if (wv == 2 && lane < 2 * kTileH) {
+    const int r = lane % kTileH, side = lane / kTileH;
+    const int ii = side ? i0 + kTileW : i0 - 1, j = j0 + r;
+    hr = r + 1;
+    hc = side ? kTileW + 1 : 0;
+    if (ii >= 0 && ii < nI && j < nJ) {
+      hz = z[j * nI + ii];
+      if (it > 0) ho = pOld[j * nI + ii];
+    }
+  }
+  // row 0's wake partners (tiles on row 0 only)
+  const int pr = (j0 == 0 && wv == 0 && i < nI) ? I.partner[i] : -1;
+  const int stopped = flagsG[fStopped];
+  const int cgDone = flagsG[fCgDone];
+  const int itersDone = flagsG[fCgIters];
+  const double norm0 = red[sCgNorm0];
+  const double rzOld = red[((it - 1) & 1) ? sCgRz1 : sCgRz0];
+  const double alphaPrev = red[sCgAlpha];
+  if (stopped) return;  // the whole block: the flags are uniform
+  // x += alpha p of iteration it - 1 (see lcgA)
+  if (it > 0 && itersDone == it) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if (live[q]) S.x[cq[q]] = xq[q] + static_cast<T>(alphaPrev) * oc[q];
+  }
+  if (cgDone) return;
+  T zPr = T(0), oPr = T(0);
+  if (pr >= 0) {
+    zPr = z[pr];
+    if (it > 0) oPr = pOld[pr];
+  }
+  double sums[2];
+  {
+    const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
+    blockSumPartials<2>(slots, I.colBlocks, sm, sums);
+  }
+  const double nrm = sums[0], rzNew = sums[1];
+  double beta = 0.0;
+  bool done = false;
+  if (it == 0) {
+    if (!(nrm > 0.0)) done = true;  // already solved (LinearSystem.cpp:144-149)
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+      gmem(I.red)[sCgNorm0] = nrm;
+      gmem(I.red)[sCgRz0] = rzNew;
+    }
+  } else {
+    if (nrm <= I.cgTol * norm0) done = true;
+    else if (!(fabs(rzOld) > 0.0)) done = true;
+    else beta = rzNew / rzOld;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !done) gmem(I.red)[(it & 1) ? sCgRz1 : sCgRz0] = rzNew;
+  }
+  if (done) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) gmem(I.flags)[fCgDone] = 1;
+    return;
+  }
+  const T b = static_cast<T>(beta);
+  auto pv = [&](T zv, T ov) -> T { return (it == 0) ? zv : static_cast<T>(zv + b * ov); };
+  T pc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    pc[q] = pv(zc[q], oc[q]);
+    sp[1 + wv + 4 * q][1 + lane] = pc[q];
+  }
+  if (hr >= 0) sp[hr][hc] = pv(hz, ho);
+  const T pPr = pv(zPr, oPr);
+  __syncthreads();
+  double part[1] = {0.0};
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int r = 1 + wv + 4 * q;
+    const int jj = jq[q];
+    const bool wake = live[q] && jj == 0 && pr >= 0;
+    // masked couplings: zero where the neighbour is missing
+    const T aS = (jj > 0) ? nB[q] : (wake ? wk[q] : T(0));
+    const T aN = (jj < nJ - 1) ? nA[q] : T(0);
+    const T aW = (i > 0) ? eL[q] : T(0);
+    const T aE = (i < nI - 1) ? eR[q] : T(0);
+    const T pS = (jj > 0) ? sp[r - 1][1 + lane] : pPr;
+    // diagonal, then below (or the wake partner), above, left, right
+    T y = dP[q] * pc[q];
+    y += aS * pS;
+    y += aN * sp[r + 1][1 + lane];
+    y += aW * sp[r][lane];
+    y += aE * sp[r][lane + 2];
+    if (live[q]) {
+      ((it & 1) ? S.pd1 : S.pd0)[cq[q]] = pc[q];
+      S.Ap[cq[q]] = y;
+      part[0] += static_cast<double>(pc[q]) * static_cast<double>(y);
+    }
+  }
+  blockSum<1>(part, sm);
+  if (threadIdx.x == 0) partialSlot(I, sCgPAp)[blockIdx.x] = part[0];
+}
+
 // Iteration `it`, launch B (column-parallel): alpha = r.z / p.Ap (kept for
 // the next launch A, which adds alpha p to x), r -= alpha Ap, |r|_1,
 // z = M^-1 r (forward and back substitution along the column), r.z.
@@ -369,7 +520,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it, int cellsA) {
   if (it >= I.cgMax) return;
   if (I.flags[fCgDone]) return;
   const LineStore<T> S = lineStore<T>(I);
-  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), lineBlocksA(I.N, cellsA));
+  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
   if (threadIdx.x == 0) bc[0] = papLane;
   __syncthreads();
   const double pap = bc[0];
@@ -487,7 +638,7 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
     lo[u] = S.pN[(j > 0 ? j - 1 : 0) * nI + ic];
   }
   if (wv == 0) {
-    const double papLane = waveSumPartials(partialSlot(I, sCgPAp), lineBlocksA(I.N, cellsA));
+    const double papLane = waveSumPartials(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
     if (lane == 0) bc[0] = papLane;
   }
   __syncthreads();
@@ -615,6 +766,11 @@ __device__ __forceinline__ void lcgFinal(const Inst* insts) {
   __global__ void __launch_bounds__(kBlock) k_lcg_a##SUFFIX##_c4(const Inst* insts, int it) {           \
     __shared__ double sm[16];                                                                          \
     lcgA<T, 4>(insts, it, sm);                                                                         \
+  }                                                                                                    \
+  __global__ void __launch_bounds__(kBlock) k_lcg_a##SUFFIX##_tile(const Inst* insts, int it) {         \
+    __shared__ double sm[16];                                                                          \
+    __shared__ T sp[kTileH + 2][kTileW + 2];                                                           \
+    lcgATile<T>(insts, it, sm, sp);                                                                    \
   }                                                                                                    \
   __global__ void __launch_bounds__(kColThreads) k_lcg_b##SUFFIX##_u16(const Inst* insts, int it, int cellsA) { \
     lcgB<T, 16>(insts, it, cellsA);                                                                    \

@@ -37,6 +37,7 @@ __global__ void k_lcg_init(const Inst*, int);
 __global__ void k_lcg_a_c1(const Inst*, int);
 __global__ void k_lcg_a_c2(const Inst*, int);
 __global__ void k_lcg_a_c4(const Inst*, int);
+__global__ void k_lcg_a_tile(const Inst*, int);
 __global__ void k_lcg_b_u16(const Inst*, int, int);
 __global__ void k_lcg_b_u32(const Inst*, int, int);
 __global__ void k_lcg_b_seg(const Inst*, int, int);
@@ -44,6 +45,7 @@ __global__ void k_lcg_init_f32(const Inst*, int);
 __global__ void k_lcg_a_f32_c1(const Inst*, int);
 __global__ void k_lcg_a_f32_c2(const Inst*, int);
 __global__ void k_lcg_a_f32_c4(const Inst*, int);
+__global__ void k_lcg_a_f32_tile(const Inst*, int);
 __global__ void k_lcg_b_f32_u16(const Inst*, int, int);
 __global__ void k_lcg_b_f32_u32(const Inst*, int, int);
 __global__ void k_lcg_b_f32_seg(const Inst*, int, int);
@@ -134,8 +136,14 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
         using KA = void (*)(const Inst*, int);
         using KB = void (*)(const Inst*, int, int);
         const int cells = cfg.lcgCells;
-        const KA ka = mixed ? (cells == 1 ? k_lcg_a_f32_c1 : cells == 4 ? k_lcg_a_f32_c4 : k_lcg_a_f32_c2)
-                            : (cells == 1 ? k_lcg_a_c1 : cells == 4 ? k_lcg_a_c4 : k_lcg_a_c2);
+        const KA ka = mixed ? (cells == 0   ? k_lcg_a_f32_tile
+                               : cells == 1 ? k_lcg_a_f32_c1
+                               : cells == 4 ? k_lcg_a_f32_c4
+                                            : k_lcg_a_f32_c2)
+                            : (cells == 0   ? k_lcg_a_tile
+                               : cells == 1 ? k_lcg_a_c1
+                               : cells == 4 ? k_lcg_a_c4
+                                            : k_lcg_a_c2);
         // columns of at most 256 rows: the segmented launch B (1024 threads per
         // 64 columns); taller ones: one lane per column, walking the tuned depth ahead
         const bool seg = cfg.maxNJ <= kLcgSegMaxRows;

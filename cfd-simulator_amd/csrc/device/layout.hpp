@@ -168,6 +168,17 @@ constexpr int kLcgSegMaxRows = 256;  // ... each wave holding at most 16 rows
 constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)
 // blocks of the line CG's launch A for n cells at `cells` cells per thread
 __host__ __device__ constexpr int lineBlocksA(int n, int cells) { return (n + kBlock * cells - 1) / (kBlock * cells); }
+// the LDS-tiled launch A (cells per thread 0 in the launch tuning): tiles of
+// kTileW columns x kTileH rows, four cells per thread
+constexpr int kTileW = 64, kTileH = 16;
+static_assert(kTileW * kTileH == 4 * kBlock, "launch A tile");
+__host__ __device__ constexpr int tilesA(int nI, int nJ) {
+  return ((nI + kTileW - 1) / kTileW) * ((nJ + kTileH - 1) / kTileH);
+}
+// launch A's blocks (and p.Ap partial sums) for an nI x nJ mesh
+__host__ __device__ constexpr int blocksA(int nI, int nJ, int cells) {
+  return cells == 0 ? tilesA(nI, nJ) : lineBlocksA(nI * nJ, cells);
+}
 
 struct Inst {
   int nI, nJ, N;

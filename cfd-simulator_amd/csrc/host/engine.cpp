@@ -324,6 +324,7 @@ void Engine::allocate() {
   cfg_.maxColBlocks = h_.colBlocks;
   cfg_.maxRbBlocks = rbBlocks(h_.nI, h_.nJ);
   cfg_.maxNJ = h_.nJ;
+  cfg_.maxTilesA = tilesA(h_.nI, h_.nJ);
   cfg_.batch = 1;
   cfg_.gsThreads = gsThreadsFor(grid_.nJ);
   cfg_.gsLds = gsLdsFor(grid_.nJ, grid_.nI, cfg_.gsThreads);
@@ -676,7 +677,7 @@ bool graphsEnabled() {
 bool IterationGraph::Key::operator==(const Key& o) const {
   return d == o.d && maxBlocks == o.maxBlocks && maxCgBlocks == o.maxCgBlocks && maxColBlocks == o.maxColBlocks &&
          lcgCells == o.lcgCells && colUnroll == o.colUnroll && maxLcgBlocks == o.maxLcgBlocks &&
-         maxRbBlocks == o.maxRbBlocks && maxNJ == o.maxNJ && batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
+         maxRbBlocks == o.maxRbBlocks && maxNJ == o.maxNJ && maxTilesA == o.maxTilesA && batch == o.batch && gsThreads == o.gsThreads && gsLds == o.gsLds && gsOrder == o.gsOrder &&
          precision == o.precision && pSolver == o.pSolver && refinePasses == o.refinePasses &&
          momSweeps == o.momSweeps && nonOrthPasses == o.nonOrthPasses && cgIters == o.cgIters && hasPRef == o.hasPRef &&
          model == o.model;
@@ -684,7 +685,7 @@ bool IterationGraph::Key::operator==(const Key& o) const {
 
 IterationGraph::Key IterationGraph::keyOf(const Inst* d, const LaunchCfg& cfg, const Inst& h) {
   return Key{d,           cfg.maxBlocks,   cfg.maxCgBlocks, cfg.maxColBlocks, cfg.lcgCells, cfg.colUnroll,
-             cfg.maxLcgBlocks, cfg.maxRbBlocks, cfg.maxNJ, cfg.batch, cfg.gsThreads,
+             cfg.maxLcgBlocks, cfg.maxRbBlocks, cfg.maxNJ, cfg.maxTilesA, cfg.batch, cfg.gsThreads,
              cfg.gsLds,   h.gsOrder,       h.precision,     h.pSolver,        h.refinePasses, h.momSweeps,
              h.nonOrthPasses, cfg.cgIters, h.hasPRef, h.model};
 }
@@ -708,11 +709,12 @@ void applyTuning(LaunchCfg& cfg, int maxCells) {
   const LaunchTuning t = launchTuning();
   cfg.lcgCells = t.lcgCells;
   cfg.colUnroll = t.colUnroll;
-  cfg.maxLcgBlocks = lineBlocksA(maxCells, t.lcgCells);
+  cfg.maxLcgBlocks = (t.lcgCells == 0) ? cfg.maxTilesA : lineBlocksA(maxCells, t.lcgCells);
 }
 
 LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
-  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, rbBlk = 1, maxNI = 0, maxNJ = 1, cgIters = 0, maxN = 1;
+  int lanes = 64, blocks = 1, cgBlocks = 1, colBlocks = 1, rbBlk = 1, maxNI = 0, maxNJ = 1, tiles = 1, cgIters = 0,
+      maxN = 1;
   for (size_t q = 0; q < count; ++q) {
     const Inst& b = insts[q];
     cgIters = std::max(cgIters, b.cgMax);
@@ -724,8 +726,9 @@ LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
     colBlocks = std::max(colBlocks, b.colBlocks);
     maxNI = std::max(maxNI, b.nI);
     maxNJ = std::max(maxNJ, b.nJ);
+    tiles = std::max(tiles, tilesA(b.nI, b.nJ));
   }
-  LaunchCfg cfg{blocks, cgBlocks, colBlocks, cgIters, 2, 32, 1, rbBlk, maxNJ, static_cast<int>(count), lanes,
+  LaunchCfg cfg{blocks, cgBlocks, colBlocks, cgIters, 2, 32, 1, rbBlk, maxNJ, tiles, static_cast<int>(count), lanes,
                 gsLdsFor(0, maxNI, lanes)};
   applyTuning(cfg, maxN);
   return cfg;

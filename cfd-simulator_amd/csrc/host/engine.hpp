@@ -27,11 +27,12 @@ struct LaunchCfg {
   int maxColBlocks; // grid.x for the column-parallel kernels (line-preconditioned CG)
   int cgIters;      // CG iterations launched per solve: the largest cap of the batch (a member
                     // with a smaller cap stops its CG at its own)
-  int lcgCells;     // line CG launch A: cells per thread (1, 2, 4)
+  int lcgCells;     // line CG launch A: cells per thread (1, 2, 4), or 0: the LDS-tiled launch
   int colUnroll;    // line CG launch B: rows of a column loaded ahead (16, 32)
   int maxLcgBlocks; // grid.x of the line CG's launch A
   int maxRbBlocks;  // grid.x of the two-colour FP32 sweep (one lane per cell of a colour)
   int maxNJ;        // the tallest member's rows (the line CG's segmented launch B holds up to 256)
+  int maxTilesA;    // the most 64 x 16 tiles of a member (the LDS-tiled launch A)
   int batch;       // grid.y
   int gsThreads;   // block size of the lexicographic sweep
   size_t gsLds;    // dynamic LDS of the lexicographic sweep
@@ -103,8 +104,8 @@ class IterationGraph {
  private:
   struct Key {
     const Inst* d;
-    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, maxRbBlocks, maxNJ, batch,
-        gsThreads;
+    int maxBlocks, maxCgBlocks, maxColBlocks, lcgCells, colUnroll, maxLcgBlocks, maxRbBlocks, maxNJ, maxTilesA,
+        batch, gsThreads;
     size_t gsLds;
     int gsOrder, precision, pSolver, refinePasses, momSweeps, nonOrthPasses, cgIters, hasPRef, model;
     bool operator==(const Key& o) const;

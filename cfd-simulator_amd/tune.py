@@ -1,9 +1,11 @@
 """Launch tuner of the line-preconditioned CG.
 
 The CG iteration of the fast path is two launches: a cell-parallel one
-(p = z + beta p, A p and its p.Ap partial sums; 1, 2 or 4 cells per thread)
-and a column-parallel one (alpha, the residual update and the tridiagonal
-solve of each wall-normal column; 16 or 32 rows loaded ahead). Every shape
+(p = z + beta p, A p and its p.Ap partial sums; 1, 2 or 4 cells per thread,
+or tiles of 64 x 16 cells staged in LDS) and a column-parallel one (alpha,
+the residual update and the tridiagonal solve of each wall-normal column; 16
+or 32 rows loaded ahead, or, for meshes of at most 256 rows, each column's
+rows split over the waves of a block, which has no depth to tune). Every shape
 computes the same iterates (the block partial sums split differently, so
 results agree to rounding), and which is fastest depends on the batch's mesh
 sizes and the device. The tuner builds the 64-instance mixed-size batch of
@@ -29,7 +31,13 @@ import cfdgpu as G  # noqa: E402
 import workloads as W  # noqa: E402
 
 TUNING_FILE = os.path.join(HERE, "workloads", "launch_tuning.json")
-SHAPES = [(c, r) for c in (1, 2, 4) for r in (16, 32)]
+CELLS = (0, 1, 2, 4)  # launch A: 0 = LDS tiles of 64 x 16 cells, else cells per thread
+SEG_MAX_ROWS = 256    # meshes of at most this many rows take the segmented launch B (no depth to tune)
+
+
+def shapes_for(max_rows: int):
+    rows = (32,) if max_rows <= SEG_MAX_ROWS else (16, 32)
+    return [(c, r) for c in CELLS for r in rows]
 
 
 def fast_settings(cg_cap: int = 15):
@@ -56,12 +64,13 @@ def tune(steps: int = 6, repeats: int = 3, warmup: int = 20, cg_cap: int = 15, l
     st = fast_settings(cg_cap)
     solvers, batch = build_batch(rows, st)
     cells = sum(m.cells for m, _ in solvers)
+    shapes = shapes_for(max(m.nj for m, _ in solvers))
     batch.iterate(warmup)  # past the start-up transient, every member's CG at its cap
     before = G.launch_tuning()
     results = []
     try:
         for rep in range(repeats):
-            for cells_per_thread, column_rows in SHAPES:
+            for cells_per_thread, column_rows in shapes:
                 G.set_launch_tuning(cells_per_thread, column_rows)
                 batch.iterate(1)  # the first launches of a shape are not timed
                 mons, stage_ms, calls = batch.iterate_timed(steps)

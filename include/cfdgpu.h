@@ -285,8 +285,9 @@ int cfdgpu_divergence(const cfdgpu_solver* solver, double* out);
 int cfdgpu_divergence_noalloc(const cfdgpu_solver* solver, double* out);
 
 /* Launch shapes of the line-preconditioned CG (CFDGPU_PSOLVE_LINE_CG), process-wide: cells per
-   thread of the cell-parallel launch (1, 2 or 4) and rows of a column loaded ahead by the
-   column sweep (16 or 32). Every shape computes the same iterates; the block partial sums of
+   thread of the cell-parallel launch (1, 2 or 4, or 0 for tiles of 64 columns x 16 rows staged
+   in LDS) and rows of a column loaded ahead by the column sweep (16 or 32; batches whose
+   meshes have at most 256 rows use the segmented column launch instead). Every shape computes the same iterates; the block partial sums of
    the cell-parallel launch split differently, so results agree to rounding (each shape is
    deterministic). The next iterate, solve, batch call or server chunk uses the new shapes. No
    reference counterpart: the hook of the launch tuner (cfd-simulator_amd/tune.py).

@@ -48,12 +48,12 @@ def test_host_side_validation_needs_no_gpu(G):
 
 def test_launch_tuning_is_validated_and_kept(G):
     before = G.launch_tuning()
-    assert before in {(c, r) for c in (1, 2, 4) for r in (16, 32)}
+    assert before in {(c, r) for c in (0, 1, 2, 4) for r in (16, 32)}
     try:
-        for c, r in ((1, 16), (4, 32), (2, 32)):
+        for c, r in ((1, 16), (0, 16), (4, 32), (2, 32)):
             G.set_launch_tuning(c, r)
             assert G.launch_tuning() == (c, r)
-        for c, r in ((3, 32), (0, 16), (2, 8), (2, 64)):
+        for c, r in ((3, 32), (-1, 16), (8, 32), (2, 8), (2, 64)):
             with pytest.raises(G.CfdError):
                 G.set_launch_tuning(c, r)
             assert G.launch_tuning() == (2, 32)  # a rejected call changes nothing

@@ -66,11 +66,12 @@ def restore_tuning(G):
     G.set_launch_tuning(*before)
 
 
-@pytest.mark.parametrize("shape", [(1, 16), (1, 32), (2, 16), (4, 16), (4, 32)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("shape", [(1, 16), (1, 32), (2, 16), (4, 16), (4, 32), (0, 32)], ids=lambda s: f"{s[0]}x{s[1]}")
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 def test_every_launch_shape_matches_the_definition(G, gpu_available, restore_tuning, precision, shape):
-    """The tuner's launch shapes (cells per thread of launch A, rows loaded
-    ahead by launch B) against the NumPy line CG, with the exit path (cap 500)."""
+    """The tuner's launch shapes (cells per thread of launch A, 0 for its LDS
+    tiles; rows loaded ahead by launch B, which a 27-row mesh runs as the
+    segmented launch) against the NumPy line CG, with the exit path (cap 500)."""
     print(f"case: 118x27 SST, line CG ({'fp64' if precision == 0 else 'fp32'}), launch shape {shape}, cap 500",
           flush=True)
     G.set_launch_tuning(*shape)
