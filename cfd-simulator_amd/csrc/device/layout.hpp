@@ -28,6 +28,16 @@ struct FaceGeo {
   double w;       // owner interpolation weight
 };
 
+// In device memory each FaceGeo field is an array of its own, so a kernel
+// reads only the fields it uses (the SST assembly only w, the gradients S and
+// w) and every field load is contiguous across a wave.
+struct FaceGeoArrays {
+  const double *Sx, *Sy, *Tx, *Ty, *g, *w;
+};
+__host__ __device__ __forceinline__ FaceGeo geo(const FaceGeoArrays& a, int f) {
+  return FaceGeo{a.Sx[f], a.Sy[f], a.Tx[f], a.Ty[f], a.g[f], a.w[f]};
+}
+
 // Per boundary face: the FaceCondition (BoundaryConditions.hpp:101-109) plus
 // the frozen far-field dead band (SimpleSolver.cpp:150-166) and the k-omega
 // SST geometric factor |S|^2/|S.d| (KOmegaSST.cpp:423-427).
@@ -202,8 +212,8 @@ struct Inst {
 
   // geometry (read-only)
   const double* V;
-  const FaceGeo* jg;
-  const FaceGeo* ig;
+  FaceGeoArrays jg;  // j-faces
+  FaceGeoArrays ig;  // i-faces
   const int* partner;  // nI entries: partner cell in row 0, or -1
   const FaceBC* bcB;   // nI
   const FaceBC* bcT;   // nI

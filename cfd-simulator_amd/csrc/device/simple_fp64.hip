@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kBlock) k_grad(const Inst* insts) {
     };
     // below: j-face (i, j)
     {
-      const FaceGeo& f = I.jg[j * nI + i];
+      const FaceGeo f = geo(I.jg, j * nI + i);
       if (j > 0) {
         add(f, faceValues(I, c, c - nI, c, f), false);
       } else {
@@ -82,21 +82,21 @@ __global__ void __launch_bounds__(kBlock) k_grad(const Inst* insts) {
     // above: j-face (i, j+1), owned by this cell
     {
       const int fi = (j + 1) * nI + i;
-      const FaceGeo& f = I.jg[fi];
+      const FaceGeo f = geo(I.jg, fi);
       if (j < nJ - 1) add(f, faceValues(I, c, c, c + nI, f), true);
       else add(f, boundaryValues(I, c, I.bcT[i], I.Fj[fi]), true);
     }
     // left: i-face (i, j)
     {
       const int fi = j * (nI + 1) + i;
-      const FaceGeo& f = I.ig[fi];
+      const FaceGeo f = geo(I.ig, fi);
       if (i > 0) add(f, faceValues(I, c, c - 1, c, f), false);
       else add(f, boundaryValues(I, c, I.bcL[j], I.Fi[fi]), true);
     }
     // right: i-face (i+1, j), owned by this cell
     {
       const int fi = j * (nI + 1) + i + 1;
-      const FaceGeo& f = I.ig[fi];
+      const FaceGeo f = geo(I.ig, fi);
       if (i < nI - 1) add(f, faceValues(I, c, c, c + 1, f), true);
       else add(f, boundaryValues(I, c, I.bcR[j], I.Fi[fi]), true);
     }
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     int mirror = -1;
     // below
     {
-      const FaceGeo& f = I.jg[j * nI + i];
+      const FaceGeo f = geo(I.jg, j * nI + i);
       const double F = I.Fj[j * nI + i];
       if (j > 0) {
         aS = momInterior(I, f, F, c - nI, c, false, row);
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     // above
     {
       const int fi = (j + 1) * nI + i;
-      const FaceGeo& f = I.jg[fi];
+      const FaceGeo f = geo(I.jg, fi);
       const double F = I.Fj[fi];
       if (j < nJ - 1) aN = momInterior(I, f, F, c, c + nI, true, row);
       else momBoundary(I, f, I.bcT[i], F, c, row);
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     // left
     {
       const int fi = j * (nI + 1) + i;
-      const FaceGeo& f = I.ig[fi];
+      const FaceGeo f = geo(I.ig, fi);
       const double F = I.Fi[fi];
       if (i > 0) aW = momInterior(I, f, F, c - 1, c, false, row);
       else momBoundary(I, f, I.bcL[j], F, c, row);
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
     // right
     {
       const int fi = j * (nI + 1) + i + 1;
-      const FaceGeo& f = I.ig[fi];
+      const FaceGeo f = geo(I.ig, fi);
       const double F = I.Fi[fi];
       if (i < nI - 1) aE = momInterior(I, f, F, c, c + 1, true, row);
       else momBoundary(I, f, I.bcR[j], F, c, row);
@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(kBlock) k_rhiechow(const Inst* insts) {
   // below
   {
     const int fi = j * nI + i;
-    const FaceGeo& f = I.jg[fi];
+    const FaceGeo f = geo(I.jg, fi);
     if (j > 0) {
       I.Fj[fi] = rcInterior(I, f, c - nI, c);
     } else {
@@ -771,18 +771,18 @@ __global__ void __launch_bounds__(kBlock) k_rhiechow(const Inst* insts) {
   }
   if (j == nJ - 1) {
     const int fi = (j + 1) * nI + i;
-    I.Fj[fi] = rcBoundary(I, I.jg[fi], I.bcT[i], I.Fj[fi], c);
+    I.Fj[fi] = rcBoundary(I, geo(I.jg, fi), I.bcT[i], I.Fj[fi], c);
   }
   // left
   {
     const int fi = j * (nI + 1) + i;
-    const FaceGeo& f = I.ig[fi];
+    const FaceGeo f = geo(I.ig, fi);
     if (i > 0) I.Fi[fi] = rcInterior(I, f, c - 1, c);
     else I.Fi[fi] = rcBoundary(I, f, I.bcL[j], I.Fi[fi], c);
   }
   if (i == nI - 1) {
     const int fi = j * (nI + 1) + i + 1;
-    I.Fi[fi] = rcBoundary(I, I.ig[fi], I.bcR[j], I.Fi[fi], c);
+    I.Fi[fi] = rcBoundary(I, geo(I.ig, fi), I.bcR[j], I.Fi[fi], c);
   }
 }
 
@@ -815,7 +815,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
     // below
     {
       const int fi = j * nI + i;
-      const FaceGeo& f = I.jg[fi];
+      const FaceGeo f = geo(I.jg, fi);
       const double F = I.Fj[fi];
       if (j > 0) {
         net -= F;
@@ -835,7 +835,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
     // above
     {
       const int fi = (j + 1) * nI + i;
-      const FaceGeo& f = I.jg[fi];
+      const FaceGeo f = geo(I.jg, fi);
       const double F = I.Fj[fi];
       net += F;
       if (j < nJ - 1) {
@@ -849,7 +849,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
     // left
     {
       const int fi = j * (nI + 1) + i;
-      const FaceGeo& f = I.ig[fi];
+      const FaceGeo f = geo(I.ig, fi);
       const double F = I.Fi[fi];
       if (i > 0) {
         net -= F;
@@ -862,7 +862,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure(const Inst* insts, int pass
     // right
     {
       const int fi = j * (nI + 1) + i + 1;
-      const FaceGeo& f = I.ig[fi];
+      const FaceGeo f = geo(I.ig, fi);
       const double F = I.Fi[fi];
       net += F;
       if (i < nI - 1) {
@@ -1262,7 +1262,7 @@ __global__ void __launch_bounds__(kBlock) k_correct_cells(const Inst* insts) {
   };
   {
     const int fi = j * nI + i;
-    const FaceGeo& f = I.jg[fi];
+    const FaceGeo f = geo(I.jg, fi);
     if (j > 0) add(f, f.w * corrAt(I, c - nI) + (1.0 - f.w) * pc, false);
     else {
       const int pr = I.partner[i];
@@ -1272,19 +1272,19 @@ __global__ void __launch_bounds__(kBlock) k_correct_cells(const Inst* insts) {
   }
   {
     const int fi = (j + 1) * nI + i;
-    const FaceGeo& f = I.jg[fi];
+    const FaceGeo f = geo(I.jg, fi);
     if (j < nJ - 1) add(f, f.w * pc + (1.0 - f.w) * corrAt(I, c + nI), true);
     else add(f, imposes(I.bcT[i], I.Fj[fi]) ? pc : 0.0, true);
   }
   {
     const int fi = j * (nI + 1) + i;
-    const FaceGeo& f = I.ig[fi];
+    const FaceGeo f = geo(I.ig, fi);
     if (i > 0) add(f, f.w * corrAt(I, c - 1) + (1.0 - f.w) * pc, false);
     else add(f, imposes(I.bcL[j], I.Fi[fi]) ? pc : 0.0, true);
   }
   {
     const int fi = j * (nI + 1) + i + 1;
-    const FaceGeo& f = I.ig[fi];
+    const FaceGeo f = geo(I.ig, fi);
     if (i < nI - 1) add(f, f.w * pc + (1.0 - f.w) * corrAt(I, c + 1), true);
     else add(f, imposes(I.bcR[j], I.Fi[fi]) ? pc : 0.0, true);
   }
@@ -1325,7 +1325,7 @@ __global__ void __launch_bounds__(kBlock) k_correct_faces(const Inst* insts) {
   if (!live) return;
   {
     const int fi = j * nI + i;
-    const FaceGeo& f = I.jg[fi];
+    const FaceGeo f = geo(I.jg, fi);
     if (j > 0) I.Fj[fi] -= corrFlux(I, f, c - nI, c, mean);
     else {
       const int pr = I.partner[i];
@@ -1335,17 +1335,17 @@ __global__ void __launch_bounds__(kBlock) k_correct_faces(const Inst* insts) {
   }
   if (j == nJ - 1) {
     const int fi = (j + 1) * nI + i;
-    corrBoundaryFlux(I, I.jg[fi], I.bcT[i], &I.Fj[fi], c, mean);
+    corrBoundaryFlux(I, geo(I.jg, fi), I.bcT[i], &I.Fj[fi], c, mean);
   }
   {
     const int fi = j * (nI + 1) + i;
-    const FaceGeo& f = I.ig[fi];
+    const FaceGeo f = geo(I.ig, fi);
     if (i > 0) I.Fi[fi] -= corrFlux(I, f, c - 1, c, mean);
     else corrBoundaryFlux(I, f, I.bcL[j], &I.Fi[fi], c, mean);
   }
   if (i == nI - 1) {
     const int fi = j * (nI + 1) + i + 1;
-    corrBoundaryFlux(I, I.ig[fi], I.bcR[j], &I.Fi[fi], c, mean);
+    corrBoundaryFlux(I, geo(I.ig, fi), I.bcR[j], &I.Fi[fi], c, mean);
   }
 }
 

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_blend(const Inst* insts) {
   };
   {
     const int fi = j * nI + i;
-    const FaceGeo& f = I.jg[fi];
+    const FaceGeo f = geo(I.jg, fi);
     if (j > 0) interior(f, c - nI, c, false);
     else {
       const int pr = I.partner[i];
@@ -82,19 +82,19 @@ __global__ void __launch_bounds__(kBlock) k_sst_blend(const Inst* insts) {
   }
   {
     const int fi = (j + 1) * nI + i;
-    const FaceGeo& f = I.jg[fi];
+    const FaceGeo f = geo(I.jg, fi);
     if (j < nJ - 1) interior(f, c, c + nI, true);
     else boundary(f, I.bcT[i], I.Fj[fi]);
   }
   {
     const int fi = j * (nI + 1) + i;
-    const FaceGeo& f = I.ig[fi];
+    const FaceGeo f = geo(I.ig, fi);
     if (i > 0) interior(f, c - 1, c, false);
     else boundary(f, I.bcL[j], I.Fi[fi]);
   }
   {
     const int fi = j * (nI + 1) + i + 1;
-    const FaceGeo& f = I.ig[fi];
+    const FaceGeo f = geo(I.ig, fi);
     if (i < nI - 1) interior(f, c, c + 1, true);
     else boundary(f, I.bcR[j], I.Fi[fi]);
   }
@@ -179,7 +179,7 @@ __device__ __forceinline__ SstRow sstRow(const Inst& I, int c, int i, int j, dou
   {
     const int fi = j * nI + i;
     const double F = I.Fj[fi];
-    if (j > 0) r.aS = sstInteriorOff<isK>(I, I.jg[fi].w, I.jgSst[fi], F, c - nI, c, false, r.d);
+    if (j > 0) r.aS = sstInteriorOff<isK>(I, I.jg.w[fi], I.jgSst[fi], F, c - nI, c, false, r.d);
     else {
       const int pr = I.partner[i];
       if (pr >= 0) r.aS = sstInteriorOff<isK>(I, 1.0, I.jgSst[fi], F, c, pr, true, r.d);
@@ -189,19 +189,19 @@ __device__ __forceinline__ SstRow sstRow(const Inst& I, int c, int i, int j, dou
   {
     const int fi = (j + 1) * nI + i;
     const double F = I.Fj[fi];
-    if (j < nJ - 1) r.aN = sstInteriorOff<isK>(I, I.jg[fi].w, I.jgSst[fi], F, c, c + nI, true, r.d);
+    if (j < nJ - 1) r.aN = sstInteriorOff<isK>(I, I.jg.w[fi], I.jgSst[fi], F, c, c + nI, true, r.d);
     else sstBoundaryRow<isK>(I, I.bcT[i], F, c, wallValue, inflow, r.d, r.b);
   }
   {
     const int fi = j * (nI + 1) + i;
     const double F = I.Fi[fi];
-    if (i > 0) r.aW = sstInteriorOff<isK>(I, I.ig[fi].w, I.igSst[fi], F, c - 1, c, false, r.d);
+    if (i > 0) r.aW = sstInteriorOff<isK>(I, I.ig.w[fi], I.igSst[fi], F, c - 1, c, false, r.d);
     else sstBoundaryRow<isK>(I, I.bcL[j], F, c, wallValue, inflow, r.d, r.b);
   }
   {
     const int fi = j * (nI + 1) + i + 1;
     const double F = I.Fi[fi];
-    if (i < nI - 1) r.aE = sstInteriorOff<isK>(I, I.ig[fi].w, I.igSst[fi], F, c, c + 1, true, r.d);
+    if (i < nI - 1) r.aE = sstInteriorOff<isK>(I, I.ig.w[fi], I.igSst[fi], F, c, c + 1, true, r.d);
     else sstBoundaryRow<isK>(I, I.bcR[j], F, c, wallValue, inflow, r.d, r.b);
   }
   return r;
@@ -332,20 +332,22 @@ __global__ void __launch_bounds__(kBlock) k_sst_wall(const Inst* insts) {
   double ymax[1] = {0.0};
   for (int e = static_cast<int>(threadIdx.x); e < total; e += kBlock) {
     const FaceBC* bc;
-    const FaceGeo* f;
-    int o;
-    if (e < nI) { bc = &I.bcB[e]; f = &I.jg[e]; o = e; }
-    else if (e < 2 * nI) { const int q = e - nI; bc = &I.bcT[q]; f = &I.jg[nJ * nI + q]; o = (nJ - 1) * nI + q; }
-    else if (e < 2 * nI + nJ) { const int q = e - 2 * nI; bc = &I.bcL[q]; f = &I.ig[q * (nI + 1)]; o = q * nI; }
-    else { const int q = e - 2 * nI - nJ; bc = &I.bcR[q]; f = &I.ig[q * (nI + 1) + nI]; o = q * nI + nI - 1; }
+    int fi, o;
+    bool jFace;
+    if (e < nI) { bc = &I.bcB[e]; fi = e; jFace = true; o = e; }
+    else if (e < 2 * nI) { const int q = e - nI; bc = &I.bcT[q]; fi = nJ * nI + q; jFace = true; o = (nJ - 1) * nI + q; }
+    else if (e < 2 * nI + nJ) { const int q = e - 2 * nI; bc = &I.bcL[q]; fi = q * (nI + 1); jFace = false; o = q * nI; }
+    else { const int q = e - 2 * nI - nJ; bc = &I.bcR[q]; fi = q * (nI + 1) + nI; jFace = false; o = q * nI + nI - 1; }
     if (bc->kind != kWall) continue;
+    const double fSx = jFace ? I.jg.Sx[fi] : I.ig.Sx[fi];
+    const double fSy = jFace ? I.jg.Sy[fi] : I.ig.Sy[fi];
     const double d = mx(I.wallDist[o], kTinySst);
     const double rho = mx(I.rho[o], kTinySst);
     const double nu = I.muMol[o] / rho;
     I.w[o] = wallOmega(I, nu, d, I.k[o]);
     // y+ with the tangential velocity of the wall cell (KOmegaSST.cpp:337-343)
-    const double area = hypot(f->Sx, f->Sy);
-    const double nx = -(f->Sx / area), ny = -(f->Sy / area);
+    const double area = hypot(fSx, fSy);
+    const double nx = -(fSx / area), ny = -(fSy / area);
     const double uo = I.u[o], vo = I.v[o];
     const double un = uo * nx + vo * ny;
     const double px = uo - nx * un, py = vo - ny * un;

@@ -242,8 +242,15 @@ void carveInst(Carver& cv, Inst& I, const Grid& grid) {
   const size_t nI = static_cast<size_t>(grid.nI), nJ = static_cast<size_t>(grid.nJ);
   {
     I.V = cv.take<double>(N);
-    I.jg = cv.take<FaceGeo>(nJf);
-    I.ig = cv.take<FaceGeo>(nIf);
+    for (FaceGeoArrays* a : {&I.jg, &I.ig}) {
+      const size_t n = (a == &I.jg) ? nJf : nIf;
+      a->Sx = cv.take<double>(n);
+      a->Sy = cv.take<double>(n);
+      a->Tx = cv.take<double>(n);
+      a->Ty = cv.take<double>(n);
+      a->g = cv.take<double>(n);
+      a->w = cv.take<double>(n);
+    }
     I.partner = cv.take<int>(nI);
     I.bcB = cv.take<FaceBC>(nI);
     I.bcT = cv.take<FaceBC>(nI);
@@ -337,8 +344,32 @@ void Engine::uploadStatic() {
     check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "upload");
   };
   up(const_cast<double*>(h_.V), grid_.area.data(), N * sizeof(double));
-  up(const_cast<FaceGeo*>(h_.jg), jg_.data(), jg_.size() * sizeof(FaceGeo));
-  up(const_cast<FaceGeo*>(h_.ig), ig_.data(), ig_.size() * sizeof(FaceGeo));
+  // each field of the face geometry as an array of its own (staged in a host
+  // buffer that lives until the stream is synchronised below)
+  std::vector<double> fields;
+  for (const std::vector<FaceGeo>* src : {&jg_, &ig_}) {
+    const size_t n = src->size();
+    const size_t at = fields.size();
+    fields.resize(at + 6 * n);
+    for (size_t f = 0; f < n; ++f) {
+      const FaceGeo& g = (*src)[f];
+      fields[at + f] = g.Sx;
+      fields[at + n + f] = g.Sy;
+      fields[at + 2 * n + f] = g.Tx;
+      fields[at + 3 * n + f] = g.Ty;
+      fields[at + 4 * n + f] = g.g;
+      fields[at + 5 * n + f] = g.w;
+    }
+  }
+  {
+    size_t at = 0;
+    for (const FaceGeoArrays* a : {&h_.jg, &h_.ig}) {
+      const size_t n = (a == &h_.jg) ? jg_.size() : ig_.size();
+      const double* const dst[6] = {a->Sx, a->Sy, a->Tx, a->Ty, a->g, a->w};
+      for (int k = 0; k < 6; ++k) up(const_cast<double*>(dst[k]), fields.data() + at + k * n, n * sizeof(double));
+      at += 6 * n;
+    }
+  }
   up(const_cast<int*>(h_.partner), grid_.bottomPartner.data(), grid_.bottomPartner.size() * sizeof(int));
   up(const_cast<FaceBC*>(h_.bcB), bcB_.data(), bcB_.size() * sizeof(FaceBC));
   up(const_cast<FaceBC*>(h_.bcT), bcT_.data(), bcT_.size() * sizeof(FaceBC));
