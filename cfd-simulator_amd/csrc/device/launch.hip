@@ -24,8 +24,7 @@ __global__ void k_cg_init(const Inst*, int);
 __global__ void k_cg_a(const Inst*, int);
 __global__ void k_cg_b(const Inst*, int);
 __global__ void k_corr_mean(const Inst*);
-__global__ void k_correct_cells(const Inst*);
-__global__ void k_correct_faces(const Inst*);
+__global__ void k_correct(const Inst*);
 __global__ void k_divergence(const Inst*, double*);
 __global__ void k_monitor(const Inst*);
 __global__ void k_sst_blend(const Inst*);
@@ -189,8 +188,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     end(CFDGPU_STAGE_PRESSURE_SOLVE);
     begin(CFDGPU_STAGE_CORRECTION);
     if (!h.hasPRef) hipLaunchKernelGGL(k_corr_mean, grid, block, 0, s, d);
-    hipLaunchKernelGGL(k_correct_cells, grid, block, 0, s, d);
-    hipLaunchKernelGGL(k_correct_faces, grid, block, 0, s, d);
+    hipLaunchKernelGGL(k_correct, grid, block, 0, s, d);
     end(CFDGPU_STAGE_CORRECTION);
   }
   begin(CFDGPU_STAGE_CORRECTION);

@@ -1248,12 +1248,10 @@ __device__ __forceinline__ double corrMean(const Inst& I, double* sm) {
   return I.hasPRef ? 0.0 : blockSumPartials(partialSlot(I, sCorrMean), I.nBlocks, sm) / static_cast<double>(I.N);
 }
 
-__global__ void __launch_bounds__(kBlock) k_correct_cells(const Inst* insts) {
-  CFD_PROLOGUE
-  __shared__ double sm[4];
-  const double mean = corrMean(I, sm);
+// p, u and v of cell c (SimpleSolver.cpp:582-632).
+__device__ __forceinline__ void correctCell(const Inst& I, int c, int i, int j, double mean) {
+  const int nI = I.nI, nJ = I.nJ;
   auto corrAt = [&](const Inst& J, int q) { return J.hasPRef ? J.x[q] : J.x[q] - mean; };
-  if (!live) return;
   double gx = 0.0, gy = 0.0;
   const double pc = corrAt(I, c);
   auto add = [&](const FaceGeo& f, double val, bool plus) {
@@ -1318,11 +1316,10 @@ __device__ __forceinline__ void corrBoundaryFlux(const Inst& I, const FaceGeo& f
   *F -= dens * mob * f.g * (0.0 - po);
 }
 
-__global__ void __launch_bounds__(kBlock) k_correct_faces(const Inst* insts) {
-  CFD_PROLOGUE
-  __shared__ double sm[4];
-  const double mean = corrMean(I, sm);
-  if (!live) return;
+// The face fluxes cell c owns: its j-face and i-face, and the top and right
+// boundary faces on the last row and column (SimpleSolver.cpp:634-660).
+__device__ __forceinline__ void correctFaces(const Inst& I, int c, int i, int j, double mean) {
+  const int nI = I.nI, nJ = I.nJ;
   {
     const int fi = j * nI + i;
     const FaceGeo f = geo(I.jg, fi);
@@ -1347,6 +1344,19 @@ __global__ void __launch_bounds__(kBlock) k_correct_faces(const Inst* insts) {
     const int fi = j * (nI + 1) + i + 1;
     corrBoundaryFlux(I, geo(I.ig, fi), I.bcR[j], &I.Fi[fi], c, mean);
   }
+}
+
+// The correction of one pressure pass: every cell's p, u, v, then the face
+// fluxes it owns. A cell's own update reads the old flux only on the boundary
+// faces it owns itself, before correcting them, so one kernel does both stages
+// with the reference's results.
+__global__ void __launch_bounds__(kBlock) k_correct(const Inst* insts) {
+  CFD_PROLOGUE
+  __shared__ double sm[4];
+  const double mean = corrMean(I, sm);
+  if (!live) return;
+  correctCell(I, c, i, j, mean);
+  correctFaces(I, c, i, j, mean);
 }
 
 // SimpleSolver::divergence (SimpleSolver.cpp:796-817) -> max |div|.
