@@ -11,9 +11,11 @@ by tests/golden/make_fixtures.py in the development container):
 
 Comparisons are made at matched iteration counts: the device runs to just
 before the oracle's stop, then one iteration at a time past it, recording
-where its own stop rule fires (within two iterations of the oracle's, or of
-the farthest stop of the oracle's round-off probes, below) and its surface
-and forces at the oracle's iteration.
+where its own stop rule fires and its surface, forces and residual at the
+oracle's iteration. The stops must agree within two iterations (or the
+farthest stop of the oracle's round-off probes, below), or, where the
+residual grazes the tolerance at the oracle's stop, the device's residual
+there must be within 2 % of the oracle's.
 
 The two differ by rounding alone: the order of global sums (block and tree
 sums on the device, one sequential pass in the oracle) and the last bits of
@@ -79,8 +81,9 @@ def stop_window(fx):
 
 
 def run_matched(G, s, fx):
-    """Returns (device stop iteration or None, forces, rows) with forces and
-    rows taken at the oracle's stop iteration."""
+    """Returns (device stop iteration or None, forces, rows, device worst
+    residual) with forces, rows and the residual taken at the oracle's stop
+    iteration."""
     n_o, tol = fx["iterations"], fx["tolerance"]
     window = stop_window(fx) or 2
     pre = max(0, n_o - window - 1)
@@ -93,10 +96,12 @@ def run_matched(G, s, fx):
             first = rep.iterations
         else:
             assert rep.status == 1, f"device status {rep.status} before the oracle's stop"
+    worst = None
     while it < n_o:
         mon = s.iterate(1)
         it += 1
-        if first is None and mon.worst() < tol:
+        worst = mon.worst()
+        if first is None and worst < tol:
             first = it
     f, rows = s.surface(speed=1.0, alpha_deg=fx["alpha"], density=1.0, reference_pressure=0.0)
     while first is None and it < n_o + window:
@@ -104,7 +109,15 @@ def run_matched(G, s, fx):
         it += 1
         if mon.worst() < tol:
             first = it
-    return first, f, rows
+    return first, f, rows, worst
+
+
+# Relative difference of the device's and the oracle's worst flow residual at
+# the oracle's stop within which a stop outside the window is put down to the
+# residual grazing the tolerance there (a residual that hovers within a few
+# per cent of the tolerance crosses it tens of iterations apart on two
+# roundings of the same trajectory).
+GRAZE = 0.02
 
 
 # Multiple of the oracle's own change under the round-off probes that a
@@ -125,10 +138,17 @@ def probe_change(fx):
     return out
 
 
-def compare(fx, first, f, rows, coef_tol, abs_floor, surf_tol):
+def compare(fx, first, f, rows, coef_tol, abs_floor, surf_tol, worst=None):
     name = fx["name"]
     window = stop_window(fx)
-    assert first is not None and f is not None, f"{name}: no stop near the oracle's {fx['iterations']}"
+    assert f is not None, f"{name}: the device stopped before the oracle's {fx['iterations']} (at {first})"
+    m = fx["monitor"]
+    oracle_worst = max(abs(m["continuity"]), abs(m["momentum_x"]), abs(m["momentum_y"]))
+    in_window = first is not None and window is not None and abs(first - fx["iterations"]) <= window
+    graze = worst is not None and abs(worst - oracle_worst) <= GRAZE * oracle_worst
+    print(f"{name}: device worst residual {worst} at the oracle's stop (oracle {oracle_worst:.6e}); "
+          f"device stop {first}", flush=True)
+    assert in_window or graze, (first, window, worst, oracle_worst)
     ref = fx["forces"]
     sens = probe_change(fx)
     errs, bars = {}, {}
@@ -145,7 +165,6 @@ def compare(fx, first, f, rows, coef_tol, abs_floor, surf_tol):
           f"(oracle {fx['upper_separation_face']},{fx['lower_separation_face']})", flush=True)
     print("  bar: " + ", ".join(f"{q} {bars[q]:.2e}" for q in bars) + f"; stop window {window}; probe change: "
           + ", ".join(f"{q} {sens[q]:.1e}" for q in sens), flush=True)
-    assert window is not None and abs(first - fx["iterations"]) <= window, (first, window)
     assert np.array_equal(rows[:, 1].astype(int), np.array(fx["station_face"]))
     for q, e in errs.items():
         assert e <= bars[q], (q, e, bars[q])
@@ -159,9 +178,9 @@ def test_laminar_256x64_to_1e8(G, gpu_available, name):
     fx = load(name)
     print(f"case: {name}: {fx['cells']} cells, oracle converged after {fx['iterations']} iterations", flush=True)
     mesh, s = build(G, fx)
-    first, f, rows = run_matched(G, s, fx)
+    first, f, rows, worst = run_matched(G, s, fx)
     # alpha 0: Cl and Cm vanish by symmetry, so they are compared absolutely (1e-8)
-    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5)
+    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5, worst=worst)
 
 
 @pytest.mark.timeout(300)
@@ -174,8 +193,8 @@ def test_sst_512x128_polar_point(G, gpu_available, name):
         rep = s.solve(fx["tolerance"], fx["limit"], check_every=100)
         assert rep.status == fx["status"]
         return
-    first, f, rows = run_matched(G, s, fx)
-    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5)
+    first, f, rows, worst = run_matched(G, s, fx)
+    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5, worst=worst)
 
 
 # Instances whose device answer is outside the bar, with what was measured.
@@ -210,5 +229,5 @@ def test_mixed_batch_coarse_instance(G, gpu_available, name):
         if fx["status"] == 2:
             assert abs(rep.iterations - fx["iterations"]) <= 2
         return
-    first, f, rows = run_matched(G, s, fx)
-    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5)
+    first, f, rows, worst = run_matched(G, s, fx)
+    compare(fx, first, f, rows, coef_tol=1e-6, abs_floor=1e-8, surf_tol=1e-5, worst=worst)
