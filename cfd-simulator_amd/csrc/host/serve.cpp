@@ -4,7 +4,10 @@
 #include "serve.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -22,6 +25,22 @@ constexpr size_t kDonorBytes = size_t(2) << 30;   // host bytes of kept warm-sta
 
 void hipCheck(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Runs task(0 .. n-1) on up to `threads` host threads (the calling one
+// included), each with `device` current. Exceptions stay with the tasks.
+void parallelFor(int n, int threads, int device, const std::function<void(int)>& task) {
+  std::atomic<int> next{0};
+  auto work = [&](bool setDevice) {
+    if (setDevice) (void)hipSetDevice(device);
+    for (int q = next++; q < n; q = next++) task(q);
+  };
+  const int extra = std::max(0, std::min(threads, n) - 1);
+  std::vector<std::thread> pool;
+  pool.reserve(static_cast<size_t>(extra));
+  for (int t = 0; t < extra; ++t) pool.emplace_back(work, true);
+  work(false);
+  for (std::thread& t : pool) t.join();
 }
 
 }  // namespace
@@ -59,6 +78,12 @@ Server::Server(const ServerOptions& options) : opt_(options), t0_(std::chrono::s
   if (!(opt_.budgetBytes > 0.0)) throw std::invalid_argument("the memory budget must be positive");
   if (opt_.model != CFDGPU_MODEL_NONE && opt_.model != CFDGPU_MODEL_LAMINAR && opt_.model != CFDGPU_MODEL_SST)
     throw std::invalid_argument("unknown turbulence model");
+  hipCheck(hipGetDevice(&device_), "hipGetDevice");
+  // the host set-up of admitted requests runs on up to 16 threads
+  // (CFDGPU_ADMIT_THREADS overrides, 1 builds them one after another)
+  const unsigned hw = std::thread::hardware_concurrency();
+  admitThreads_ = std::max(1, std::min(16, static_cast<int>(hw ? hw : 1)));
+  if (const char* e = std::getenv("CFDGPU_ADMIT_THREADS")) admitThreads_ = std::max(1, std::min(64, std::atoi(e)));
   hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   hipCheck(hipMalloc(&dInsts_, sizeof(Inst) * static_cast<size_t>(opt_.maxSlots)), "hipMalloc (roster)");
   hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hFlags_), sizeof(int) * kIFlags * static_cast<size_t>(opt_.maxSlots),
@@ -106,26 +131,29 @@ int Server::submit(const ServeRequest& r) {
 
 // One mesh per (section, C-grid options), built on first use and kept while
 // any resident solve refers to it or it is among the most recent.
-std::shared_ptr<const Grid> Server::meshFor(const ServeRequest& r) {
+Server::MeshKey Server::meshKeyOf(const ServeRequest& r) {
   const CGridOptions& o = r.mesh;
-  const MeshKey key{r.m, r.p, r.t, r.chord, o.upstream, o.downstream, o.vertical, o.surfacePoints,
-                    o.wakePoints, o.normalPoints, o.firstLayer, o.blendLength, o.smoothingPasses};
+  return MeshKey{r.m, r.p, r.t, r.chord, o.upstream, o.downstream, o.vertical, o.surfacePoints,
+                 o.wakePoints, o.normalPoints, o.firstLayer, o.blendLength, o.smoothingPasses};
+}
+
+std::shared_ptr<const Grid> Server::cachedMesh(const MeshKey& key) {
   for (auto it = meshes_.begin(); it != meshes_.end(); ++it) {
     if (it->first == key) {
       meshes_.splice(meshes_.begin(), meshes_, it);
       return meshes_.front().second;
     }
   }
-  auto g = std::make_shared<Grid>();
-  std::string err;
-  if (!makeCGrid(*g, r.m, r.p, r.t, r.chord, o, err)) throw std::invalid_argument(err);
-  meshes_.emplace_front(key, g);
+  return nullptr;
+}
+
+void Server::keepMesh(const MeshKey& key, std::shared_ptr<const Grid> g) {
+  meshes_.emplace_front(key, std::move(g));
   // drop the oldest meshes no solve holds
   for (auto it = meshes_.end(); meshes_.size() > kMeshCache && it != meshes_.begin();) {
     --it;
     if (it->second.use_count() == 1) it = meshes_.erase(it);
   }
-  return g;
 }
 
 // A new solver for the slot's request: the uniform field of its freestream
@@ -187,43 +215,111 @@ void Server::startAttempt(Slot& s, bool cold) {
   s.engine = std::move(e);
 }
 
-// Admits the earliest queued request if it has arrived, a slot is free and
-// its device state fits the budget. Requests are admitted in arrival order.
-bool Server::admitOne() {
-  if (pending_.empty()) return false;
-  const int ticket = pending_.front();
+// A request that cannot be served (its device state exceeds the budget, or
+// its mesh or solver could not be built) leaves with status 3.
+void Server::reject(int ticket, const std::shared_ptr<const Grid>& grid) {
   const ServeRequest& r = requests_[static_cast<size_t>(ticket)];
-  if (r.arrival > clock()) return false;
-  if (static_cast<int>(active_.size() + joining_.size()) >= opt_.maxSlots) return false;
-  auto s = std::make_unique<Slot>();
-  s->ticket = ticket;
-  try {
-    s->grid = meshFor(r);
-    const size_t need = Engine::deviceBytesFor(*s->grid);
-    if (static_cast<double>(need) > opt_.budgetBytes) throw std::invalid_argument("exceeds the memory budget");
-    if (resident_ + static_cast<double>(need) > opt_.budgetBytes) return false;  // waits for memory
-    pending_.pop_front();
-    s->admitted = clock();
-    startAttempt(*s, false);
-  } catch (const std::exception&) {
-    if (!pending_.empty() && pending_.front() == ticket) pending_.pop_front();
-    cfdgpu_serve_result res{};
-    res.ticket = ticket;
-    res.status = 3;
-    res.warm_from = -1;
-    res.upper_separation = res.lower_separation = -1.0;
-    res.arrival_s = r.arrival;
-    res.admitted_s = res.finished_s = clock();
-    res.cells = s->grid ? s->grid->cells() : 0;
-    done_.push_back(res);
-    ++finished_;
-    return true;
+  cfdgpu_serve_result res{};
+  res.ticket = ticket;
+  res.status = 3;
+  res.warm_from = -1;
+  res.upper_separation = res.lower_separation = -1.0;
+  res.arrival_s = r.arrival;
+  res.admitted_s = res.finished_s = clock();
+  res.cells = grid ? grid->cells() : 0;
+  done_.push_back(res);
+  ++finished_;
+}
+
+// Admits, in arrival order, the queued requests that have arrived while slots
+// are free and their device state fits the budget (the first that does not
+// fit waits for memory, and the ones behind it wait with it). Their meshes
+// (the ones not cached) and their solvers (host geometry, device arena,
+// uploads, wall distance) are built on host threads side by side: the set-up
+// of a 1024 x 256 request takes tens of milliseconds on the host, and a burst
+// of arrivals would otherwise hold the next chunk back for the sum of them.
+void Server::admitReady() {
+  const size_t busy = active_.size() + joining_.size();
+  const size_t free = busy < static_cast<size_t>(opt_.maxSlots) ? static_cast<size_t>(opt_.maxSlots) - busy : 0;
+  std::vector<int> cand;
+  for (const int t : pending_) {
+    if (cand.size() >= free || requests_[static_cast<size_t>(t)].arrival > clock()) break;
+    cand.push_back(t);
   }
-  resident_ += static_cast<double>(s->bytes);
-  peakResident_ = std::max(peakResident_, resident_);
-  ++admitted_;
-  joining_.push_back(std::move(s));
-  return true;
+  if (cand.empty()) return;
+  // meshes: the cached ones, then the missing distinct keys built in parallel
+  const size_t n = cand.size();
+  std::vector<MeshKey> keys(n);
+  std::vector<std::shared_ptr<const Grid>> grids(n);
+  std::vector<int> build;  // index of the first candidate of each missing key
+  for (size_t k = 0; k < n; ++k) {
+    keys[k] = meshKeyOf(requests_[static_cast<size_t>(cand[k])]);
+    grids[k] = cachedMesh(keys[k]);
+    if (grids[k]) continue;
+    bool dup = false;
+    for (const int b : build) dup = dup || keys[static_cast<size_t>(b)] == keys[k];
+    if (!dup) build.push_back(static_cast<int>(k));
+  }
+  std::vector<std::shared_ptr<Grid>> built(build.size());
+  parallelFor(static_cast<int>(build.size()), admitThreads_, device_, [&](int b) {
+    const ServeRequest& r = requests_[static_cast<size_t>(cand[static_cast<size_t>(build[static_cast<size_t>(b)])])];
+    try {
+      auto g = std::make_shared<Grid>();
+      std::string err;
+      if (makeCGrid(*g, r.m, r.p, r.t, r.chord, r.mesh, err)) built[static_cast<size_t>(b)] = g;
+    } catch (const std::exception&) {
+    }
+  });
+  for (size_t b = 0; b < build.size(); ++b)
+    if (built[b]) keepMesh(keys[static_cast<size_t>(build[b])], built[b]);
+  for (size_t k = 0; k < n; ++k)
+    if (!grids[k]) grids[k] = cachedMesh(keys[k]);
+  // the budget, in arrival order
+  std::vector<std::unique_ptr<Slot>> admit;
+  double planned = 0.0;
+  size_t taken = 0;
+  for (size_t k = 0; k < n; ++k) {
+    const int ticket = cand[k];
+    if (!grids[k]) {  // the mesh could not be built
+      reject(ticket, nullptr);
+      ++taken;
+      continue;
+    }
+    const double need = static_cast<double>(Engine::deviceBytesFor(*grids[k]));
+    if (need > opt_.budgetBytes) {
+      reject(ticket, grids[k]);
+      ++taken;
+      continue;
+    }
+    if (resident_ + planned + need > opt_.budgetBytes) break;  // waits for memory
+    planned += need;
+    auto s = std::make_unique<Slot>();
+    s->ticket = ticket;
+    s->grid = grids[k];
+    s->admitted = clock();
+    admit.push_back(std::move(s));
+    ++taken;
+  }
+  for (size_t k = 0; k < taken; ++k) pending_.pop_front();
+  // the solvers, in parallel
+  std::vector<char> ok(admit.size(), 0);
+  parallelFor(static_cast<int>(admit.size()), admitThreads_, device_, [&](int q) {
+    try {
+      startAttempt(*admit[static_cast<size_t>(q)], false);
+      ok[static_cast<size_t>(q)] = 1;
+    } catch (const std::exception&) {
+    }
+  });
+  for (size_t q = 0; q < admit.size(); ++q) {
+    if (!ok[q]) {
+      reject(admit[q]->ticket, admit[q]->grid);
+      continue;
+    }
+    resident_ += static_cast<double>(admit[q]->bytes);
+    peakResident_ = std::max(peakResident_, resident_);
+    ++admitted_;
+    joining_.push_back(std::move(admit[q]));
+  }
 }
 
 void Server::finish(Slot& s, int status, int iterations, double continuity) {
@@ -409,8 +505,7 @@ bool Server::settled(Slot& s, int iteration) {
 void Server::step() {
   launchChunk();
   // admission overlaps the chunk running on the device
-  while (admitOne()) {
-  }
+  admitReady();
   if (launched_) retire();
   if (!joining_.empty()) {
     for (auto& s : joining_) active_.push_back(std::move(s));

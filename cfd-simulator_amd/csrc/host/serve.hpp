@@ -89,8 +89,11 @@ class Server {
   using MeshKey = std::tuple<int, int, int, double, double, double, double, int, int, int, double, double, int>;
   using DonorKey = std::tuple<int, int, int, double, int, int, double, double>;
 
-  std::shared_ptr<const Grid> meshFor(const ServeRequest& r);
-  bool admitOne();
+  static MeshKey meshKeyOf(const ServeRequest& r);
+  std::shared_ptr<const Grid> cachedMesh(const MeshKey& key);
+  void keepMesh(const MeshKey& key, std::shared_ptr<const Grid> g);
+  void admitReady();
+  void reject(int ticket, const std::shared_ptr<const Grid>& grid);
   void startAttempt(Slot& s, bool cold);
   void finish(Slot& s, int status, int iterations, double continuity);
   bool settled(Slot& s, int iteration);
@@ -98,6 +101,8 @@ class Server {
   void retire();
 
   ServerOptions opt_;
+  int device_{0};        // the GPU the server's solves run on
+  int admitThreads_{1};  // host threads that build admitted requests' meshes and solvers
   std::chrono::steady_clock::time_point t0_;
   std::vector<ServeRequest> requests_;
   std::deque<int> pending_;  // tickets in arrival order
