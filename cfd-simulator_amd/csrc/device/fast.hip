@@ -915,7 +915,7 @@ __global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int
   const int nb = rbBlocks(nI, nJ);
   if (static_cast<int>(blockIdx.x) >= nb) return;
   if (I.flags[fStopped]) return;
-  const int q = xcdTile(static_cast<int>(blockIdx.x), nb) * kBlock + static_cast<int>(threadIdx.x);
+  const int q0 = xcdTile(static_cast<int>(blockIdx.x), nb) * (kBlock * kRbCells) + static_cast<int>(threadIdx.x);
   const int countC = color ? N - count0 : count0;
   const int countO = color ? count0 : N - count0;
   const float* const Am[2][5] = {{I.fMom[0], I.fMom[1], I.fMom[2], I.fMom[3], I.fMom[4]},
@@ -924,22 +924,26 @@ __global__ void __launch_bounds__(kBlock) k_rb32(const Inst* insts, int sys, int
                                  {I.fTurb[1][0], I.fTurb[1][1], I.fTurb[1][2], I.fTurb[1][3], I.fTurb[1][4]}};
   const float* const R[2] = {sys == 0 ? I.fMomRes[0] : I.fTurbRes[0], sys == 0 ? I.fMomRes[1] : I.fTurbRes[1]};
   float* const D[2] = {sys == 0 ? I.fMomDelta[0] : I.fTurbDelta[0], sys == 0 ? I.fMomDelta[1] : I.fTurbDelta[1]};
-  if (q < countC) {
-    int i, j;
-    rbCell(q, color, nI, i, j);
-    const int pk = (color ? count0 : 0) + q;
-    float d[2];
-    // u and v share their coefficients (sys 0); k and omega have their own
-    if (sys == 0) rbSolveCell<2>(Am, R, D, I, i, j, pk, first != 0, d);
-    else rbSolveCell<2>(At, R, D, I, i, j, pk, first != 0, d);
-    if (last) rbApply(I, sys, j * nI + i, d);
-  }
-  if (last && q < countO) {
-    int i, j;
-    rbCell(q, 1 - color, nI, i, j);
-    const int pk = (color ? 0 : count0) + q;
-    const float d[2] = {D[0][pk], D[1][pk]};
-    rbApply(I, sys, j * nI + i, d);
+#pragma unroll
+  for (int k = 0; k < kRbCells; ++k) {
+    const int q = q0 + k * kBlock;
+    if (q < countC) {
+      int i, j;
+      rbCell(q, color, nI, i, j);
+      const int pk = (color ? count0 : 0) + q;
+      float d[2];
+      // u and v share their coefficients (sys 0); k and omega have their own
+      if (sys == 0) rbSolveCell<2>(Am, R, D, I, i, j, pk, first != 0, d);
+      else rbSolveCell<2>(At, R, D, I, i, j, pk, first != 0, d);
+      if (last) rbApply(I, sys, j * nI + i, d);
+    }
+    if (last && q < countO) {
+      int i, j;
+      rbCell(q, 1 - color, nI, i, j);
+      const int pk = (color ? 0 : count0) + q;
+      const float d[2] = {D[0][pk], D[1][pk]};
+      rbApply(I, sys, j * nI + i, d);
+    }
   }
 }
 

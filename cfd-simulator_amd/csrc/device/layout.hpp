@@ -165,11 +165,14 @@ __host__ __device__ inline void rbCell(int q, int C, int nI, int& i, int& j) {
 }
 
 constexpr int kBlock = 256;          // threads per block for cell-parallel kernels
-// Blocks of a one-lane-per-cell launch over the larger colour.
+// The two-colour sweep's lanes each take kRbCells cells of the swept colour
+// (kBlock apart; 1 measured faster than 2); blocks of such a launch over the
+// larger colour.
+constexpr int kRbCells = 1;
 __host__ __device__ inline int rbBlocks(int nI, int nJ) {
   const int c0 = rbRowBase(nJ, 0, nI), c1 = nI * nJ - c0;
   const int m = c0 > c1 ? c0 : c1;
-  return (m + kBlock - 1) / kBlock;
+  return (m + kBlock * kRbCells - 1) / (kBlock * kRbCells);
 }
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
 constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)

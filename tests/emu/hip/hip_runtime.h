@@ -206,6 +206,7 @@ inline const char* hipGetErrorString(hipError_t) { return "emulation error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipSetDevice(int d) { return d == 0 ? hipSuccess : 1; }
 #define __align__(n) alignas(n)
 #define HIP_SYMBOL(x) (&(x))
 template <class T>
