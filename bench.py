@@ -109,7 +109,7 @@ def precision_of(G, args):
 def settings_summary(st):
     return {"precision": ["fp64", "mixed"][st.precision], "gs_ordering": ["lexicographic", "two-colour"][st.gs_ordering],
             "pressure_solver": ["jacobi-cg", "line-cg"][st.pressure_solver], "cg_cap": st.pressure_iterations,
-            "refinement_passes": st.refinement_passes}
+            "refinement_passes": st.refinement_passes, "momentum_sweeps": st.momentum_sweeps}
 
 
 def path_settings(G, args, **kw):
@@ -311,7 +311,7 @@ def run_roofline(args):
     if args.only:
         rows = [rows[int(q)] for q in args.only.split(",")]
     fast_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=G.MIXED, gs_ordering=1,
-                                 pressure_solver=1, pressure_iterations=args.fast_cg)
+                                 pressure_solver=1, pressure_iterations=args.fast_cg, momentum_sweeps=args.fast_sweeps)
     cells, ms_per_step, stage_ms, calls, cg_cells = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
     print(json.dumps({"diagnostic": f"fast path on {len(rows)} instances of the 64-instance mixed-size batch: step "
                                     "time and the line-CG roofline (not the metric)",
@@ -331,7 +331,7 @@ def run_headline(args):
     H = HEADLINE
     rows = W.load(W.MIXED_BATCH_FILE)
     fast_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0, precision=G.MIXED, gs_ordering=1,
-                                 pressure_solver=1, pressure_iterations=args.fast_cg)
+                                 pressure_solver=1, pressure_iterations=args.fast_cg, momentum_sweeps=args.fast_sweeps)
     plain_st = G.default_settings(intensity=0.001, viscosity_ratio=10.0)
     common = dict(tolerance=H["tolerance"], max_iterations=H["max_iterations"], chunk=H["chunk"], slots=H["slots"],
                   settle_window=H["settle_window"], settle_cl=H["settle_cl"], settle_cd=H["settle_cd"])
@@ -901,6 +901,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the headline's timed region and its line-CG roofline (for kernel traces)")
     ap.add_argument("--fast-cg", type=int, default=15, help="line-CG cap of the fast path in the headline run")
+    ap.add_argument("--fast-sweeps", type=int, default=3,
+                    help="momentum sweeps per outer iteration on the fast path (the reference's default 3)")
     ap.add_argument("--fast-cg-medium", type=int, default=None, help="line-CG cap on the 512x128 instances")
     ap.add_argument("--fast-cg-fine", type=int, default=None, help="line-CG cap on the 1024x256 instances")
     ap.add_argument("--plain-results", help="headline: take the plain FP64 run from this --results-file")
