@@ -276,7 +276,24 @@ def apply_tuning(args):
     LAUNCH_TUNING.update(cells_per_thread=cells, column_rows=rows)
 
 
-def batch_roofline(fast_st, cells, stage_ms, calls, cg_cells):
+LCG_SEG_MAX_ROWS = 256  # launch B splits each column over several waves up to this many rows (layout.hpp)
+
+
+def lcg_kernels(max_rows):
+    """The names of the line-CG launch pair the engine picks for a batch whose widest
+    member has max_rows wall-normal rows under the current launch tuning (launch.hip)."""
+    cells = LAUNCH_TUNING["cells_per_thread"]
+    a = "k_lcg_a_f32_tile" if cells == 0 else f"k_lcg_a_f32_c{cells}"
+    b = "k_lcg_b_f32_seg" if max_rows <= LCG_SEG_MAX_ROWS else f"k_lcg_b_f32_u{LAUNCH_TUNING['column_rows']}"
+    return f"{a} + {b}"
+
+
+def batch_max_rows(W, rows):
+    """Wall-normal rows of the widest member of a manifest's batch."""
+    return max(W.LEVELS[r["level"]][2] - 1 for r in rows)
+
+
+def batch_roofline(fast_st, cells, stage_ms, calls, cg_cells, max_rows):
     """The roofline object of the fast path's line-CG iteration on the batch."""
     tm = traffic_model()
     # one line-CG iteration of the whole batch (launch A + launch B): the CG loop's HIP-event time
@@ -289,8 +306,7 @@ def batch_roofline(fast_st, cells, stage_ms, calls, cg_cells):
     achieved = algo / (unit_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES.get("line_cg_iteration_mixed"),
-                "kernel": f"k_lcg_a_f32_c{LAUNCH_TUNING['cells_per_thread']} + "
-                          f"k_lcg_b_f32_u{LAUNCH_TUNING['column_rows']}", "stage": "cg_iterations",
+                "kernel": lcg_kernels(max_rows), "stage": "cg_iterations",
                 "algorithmic_bytes_per_launch": algo, "launch_ms": unit_ms, "launch_pairs": pairs,
                 "bytes_per_cell": tm["line_cg_iteration_mixed"], "cells": cells, "iterating_cells": active,
                 "note": "one line-CG iteration of the whole batch (launch A + launch B), HIP events on the "
@@ -317,7 +333,7 @@ def run_roofline(args):
                                     "time and the line-CG roofline (not the metric)",
                       "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "cells": cells,
                       "fast_path": settings_summary(fast_st), "launch_tuning": dict(LAUNCH_TUNING),
-                      "roofline": batch_roofline(fast_st, cells, stage_ms, calls, cg_cells),
+                      "roofline": batch_roofline(fast_st, cells, stage_ms, calls, cg_cells, batch_max_rows(W, rows)),
                       "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()}}), flush=True)
 
 
@@ -337,7 +353,7 @@ def run_headline(args):
                   settle_window=H["settle_window"], settle_cl=H["settle_cl"], settle_cd=H["settle_cd"])
     # the timed unit of the roofline: K outer iterations of the whole batch after W untimed
     cells, ms_per_step, stage_ms, calls, cg_cells = roofline_batch(G, W, rows, fast_st, args.steps, args.warmup)
-    roofline = batch_roofline(fast_st, cells, stage_ms, calls, cg_cells)
+    roofline = batch_roofline(fast_st, cells, stage_ms, calls, cg_cells, batch_max_rows(W, rows))
     caps = {"coarse": args.fast_cg, "medium": args.fast_cg_medium or args.fast_cg,
             "fine": args.fast_cg_fine or args.fast_cg}
     fast_st.pressure_iterations = max(caps.values())

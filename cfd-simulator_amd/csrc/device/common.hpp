@@ -200,10 +200,12 @@ __device__ __forceinline__ void wavePartials(const double* const (&p)[NS], int n
 }
 
 // Single-slot forms for wave 0; the result is in lane 0.
+// (UNROLL sets only how many loads are in flight per lane, not the order of the sum)
+template <int UNROLL = kPartialUnroll>
 __device__ __forceinline__ double waveSumPartials(const double* p, int n) {
   const double* const s[1] = {p};
   double r[1];
-  wavePartials<1>(s, n, 0.0, OpSum{}, r);
+  wavePartials<1, OpSum, UNROLL>(s, n, 0.0, OpSum{}, r);
   return r[0];
 }
 __device__ __forceinline__ double waveMaxPartials(const double* p, int n, double identity = 0.0) {

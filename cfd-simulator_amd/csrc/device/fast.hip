@@ -309,7 +309,7 @@ __device__ __forceinline__ void lcgA(const Inst* insts, int it, double* sm) {
   double sums[2];
   {
     const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
-    blockSumPartials<2>(slots, I.colBlocks, sm, sums);
+    blockSumPartials<2, 1>(slots, I.colBlocks, sm, sums);
   }
   const double nrm = sums[0], rzNew = sums[1];
   double beta = 0.0;
@@ -450,7 +450,7 @@ __device__ __forceinline__ void lcgATile(const Inst* insts, int it, double* sm, 
   double sums[2];
   {
     const double* const slots[2] = {partialSlot(I, sCgNorm), partialSlot(I, sCgRz)};
-    blockSumPartials<2>(slots, I.colBlocks, sm, sums);
+    blockSumPartials<2, 1>(slots, I.colBlocks, sm, sums);
   }
   const double nrm = sums[0], rzNew = sums[1];
   double beta = 0.0;
@@ -520,7 +520,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it, int cellsA) {
   if (it >= I.cgMax) return;
   if (I.flags[fCgDone]) return;
   const LineStore<T> S = lineStore<T>(I);
-  const double papLane = waveSumPartials(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
+  const double papLane = waveSumPartials<4>(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
   if (threadIdx.x == 0) bc[0] = papLane;
   __syncthreads();
   const double pap = bc[0];
@@ -625,8 +625,11 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
   const int L = (nJ + kSegWaves - 1) / kSegWaves;
   const int j0 = wv * L;
   const int n = max(0, min(nJ, j0 + L) - j0);  // this wave's rows
-  // every operand of this wave's rows, issued before the partial sums
-  T ap[kSegRows], rr[kSegRows], lo[kSegRows], iv[kSegRows], up[kSegRows];
+  // every operand of this wave's rows, issued before the partial sums. The
+  // back substitution's factor up_j = pN_j inv_j is formed here from the
+  // coupling above row j (lo of row j + 1, or pNTop for the wave's last row)
+  // as the start of the pass formed it, instead of being read.
+  T ap[kSegRows], rr[kSegRows], lo[kSegRows], iv[kSegRows];
 #pragma unroll
   for (int u = 0; u < kSegRows; ++u) {
     const int j = min(j0 + u, nJ - 1);
@@ -634,11 +637,11 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
     ap[u] = S.Ap[c];
     rr[u] = S.r[c];
     iv[u] = S.inv[c];
-    up[u] = S.up[c];
     lo[u] = S.pN[(j > 0 ? j - 1 : 0) * nI + ic];
   }
+  const T pNTop = S.pN[min(j0 + max(n, 1) - 1, nJ - 1) * nI + ic];
   if (wv == 0) {
-    const double papLane = waveSumPartials(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
+    const double papLane = waveSumPartials<4>(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
     if (lane == 0) bc[0] = papLane;
   }
   __syncthreads();
@@ -682,6 +685,9 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
     }
   }
   // back substitution: the segment's composed map z_bottom = C z_above + D
+  T up[kSegRows];
+#pragma unroll
+  for (int u = 0; u < kSegRows; ++u) up[u] = ((u + 1 < n) ? lo[u + 1] : pNTop) * iv[u];
   T Cm = T(1), Dm = T(0);
 #pragma unroll
   for (int u = kSegRows - 1; u >= 0; --u) {

@@ -765,6 +765,12 @@ LaunchCfg launchCfgFor(const Inst* insts, size_t count) {
   return cfg;
 }
 
+std::vector<Inst> largestFirst(const std::vector<Inst>& members) {
+  std::vector<Inst> out(members);
+  std::stable_sort(out.begin(), out.end(), [](const Inst& a, const Inst& b) { return a.N > b.N; });
+  return out;
+}
+
 bool sameSchedule(const Inst& b, const Inst& a) {
   return b.model == a.model && b.momSweeps == a.momSweeps &&
          b.nonOrthPasses == a.nonOrthPasses && b.gsOrder == a.gsOrder && b.precision == a.precision &&
@@ -1168,7 +1174,8 @@ void Batch::gather() {
       throw std::invalid_argument("a batched solver's closure, sweeps, CG cap, correctors, ordering, precision, "
                                   "pressure solver or anchoring changed after the batch was made");
   }
-  check(hipMemcpyAsync(dInsts_, host_.data(), sizeof(Inst) * host_.size(), hipMemcpyHostToDevice, stream_),
+  devOrder_ = largestFirst(host_);
+  check(hipMemcpyAsync(dInsts_, devOrder_.data(), sizeof(Inst) * devOrder_.size(), hipMemcpyHostToDevice, stream_),
         "upload batch descriptors");
   cfg_ = launchCfgFor(host_.data(), host_.size());  // (the launch tuning may have changed)
 }

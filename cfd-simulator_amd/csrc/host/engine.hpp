@@ -49,6 +49,10 @@ void launchIteration(const Inst* dInst, const LaunchCfg& cfg, const Inst& h, hip
                      StageRecorder* rec = nullptr);
 // The launch shape of a batch of instances (the widest member's grid of each kind).
 LaunchCfg launchCfgFor(const Inst* insts, size_t count);
+// The order a batch's descriptors are uploaded in (blockIdx.y): the largest
+// meshes first, ties in member order, so the hardware dispatches their blocks
+// first and the small members' blocks fill the end of each launch.
+std::vector<Inst> largestFirst(const std::vector<Inst>& members);
 // Process-wide launch tuning of the line CG (cfdgpu_set_launch_tuning); read
 // whenever a launch shape is formed, so a change takes effect at the next call
 // (a captured iteration graph is captured again).
@@ -241,7 +245,8 @@ class Batch {
   void gather();
   std::unique_ptr<StageClock> clock_;
   std::vector<Engine*> members_;
-  std::vector<Inst> host_;
+  std::vector<Inst> host_;     // in member order
+  std::vector<Inst> devOrder_;  // as uploaded: largest mesh first
   Inst* dInsts_{nullptr};
   hipStream_t stream_{nullptr};
   LaunchCfg cfg_{};

@@ -403,6 +403,11 @@ void Server::finish(Slot& s, int status, int iterations, double continuity) {
 void Server::launchChunk() {
   if (active_.empty()) return;
   if (rosterDirty_) {
+    // the largest meshes first, as a batch uploads them (largestFirst): the
+    // slots themselves are reordered, so host_[q] stays slot q's descriptor
+    std::stable_sort(active_.begin(), active_.end(), [](const std::unique_ptr<Slot>& a, const std::unique_ptr<Slot>& b) {
+      return a->engine->inst().N > b->engine->inst().N;
+    });
     host_.resize(active_.size());
     for (size_t q = 0; q < active_.size(); ++q) host_[q] = active_[q]->engine->refreshInst();
     cfg_ = launchCfgFor(host_.data(), host_.size());
