@@ -1244,8 +1244,10 @@ __global__ void __launch_bounds__(kBlock) k_corr_mean(const Inst* insts) {
 // the imposed-velocity test of the first still sees the uncorrected fluxes.
 // ---------------------------------------------------------------------------
 
+// (eight partials in flight per lane rather than sixteen: the kernel then
+// fits in 60 registers and eight waves per SIMD; the order of the sum is the same)
 __device__ __forceinline__ double corrMean(const Inst& I, double* sm) {
-  return I.hasPRef ? 0.0 : blockSumPartials(partialSlot(I, sCorrMean), I.nBlocks, sm) / static_cast<double>(I.N);
+  return I.hasPRef ? 0.0 : blockSumPartials<8>(partialSlot(I, sCorrMean), I.nBlocks, sm) / static_cast<double>(I.N);
 }
 
 // p, u and v of cell c (SimpleSolver.cpp:582-632).
