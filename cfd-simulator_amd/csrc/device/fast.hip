@@ -850,25 +850,37 @@ __global__ void __launch_bounds__(kBlock) k_refine_final(const Inst* insts) {
 // zero. The arrays are colour-packed (rbIndex): a launch reads its colour's
 // rows contiguously and the other colour's corrections at the neighbours.
 // The C-grid has an even nI, so a wake cell and its partner differ in colour.
+// Every operand is loaded without a branch around it (a missing neighbour is
+// read at the cell's own place and its term dropped by a select; a zero
+// diagonal's row computed and discarded), so a lane's loads are one round
+// trip. The sum is the one the branches would form: off starts at +0 and a
+// sum of floats is -0 only when both terms are, so adding +0 for a dropped
+// term leaves it unchanged.
 template <int NS>
 __device__ __forceinline__ void rbDefect(const float* const (&A)[NS][5], const float* const (&R)[NS],
                                          float* const (&D)[NS], int pk, const int (&nb)[4], const bool (&has)[4],
                                          bool first, float (&out)[NS]) {
+  float d[NS], a[NS][4], rhs[NS], dn[NS][4];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const float d = A[s][0][pk];
-    float delta = 0.0f;
-    if (fabsf(d) > 0.0f) {
-      float off = 0.0f;
-      if (!first) {
-        if (has[0]) off += A[s][1][pk] * D[s][nb[0]];
-        if (has[1]) off += A[s][4][pk] * D[s][nb[1]];
-        if (has[2]) off += A[s][2][pk] * D[s][nb[2]];
-        if (has[3]) off += A[s][3][pk] * D[s][nb[3]];
-      }
-      delta = (R[s][pk] - off) / d;
+    d[s] = A[s][0][pk];
+    rhs[s] = R[s][pk];
+    // the coefficients in cellFaces order: below, right, above, left
+    a[s][0] = A[s][1][pk];
+    a[s][1] = A[s][4][pk];
+    a[s][2] = A[s][2][pk];
+    a[s][3] = A[s][3][pk];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dn[s][k] = first ? 0.0f : D[s][nb[k]];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    float off = 0.0f;
+    if (!first) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) off += has[k] ? a[s][k] * dn[s][k] : 0.0f;
     }
-    out[s] = delta;
+    out[s] = (fabsf(d[s]) > 0.0f) ? (rhs[s] - off) / d[s] : 0.0f;
   }
 }
 
@@ -882,7 +894,7 @@ __device__ __forceinline__ void rbSolveCell(const float* const (&A)[NS][5], cons
                                             float (&out)[NS]) {
   const int nI = I.nI, nJ = I.nJ;
   // the four neighbours in cellFaces order: below (or the wake partner), right, above, left
-  int nb[4] = {0, 0, 0, 0};
+  int nb[4] = {pk, pk, pk, pk};
   bool has[4] = {false, false, false, false};
   if (!first) {
     if (j > 0) {
@@ -895,9 +907,12 @@ __device__ __forceinline__ void rbSolveCell(const float* const (&A)[NS][5], cons
         has[0] = true;
       }
     }
-    if (i < nI - 1) { nb[1] = rbIndex(i + 1, j, nI, nJ); has[1] = true; }
-    if (j < nJ - 1) { nb[2] = rbIndex(i, j + 1, nI, nJ); has[2] = true; }
-    if (i > 0) { nb[3] = rbIndex(i - 1, j, nI, nJ); has[3] = true; }
+    has[1] = i < nI - 1;
+    has[2] = j < nJ - 1;
+    has[3] = i > 0;
+    nb[1] = has[1] ? rbIndex(i + 1, j, nI, nJ) : pk;
+    nb[2] = has[2] ? rbIndex(i, j + 1, nI, nJ) : pk;
+    nb[3] = has[3] ? rbIndex(i - 1, j, nI, nJ) : pk;
   }
   rbDefect<NS>(A, R, D, pk, nb, has, first, out);
 #pragma unroll
