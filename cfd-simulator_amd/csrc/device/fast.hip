@@ -590,7 +590,7 @@ __device__ __forceinline__ void lcgB(const Inst* insts, int it, int cellsA) {
 
 // Launch B with every column's rows split over the waves of a block: one
 // block per 64-column tile, wave w owning rows [w L, w L + L) of its 64
-// columns, L = ceil(nJ / kSegWaves) <= kSegRows, all of them in registers
+// columns, L = kSegRows, all of them in registers
 // (loads along a row are contiguous across the wave's lanes). The forward
 // solve y_j = w_j / pivot_j, w_j = r_j - l_j y_{j-1}, is the affine recurrence
 // y_j = a_j y_{j-1} + b_j (a_j = -l_j inv_j, b_j = r_j inv_j): each wave
@@ -622,24 +622,29 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
   const bool col = i < nI;
   const int ic = col ? i : nI - 1;  // idle lanes read a real column and store nothing
   const LineStore<T> S = lineStore<T>(I);
-  const int L = (nJ + kSegWaves - 1) / kSegWaves;
-  const int j0 = wv * L;
-  const int n = max(0, min(nJ, j0 + L) - j0);  // this wave's rows
+  // segments of kSegRows rows: a column of nJ rows occupies the first
+  // ceil(nJ / kSegRows) waves, and the waves above its top row load nothing
+  // (their maps are the identity)
+  const int j0 = wv * kSegRows;
+  const int n = max(0, min(nJ, j0 + kSegRows) - j0);  // this wave's rows
   // every operand of this wave's rows, issued before the partial sums. The
   // back substitution's factor up_j = pN_j inv_j is formed here from the
   // coupling above row j (lo of row j + 1, or pNTop for the wave's last row)
   // as the start of the pass formed it, instead of being read.
-  T ap[kSegRows], rr[kSegRows], lo[kSegRows], iv[kSegRows];
+  T ap[kSegRows] = {}, rr[kSegRows] = {}, lo[kSegRows] = {}, iv[kSegRows] = {};
+  T pNTop = T(0);
+  if (n > 0) {
 #pragma unroll
-  for (int u = 0; u < kSegRows; ++u) {
-    const int j = min(j0 + u, nJ - 1);
-    const int c = j * nI + ic;
-    ap[u] = S.Ap[c];
-    rr[u] = S.r[c];
-    iv[u] = S.inv[c];
-    lo[u] = S.pN[(j > 0 ? j - 1 : 0) * nI + ic];
+    for (int u = 0; u < kSegRows; ++u) {
+      const int j = min(j0 + u, nJ - 1);
+      const int c = j * nI + ic;
+      ap[u] = S.Ap[c];
+      rr[u] = S.r[c];
+      iv[u] = S.inv[c];
+      lo[u] = S.pN[(j > 0 ? j - 1 : 0) * nI + ic];
+    }
+    pNTop = S.pN[(j0 + n - 1) * nI + ic];
   }
-  const T pNTop = S.pN[min(j0 + max(n, 1) - 1, nJ - 1) * nI + ic];
   if (wv == 0) {
     const double papLane = waveSumPartials<4>(partialSlot(I, sCgPAp), blocksA(I.nI, I.nJ, cellsA));
     if (lane == 0) bc[0] = papLane;
@@ -738,19 +743,37 @@ __device__ __forceinline__ void lcgBSeg(const Inst* insts, int it, int cellsA) {
 // hand-over of x to the FP64 correction the velocity and flux updates read.
 template <typename T>
 __device__ __forceinline__ void lcgFinal(const Inst* insts) {
-  FAST_CELL_PROLOGUE
-  (void)i; (void)j; (void)nI; (void)nJ;
-  if (!live) return;
+  const Inst& I = insts[blockIdx.y];
+  const int nq = quadBlocks(I.nBlocks);
+  if (static_cast<int>(blockIdx.x) >= nq) return;
+  if (I.flags[fStopped]) return;
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), nq);
   const int n = I.flags[fCgIters];
   const bool capped = n == I.cgMax && n != 0;
   const LineStore<T> S = lineStore<T>(I);
   const CFD_GLOBAL T* p = ((n - 1) & 1) ? S.pd1 : S.pd0;
-  if constexpr (sizeof(T) == sizeof(double)) {
-    if (capped) gmem(I.x)[c] += I.red[sCgAlpha] * static_cast<double>(p[c]);
-  } else {
-    // the FP32 accumulation handed over in FP64, with the last step in FP64
-    const double last = capped ? I.red[sCgAlpha] * static_cast<double>(p[c]) : 0.0;
-    gmem(I.x)[c] = static_cast<double>(S.x[c]) + last;
+  const double alpha = I.red[sCgAlpha];
+  CFD_GLOBAL double* x = gmem(I.x);
+  // four cells per thread (quadBlocks), every load issued before the stores
+  T pv[kQuad], xv[kQuad];
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const int c = min((tile * kQuad + q) * kBlock + static_cast<int>(threadIdx.x), I.N - 1);
+    pv[q] = p[c];
+    xv[q] = S.x[c];
+  }
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const int c = (tile * kQuad + q) * kBlock + static_cast<int>(threadIdx.x);
+    if (c >= I.N) continue;
+    if constexpr (sizeof(T) == sizeof(double)) {
+      // (S.x is the FP64 correction itself)
+      if (capped) x[c] = static_cast<double>(xv[q]) + alpha * static_cast<double>(pv[q]);
+    } else {
+      // the FP32 accumulation handed over in FP64, with the last step in FP64
+      const double last = capped ? alpha * static_cast<double>(pv[q]) : 0.0;
+      x[c] = static_cast<double>(xv[q]) + last;
+    }
   }
 }
 

@@ -70,6 +70,7 @@ size_t gsLdsFor(int nJ, int nI, int threads) {
 
 void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStream_t s, StageRecorder* rec) {
   const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
+  const dim3 quadGrid(static_cast<unsigned>(quadBlocks(cfg.maxBlocks)), static_cast<unsigned>(cfg.batch));
   const dim3 block(kBlock);
   const dim3 one((cfg.batch + 63) / 64);
   const bool lex = h.gsOrder == 0;
@@ -158,7 +159,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
           hipLaunchKernelGGL(kb, colGrid, colB, 0, s, d, it, cells);
         }
         end(CFDGPU_STAGE_CG_ITERATIONS);
-        hipLaunchKernelGGL(mixed ? k_lcg_final_f32 : k_lcg_final, grid, block, 0, s, d);
+        hipLaunchKernelGGL(mixed ? k_lcg_final_f32 : k_lcg_final, quadGrid, block, 0, s, d);
       } else if (mixed) {
         hipLaunchKernelGGL(k_cg_init_f32, cgGrid, block, 0, s, d, rp);
         begin(CFDGPU_STAGE_CG_ITERATIONS);
@@ -192,7 +193,7 @@ void launchIteration(const Inst* d, const LaunchCfg& cfg, const Inst& h, hipStre
     end(CFDGPU_STAGE_CORRECTION);
   }
   begin(CFDGPU_STAGE_CORRECTION);
-  hipLaunchKernelGGL(k_divergence, grid, block, 0, s, d, static_cast<double*>(nullptr));
+  hipLaunchKernelGGL(k_divergence, quadGrid, block, 0, s, d, static_cast<double*>(nullptr));
   end(CFDGPU_STAGE_CORRECTION);
   if (h.model == 2) {
     begin(CFDGPU_STAGE_SST_BLENDING);
@@ -249,7 +250,7 @@ void launchWallDistance(const double2* centroid, int n, const double2* walls, in
 }
 
 void launchDivergence(const Inst* d, const LaunchCfg& cfg, double* out, hipStream_t s) {
-  const dim3 grid(static_cast<unsigned>(cfg.maxBlocks), static_cast<unsigned>(cfg.batch));
+  const dim3 grid(static_cast<unsigned>(quadBlocks(cfg.maxBlocks)), static_cast<unsigned>(cfg.batch));
   hipLaunchKernelGGL(k_divergence, grid, dim3(kBlock), 0, s, d, out);
 }
 

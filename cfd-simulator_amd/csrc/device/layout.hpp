@@ -176,6 +176,11 @@ __host__ __device__ inline int rbBlocks(int nI, int nJ) {
 }
 constexpr int kCgCells = 2;          // cells per thread in the CG launches (1, 2, 4, 8 measured: 2 is fastest)
 constexpr int kColThreads = 64;      // columns per block of a column-parallel launch (one wave)
+// Launches with four cells per thread (cells (t kQuad + q) kBlock + threadIdx.x
+// of tile t, q = 0 .. kQuad - 1): kernels with few loads per cell, whose
+// waves would otherwise wait on one memory round trip each
+constexpr int kQuad = 4;
+__host__ __device__ constexpr int quadBlocks(int nBlocks) { return (nBlocks + kQuad - 1) / kQuad; }
 constexpr int kLcgSegThreads = 1024; // the line CG's segmented launch B: 16 waves over one 64-column tile
 constexpr int kLcgSegMaxRows = 256;  // ... each wave holding at most 16 rows
 constexpr int kNoAllocMaxCells = 1 << 20;  // cells of cfdgpu_divergence_noalloc's staging array (8 MiB)

@@ -1377,17 +1377,39 @@ __device__ __forceinline__ double divergenceAt(const Inst& I, int c, int i, int 
   return (a > 0.0) ? net / a : 0.0;
 }
 
+// Four cells per thread (quadBlocks): the block's max goes to the first of
+// its tiles' partial slots and zero, the identity of a max of |d|, to the
+// others, so the monitor's reduction over nBlocks slots is unchanged.
 __global__ void __launch_bounds__(kBlock) k_divergence(const Inst* insts, double* out) {
-  CFD_PROLOGUE
+  const Inst& I = insts[blockIdx.y];
+  const int nq = quadBlocks(I.nBlocks);
+  if (static_cast<int>(blockIdx.x) >= nq) return;
+  if (I.flags[fStopped]) return;
+  const int tile = xcdTile(static_cast<int>(blockIdx.x), nq);
   __shared__ double sm[16];
+  const int nI = I.nI;
+  double d[kQuad];
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const int c = (tile * kQuad + q) * kBlock + static_cast<int>(threadIdx.x);
+    d[q] = (c < I.N) ? divergenceAt(I, c, c % nI, c / nI) : 0.0;
+  }
   double part[1] = {0.0};
-  if (live) {
-    const double d = divergenceAt(I, c, i, j);
-    if (out) out[c] = d;
-    part[0] = mx(0.0, fabs(d));
+#pragma unroll
+  for (int q = 0; q < kQuad; ++q) {
+    const int c = (tile * kQuad + q) * kBlock + static_cast<int>(threadIdx.x);
+    if (c < I.N) {
+      if (out) out[c] = d[q];
+      part[0] = mx(part[0], fabs(d[q]));
+    }
   }
   blockMax<1>(part, sm);
-  if (threadIdx.x == 0) partialSlot(I, sMaxDiv)[blockIdx.x] = part[0];
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < kQuad; ++q) {
+      const int slot = tile * kQuad + q;
+      if (slot < I.nBlocks) partialSlot(I, sMaxDiv)[slot] = (q == 0) ? part[0] : 0.0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
