@@ -51,9 +51,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 # x 2.00, WRITE_SIZE x 1.00): the plain path's single 1024x256 solve
 # (profiles/naca2412_1024x256_sst_traffic.md) and the fast path's line-CG
 # iteration over the 64-instance batch with every member iterating (launch A
-# 290.9 MB + launch B 227.3 MB; profiles/mixed_batch_64_fast_kernels.md).
+# 299.3 MB + launch B 194.8 MB; profiles/mixed_batch_64_fast_kernels.md).
 TRAFFIC_BYTES = {"k_gs_wave_mom": 142.32e6, "k_gs_wave_turb": 135.59e6, "k_cg_a + k_cg_b": 32.08e6,
-                 "line_cg_iteration_mixed": 518.2e6}
+                 "line_cg_iteration_mixed": 494.1e6}
 
 WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
                 first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
@@ -75,9 +75,9 @@ def traffic_model():
         # bytes per cell of one line-CG iteration on the mixed path (FP32 storage throughout,
         # the correction x included): launch A reads z, p_old, aP, aN, aE and x (4 B each) and
         # writes x, p, Ap = 36 (x += alpha p of the previous iteration rides in launch A);
-        # launch B (rows in registers) reads Ap, r, 1/pivot, the eliminated coupling and aN
-        # and writes r, z = 28
-        "line_cg_iteration_mixed": 64,
+        # launch B (rows in registers) reads Ap, r, 1/pivot and aN (the back substitution's
+        # factor aN / pivot formed from the two) and writes r, z = 24
+        "line_cg_iteration_mixed": 60,
     }
 
 
