@@ -28,11 +28,22 @@ struct FaceGeo {
   double w;       // owner interpolation weight
 };
 
+// The descriptor's device arrays: in device code they point into the global
+// address space, so every access through them is a GLOBAL instruction. A
+// FLAT (generic) access also counts against the scalar-and-LDS wait counter,
+// so every wait for a descriptor field or an LDS result would also wait for
+// all loads in flight. Host code sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CFD_G __attribute__((address_space(1)))
+#else
+#define CFD_G
+#endif
+
 // In device memory each FaceGeo field is an array of its own, so a kernel
 // reads only the fields it uses (the SST assembly only w, the gradients S and
 // w) and every field load is contiguous across a wave.
 struct FaceGeoArrays {
-  const double *Sx, *Sy, *Tx, *Ty, *g, *w;
+  CFD_G const double *Sx, *Sy, *Tx, *Ty, *g, *w;
 };
 __host__ __device__ __forceinline__ FaceGeo geo(const FaceGeoArrays& a, int f) {
   return FaceGeo{a.Sx[f], a.Sy[f], a.Tx[f], a.Ty[f], a.g[f], a.w[f]};
@@ -219,31 +230,31 @@ struct Inst {
   int colBlocks;   // blocks of a column-parallel launch (kColThreads columns each)
 
   // geometry (read-only)
-  const double* V;
+  CFD_G const double* V;
   FaceGeoArrays jg;  // j-faces
   FaceGeoArrays ig;  // i-faces
-  const int* partner;  // nI entries: partner cell in row 0, or -1
-  const FaceBC* bcB;   // nI
-  const FaceBC* bcT;   // nI
-  const FaceBC* bcL;   // nJ
-  const FaceBC* bcR;   // nJ
-  const double* wallDist;
-  const double* jgSst;  // |g| for SST on j-faces (interior and wake)
-  const double* igSst;
+  CFD_G const int* partner;  // nI entries: partner cell in row 0, or -1
+  CFD_G const FaceBC* bcB;   // nI
+  CFD_G const FaceBC* bcT;   // nI
+  CFD_G const FaceBC* bcL;   // nJ
+  CFD_G const FaceBC* bcR;   // nJ
+  CFD_G const double* wallDist;
+  CFD_G const double* jgSst;  // |g| for SST on j-faces (interior and wake)
+  CFD_G const double* igSst;
 
   // flow state (FP64)
-  double *u, *v, *p, *rho, *mu, *muMol;
-  double *Fj, *Fi;
+  CFD_G double *u, *v, *p, *rho, *mu, *muMol;
+  CFD_G double *Fj, *Fi;
   double2 *gU, *gV, *gP;
 
   // momentum system (cell-centric rows; aP is the relaxed diagonal = momentumDiagonal_)
-  double *aP, *aS, *aN, *aW, *aE, *bx, *by;
+  CFD_G double *aP, *aS, *aN, *aW, *aE, *bx, *by;
   // pressure system: diagonal, coefficient of the above / right face, wake coefficient
-  double *pP, *pN, *pE, *pWk, *pb;
+  CFD_G double *pP, *pN, *pE, *pWk, *pb;
   // CG vectors
-  double *x, *r, *z, *pd0, *pd1, *Ap;
+  CFD_G double *x, *r, *z, *pd0, *pd1, *Ap;
   // ping-pong buffers of the two-colour sweep's callers (unused by the lexicographic one)
-  double *gsTmp0, *gsTmp1;
+  CFD_G double *gsTmp0, *gsTmp1;
 
   // Lexicographic sweep in the anti-diagonal ("skewed") layout: cell (i, j) is
   // stored at skewIndex(i, j, gsR), so the cells of one wavefront step are
@@ -257,16 +268,16 @@ struct Inst {
   // b_y in its own array. Unknowns in pairs too: (u, v) for momentum, (phi, -)
   // for k and omega; x[0] holds the starting values, x[1] the ping-pong copy.
   double2* skMomRow[3];
-  double* skMomBy;
+  CFD_G double* skMomBy;
   double2* skMomX[2];
   double2* skTurbRow[2][3];  // [k, omega][pair]
   double2* skTurbX[2][2];    // [k, omega][copy]
 
   // k-omega SST state and systems
-  double *k, *w, *mut, *f1, *f2, *cd, *strain, *prod;
+  CFD_G double *k, *w, *mut, *f1, *f2, *cd, *strain, *prod;
   double2 *gK, *gW;
-  double *kP, *kS, *kN, *kW, *kE, *kB;
-  double *wP, *wS, *wN, *wW, *wE, *wB;
+  CFD_G double *kP, *kS, *kN, *kW, *kE, *kB;
+  CFD_G double *wP, *wS, *wN, *wW, *wE, *wB;
   double kIn, wIn;
   double sigK1, sigW1, beta1, sigK2, sigW2, beta2, betaStar, kappa, a1, prodLimit;
   double gamma1, gamma2, betaStarQ;  // gamma1(), gamma2(), betaStar^(1/4) (host-computed)
@@ -274,36 +285,36 @@ struct Inst {
   // mixed precision: the pressure system and the CG vectors stored in FP32
   // (the Jacobi CG's arithmetic, its reductions and x stay FP64; the line CG
   // accumulates x in FP32, fx, and hands it over in FP64 at the end)
-  float *fpP, *fpN, *fpE, *fpWk;
-  float *fr, *fz, *fpd0, *fpd1, *fAp, *fx;
+  CFD_G float *fpP, *fpN, *fpE, *fpWk;
+  CFD_G float *fr, *fz, *fpd0, *fpd1, *fAp, *fx;
 
   // Line-preconditioned CG: 1 / pivot of each column's tridiagonal factor
   // (Thomas) and the eliminated upper coefficient (pN / pivot), in the CG's
   // storage type; the refinement's FP64 residual right-hand side and
   // accumulated correction.
-  double *lInv, *lUp;
-  float *flInv, *flUp;
-  double *rb, *xacc;
+  CFD_G double *lInv, *lUp;
+  CFD_G float *flInv, *flUp;
+  CFD_G double *rb, *xacc;
 
   // Mixed path with two-colour sweeps: every swept system in FP32 as
   // coefficients (aP, aS, aN, aW, aE), the FP64 defect b - A phi rounded to
   // FP32, and the FP32 correction the sweeps solve for (phi += delta after
   // the last sweep, in FP64). Momentum: u and v share the coefficients.
   // Cell (i, j) of these arrays is at rbIndex(i, j, nI, nJ).
-  float* fMom[5];
-  float* fMomRes[2];
-  float* fMomDelta[2];
-  float* fTurb[2][5];     // [k, omega][coefficient]
-  float* fTurbRes[2];
-  float* fTurbDelta[2];
+  CFD_G float* fMom[5];
+  CFD_G float* fMomRes[2];
+  CFD_G float* fMomDelta[2];
+  CFD_G float* fTurb[2][5];     // [k, omega][coefficient]
+  CFD_G float* fTurbRes[2];
+  CFD_G float* fTurbDelta[2];
 
   // reductions
-  double* red;        // kSlots scalars
-  double* partials;   // kSlots * kMaxRedBlocks
-  unsigned* counter;  // one arrival counter per slot group
-  int* flags;         // kIFlags
-  DevMonitor* monitor;
-  DevMonitor* history;  // optional ring of monitors, one per outer iteration
+  CFD_G double* red;        // kSlots scalars
+  CFD_G double* partials;   // kSlots * kMaxRedBlocks
+  CFD_G unsigned* counter;  // one arrival counter per slot group
+  CFD_G int* flags;         // kIFlags
+  CFD_G DevMonitor* monitor;
+  CFD_G DevMonitor* history;  // optional ring of monitors, one per outer iteration
   int historyCap;
 };
 
