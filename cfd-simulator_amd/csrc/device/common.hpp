@@ -142,6 +142,15 @@ __device__ __forceinline__ const CFD_GLOBAL T* gmem(const T* p) {
 typedef double dpair __attribute__((vector_size(16)));
 __device__ __forceinline__ const CFD_GLOBAL dpair* gpair(const double2* p) { return (const CFD_GLOBAL dpair*)(p); }
 __device__ __forceinline__ CFD_GLOBAL dpair* gpair(double2* p) { return (CFD_GLOBAL dpair*)(p); }
+// Element i of a double2 array, and a store to it, as one 16-byte GLOBAL access.
+__device__ __forceinline__ double2 ld2(const double2* p, int i) {
+  const dpair v = gpair(p)[i];
+  return make_double2(v[0], v[1]);
+}
+__device__ __forceinline__ void st2(double2* p, int i, double x, double y) {
+  const dpair v = {x, y};
+  gpair(p)[i] = v;
+}
 
 // Cross-block scalars: each kernel writes one partial per block and the
 // kernel that needs the total sums the partials itself, in a fixed order, at

@@ -105,9 +105,9 @@ __global__ void __launch_bounds__(kBlock) k_grad(const Inst* insts) {
       const double inv = 1.0 / vol;
       ux *= inv; uy *= inv; vx *= inv; vy *= inv; px *= inv; py *= inv;
     }
-    I.gU[c] = make_double2(ux, uy);
-    I.gV[c] = make_double2(vx, vy);
-    I.gP[c] = make_double2(px, py);
+    st2(I.gU, c, ux, uy);
+    st2(I.gV, c, vx, vy);
+    st2(I.gP, c, px, py);
     vmax[0] = mx(0.0, hypot(I.u[c], I.v[c]));
   }
   blockMax<1>(vmax, sm);
@@ -139,7 +139,7 @@ __device__ __forceinline__ double momInterior(const Inst& I, const FaceGeo& f, d
     row.d += inward + diff;
     off = -outward - diff;
   }
-  const double2 guo = I.gU[o], gun = I.gU[n], gvo = I.gV[o], gvn = I.gV[n];
+  const double2 guo = ld2(I.gU, o), gun = ld2(I.gU, n), gvo = ld2(I.gV, o), gvn = ld2(I.gV, n);
   const double fux = guo.x * w + gun.x * (1.0 - w), fuy = guo.y * w + gun.y * (1.0 - w);
   const double fvx = gvo.x * w + gvn.x * (1.0 - w), fvy = gvo.y * w + gvn.y * (1.0 - w);
   const double cu = I.blend * visc * (fux * f.Tx + fuy * f.Ty);
@@ -169,7 +169,7 @@ __device__ __forceinline__ double momWake(const Inst& I, const FaceGeo& f, doubl
   const double inward = mx(-F, 0.0);
   row.d += outward + diff;
   const double off = -inward - diff;
-  const double2 guo = I.gU[c], gun = I.gU[a], gvo = I.gV[c], gvn = I.gV[a];
+  const double2 guo = ld2(I.gU, c), gun = ld2(I.gU, a), gvo = ld2(I.gV, c), gvn = ld2(I.gV, a);
   const double fux = guo.x * w + gun.x * (1.0 - w), fuy = guo.y * w + gun.y * (1.0 - w);
   const double fvx = gvo.x * w + gvn.x * (1.0 - w), fvy = gvo.y * w + gvn.y * (1.0 - w);
   row.bx += I.blend * visc * (fux * f.Tx + fuy * f.Ty);
@@ -182,7 +182,7 @@ __device__ __forceinline__ void momBoundary(const Inst& I, const FaceGeo& f, con
   const double visc = I.mu[c];
   const double diff = visc * f.g;
   if (imposes(bc, F)) {
-    const double2 gu = I.gU[c], gv = I.gV[c];
+    const double2 gu = ld2(I.gU, c), gv = ld2(I.gV, c);
     row.bx -= F * bc.u;
     row.by -= F * bc.v;
     row.d += diff;
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_mom(const Inst* insts) {
       if (i < nI - 1) aE = momInterior(I, f, F, c, c + 1, true, row);
       else momBoundary(I, f, I.bcR[j], F, c, row);
     }
-    const double2 gp = I.gP[c];
+    const double2 gp = ld2(I.gP, c);
     const double vol = I.V[c];
     row.bx -= gp.x * vol;
     row.by -= gp.y * vol;
@@ -736,7 +736,7 @@ __device__ __forceinline__ double rcInterior(const Inst& I, const FaceGeo& f, in
   const double uy = I.v[o] * w + I.v[n] * (1.0 - w);
   const double dens = w * I.rho[o] + (1.0 - w) * I.rho[n];
   const double mob = w * (I.V[o] / I.aP[o]) + (1.0 - w) * (I.V[n] / I.aP[n]);
-  const double2 go = I.gP[o], gn = I.gP[n];
+  const double2 go = ld2(I.gP, o), gn = ld2(I.gP, n);
   const double sgx = go.x * w + gn.x * (1.0 - w);
   const double sgy = go.y * w + gn.y * (1.0 - w);
   const double compact = (I.p[n] - I.p[o]) * f.g;
@@ -749,7 +749,7 @@ __device__ __forceinline__ double rcBoundary(const Inst& I, const FaceGeo& f, co
   if (imposes(bc, Fold)) return dens * (bc.u * f.Sx + bc.v * f.Sy);
   const double mob = I.V[o] / I.aP[o];
   const double compact = (bc.p - I.p[o]) * f.g;
-  const double2 g = I.gP[o];
+  const double2 g = ld2(I.gP, o);
   const double smooth = g.x * f.Sx + g.y * f.Sy;
   return dens * (I.u[o] * f.Sx + I.v[o] * f.Sy) - dens * mob * (compact - smooth);
 }

@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_blend(const Inst* insts) {
   SST_PROLOGUE
   if (!live) return;
   // strain from the gradients of the start of the iteration (KOmegaSST.cpp:202-216)
-  const double2 gu = I.gU[c], gv = I.gV[c];
+  const double2 gu = ld2(I.gU, c), gv = ld2(I.gV, c);
   const double s11 = gu.x, s22 = gv.y, s12 = 0.5 * (gu.y + gv.x);
   const double S = sqrt(2.0 * (s11 * s11 + s22 * s22 + 2.0 * s12 * s12));
   I.strain[c] = S;
@@ -103,8 +103,8 @@ __global__ void __launch_bounds__(kBlock) k_sst_blend(const Inst* insts) {
     const double inv = 1.0 / vol;
     kx *= inv; ky *= inv; wx *= inv; wy *= inv;
   }
-  I.gK[c] = make_double2(kx, ky);
-  I.gW[c] = make_double2(wx, wy);
+  st2(I.gK, c, kx, ky);
+  st2(I.gW, c, wx, wy);
 
   // blending functions (KOmegaSST.cpp:247-286)
   const double d = mx(I.wallDist[c], kTinySst);
