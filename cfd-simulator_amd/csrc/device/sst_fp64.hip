@@ -135,91 +135,119 @@ struct SstRow {
   double d, b, aS, aN, aW, aE;
 };
 
-__device__ __forceinline__ double sigmaK(const Inst& I, int q) { return I.f1[q] * I.sigK1 + (1.0 - I.f1[q]) * I.sigK2; }
-__device__ __forceinline__ double sigmaW(const Inst& I, int q) { return I.f1[q] * I.sigW1 + (1.0 - I.f1[q]) * I.sigW2; }
-
-// KOmegaSST.cpp:384-444 for both equations at once. isK selects sigma.
+// The diffusivity mu + sigma(F1) mu_t of the k (isK) or omega row at a cell
+// (KOmegaSST.cpp:384-444).
 template <bool isK>
-__device__ __forceinline__ double sstDiffusivity(const Inst& I, int q) {
-  return I.muMol[q] + (isK ? sigmaK(I, q) : sigmaW(I, q)) * I.mut[q];
+__device__ __forceinline__ double sstGamma(const Inst& I, double muMol, double f1, double mut) {
+  return muMol + (isK ? f1 * I.sigK1 + (1.0 - f1) * I.sigK2 : f1 * I.sigW1 + (1.0 - f1) * I.sigW2) * mut;
 }
 
-template <bool isK>
-__device__ __forceinline__ double sstInteriorOff(const Inst& I, double w, double gs, double F, int o, int n,
-                                                 bool cellIsOwner, double& d) {
-  const double diffusivity = w * sstDiffusivity<isK>(I, o) + (1.0 - w) * sstDiffusivity<isK>(I, n);
-  const double diff = diffusivity * gs;
+// One face of a row (KOmegaSST.cpp:384-444). An interior face (owner value
+// gO, neighbour value gN of the diffusivity, owner weight w) adds its
+// outflow, or inflow, and its diffusion to the diagonal and gives the
+// off-diagonal coefficient; a boundary face (the cell's diffusivity gC)
+// adds to the diagonal and the source, its value imposed on a wall or
+// inlet face and on an inflowing far-field face. Both are formed and a
+// select keeps one, so no load waits on the face's kind.
+__device__ __forceinline__ double sstFace(bool interior, bool owner, double w, double gO, double gN, double gs,
+                                          double F, double bcG, int bcKind, double gC, double wallValue,
+                                          double inflow, double& d, double& b) {
+  const double diff = (w * gO + (1.0 - w) * gN) * gs;
   const double outward = mx(F, 0.0), inward = mx(-F, 0.0);
-  if (cellIsOwner) {
-    d += outward + diff;
-    return -inward - diff;
-  }
-  d += inward + diff;
-  return -outward - diff;
+  const double dI = owner ? outward + diff : inward + diff;
+  const double aI = owner ? -inward - diff : -outward - diff;
+  const double diffB = gC * bcG;
+  const bool imposed = bcKind == kWall || bcKind == kInlet || (bcKind == kFarField && F < 0.0);
+  const double value = (bcKind == kWall) ? wallValue : inflow;
+  const double addD = imposed ? diffB + mx(F, 0.0) : mx(F, 0.0);
+  const double addB = imposed ? (diffB - mn(F, 0.0)) * value : 0.0;
+  d += interior ? dI : addD;
+  // +0 on an interior face leaves b unchanged: a sum of doubles is -0 only
+  // when both terms are, so b, which starts at +0, is never -0
+  b += interior ? 0.0 : addB;
+  return interior ? aI : 0.0;
 }
 
-template <bool isK>
-__device__ __forceinline__ void sstBoundaryRow(const Inst& I, const FaceBC& bc, double F, int o, double wallValue,
-                                               double inflow, double& d, double& b) {
-  const double diff = sstDiffusivity<isK>(I, o) * bc.gSst;
-  const bool imposed = bc.kind == kWall || bc.kind == kInlet || (bc.kind == kFarField && F < 0.0);
-  const double value = (bc.kind == kWall) ? wallValue : inflow;
-  // both sums updated on every path (b gains +0 where nothing is imposed:
-  // b is never -0, so the sum is unchanged), which keeps d and b in registers
-  const double addD = imposed ? diff + mx(F, 0.0) : mx(F, 0.0);
-  const double addB = imposed ? (diff - mn(F, 0.0)) * value : 0.0;
-  d += addD;
-  b += addB;
-}
+// The stencil of cell c: its four faces in cellFaces order (below or the
+// wake partner, above, left, right; Mesh.cpp:206-213), the neighbour across
+// each (the cell itself where there is none) and what the faces carry.
+struct SstStencil {
+  int nb[4];
+  bool interior[4];
+  bool sOwner;  // row 0: the cell owns its face below (a wake face)
+  double w[4], gs[4], F[4], bcG[4];
+  int bcKind[4];
+};
 
-template <bool isK>
-__device__ __forceinline__ SstRow sstRow(const Inst& I, int c, int i, int j, double wallValue, double inflow) {
+__device__ __forceinline__ SstStencil sstStencil(const Inst& I, int c, int i, int j) {
   const int nI = I.nI, nJ = I.nJ;
+  SstStencil t;
+  // the wake partner of a row-0 cell: the only operand whose load waits on another
+  const int pr = (j == 0) ? I.partner[i] : -1;
+  const int fS = j * nI + i, fN = (j + 1) * nI + i, fW = j * (nI + 1) + i, fE = fW + 1;
+  t.nb[0] = (j > 0) ? c - nI : ((pr >= 0) ? pr : c);
+  t.nb[1] = (j < nJ - 1) ? c + nI : c;
+  t.nb[2] = (i > 0) ? c - 1 : c;
+  t.nb[3] = (i < nI - 1) ? c + 1 : c;
+  t.sOwner = j == 0;
+  t.interior[0] = j > 0 || pr >= 0;
+  t.interior[1] = j < nJ - 1;
+  t.interior[2] = i > 0;
+  t.interior[3] = i < nI - 1;
+  // a wake face weights the cell itself fully (owner c, neighbour the partner)
+  t.w[0] = (j > 0) ? I.jg.w[fS] : 1.0;
+  t.w[1] = I.jg.w[fN];
+  t.w[2] = I.ig.w[fW];
+  t.w[3] = I.ig.w[fE];
+  t.gs[0] = I.jgSst[fS];
+  t.gs[1] = I.jgSst[fN];
+  t.gs[2] = I.igSst[fW];
+  t.gs[3] = I.igSst[fE];
+  t.F[0] = I.Fj[fS];
+  t.F[1] = I.Fj[fN];
+  t.F[2] = I.Fi[fW];
+  t.F[3] = I.Fi[fE];
+  const FaceBC* const bcs[4] = {&I.bcB[i], &I.bcT[i], &I.bcL[j], &I.bcR[j]};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    t.bcG[q] = bcs[q]->gSst;
+    t.bcKind[q] = bcs[q]->kind;
+  }
+  return t;
+}
+
+// The k (isK) or omega row of cell c from its stencil and the diffusivities
+// of the cell (g[4]) and of its neighbours (g[0..3], in the stencil's order).
+template <bool isK>
+__device__ __forceinline__ SstRow sstRow(const SstStencil& t, const double (&g)[5], double wallValue,
+                                         double inflow) {
   SstRow r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  {
-    const int fi = j * nI + i;
-    const double F = I.Fj[fi];
-    if (j > 0) r.aS = sstInteriorOff<isK>(I, I.jg.w[fi], I.jgSst[fi], F, c - nI, c, false, r.d);
-    else {
-      const int pr = I.partner[i];
-      if (pr >= 0) r.aS = sstInteriorOff<isK>(I, 1.0, I.jgSst[fi], F, c, pr, true, r.d);
-      else sstBoundaryRow<isK>(I, I.bcB[i], F, c, wallValue, inflow, r.d, r.b);
-    }
-  }
-  {
-    const int fi = (j + 1) * nI + i;
-    const double F = I.Fj[fi];
-    if (j < nJ - 1) r.aN = sstInteriorOff<isK>(I, I.jg.w[fi], I.jgSst[fi], F, c, c + nI, true, r.d);
-    else sstBoundaryRow<isK>(I, I.bcT[i], F, c, wallValue, inflow, r.d, r.b);
-  }
-  {
-    const int fi = j * (nI + 1) + i;
-    const double F = I.Fi[fi];
-    if (i > 0) r.aW = sstInteriorOff<isK>(I, I.ig.w[fi], I.igSst[fi], F, c - 1, c, false, r.d);
-    else sstBoundaryRow<isK>(I, I.bcL[j], F, c, wallValue, inflow, r.d, r.b);
-  }
-  {
-    const int fi = j * (nI + 1) + i + 1;
-    const double F = I.Fi[fi];
-    if (i < nI - 1) r.aE = sstInteriorOff<isK>(I, I.ig.w[fi], I.igSst[fi], F, c, c + 1, true, r.d);
-    else sstBoundaryRow<isK>(I, I.bcR[j], F, c, wallValue, inflow, r.d, r.b);
-  }
+  const double gC = g[4];
+  // below: the face's owner is the cell below (neighbour the cell), or on a
+  // wake face the cell itself (neighbour its partner)
+  r.aS = sstFace(t.interior[0], t.sOwner, t.w[0], t.sOwner ? gC : g[0], t.sOwner ? g[0] : gC, t.gs[0], t.F[0],
+                 t.bcG[0], t.bcKind[0], gC, wallValue, inflow, r.d, r.b);
+  r.aN = sstFace(t.interior[1], true, t.w[1], gC, g[1], t.gs[1], t.F[1], t.bcG[1], t.bcKind[1], gC, wallValue,
+                 inflow, r.d, r.b);
+  r.aW = sstFace(t.interior[2], false, t.w[2], g[2], gC, t.gs[2], t.F[2], t.bcG[2], t.bcKind[2], gC, wallValue,
+                 inflow, r.d, r.b);
+  r.aE = sstFace(t.interior[3], true, t.w[3], gC, g[3], t.gs[3], t.F[3], t.bcG[3], t.bcKind[3], gC, wallValue,
+                 inflow, r.d, r.b);
   return r;
 }
 
-// b - A phi of one row (the signed defect; its magnitude is the residual term)
-__device__ __forceinline__ double sstDefect(const Inst& I, const SstRow& r, const double* phi, int c, int i,
-                                            int j) {
-  const int nI = I.nI, nJ = I.nJ;
-  double ax = r.d * phi[c];
-  if (j > 0) ax += r.aS * phi[c - nI];
-  else {
-    const int pr = I.partner[i];
-    if (pr >= 0) ax += r.aS * phi[pr];
-  }
-  if (j < nJ - 1) ax += r.aN * phi[c + nI];
-  if (i > 0) ax += r.aW * phi[c - 1];
-  if (i < nI - 1) ax += r.aE * phi[c + 1];
+// b - A phi of one row (the signed defect; its magnitude is the residual
+// term), every neighbour read without a branch (a missing one at the cell
+// itself, its term dropped by a select: ax may then differ from the
+// branching sum only in the sign of a zero, which b - ax does not see, b
+// never being -0)
+__device__ __forceinline__ double sstDefect(const SstStencil& t, const SstRow& r, const CFD_G double* phi, int c) {
+  const double pc = phi[c], p0 = phi[t.nb[0]], p1 = phi[t.nb[1]], p2 = phi[t.nb[2]], p3 = phi[t.nb[3]];
+  double ax = r.d * pc;
+  ax += t.interior[0] ? r.aS * p0 : 0.0;
+  ax += t.interior[1] ? r.aN * p1 : 0.0;
+  ax += t.interior[2] ? r.aW * p2 : 0.0;
+  ax += t.interior[3] ? r.aE * p3 : 0.0;
   return r.b - ax;
 }
 
@@ -246,6 +274,25 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
   __shared__ double sm[16];
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   if (live) {
+    const SstStencil t = sstStencil(I, c, i, j);
+    // the diffusivities of the neighbours and of the cell, their operands
+    // loaded once for both rows
+    double gk[5], gw[5];
+    {
+      const int q5[5] = {t.nb[0], t.nb[1], t.nb[2], t.nb[3], c};
+      double mu[5], fo[5], mt[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        mu[q] = I.muMol[q5[q]];
+        fo[q] = I.f1[q5[q]];
+        mt[q] = I.mut[q5[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        gk[q] = sstGamma<true>(I, mu[q], fo[q], mt[q]);
+        gw[q] = sstGamma<false>(I, mu[q], fo[q], mt[q]);
+      }
+    }
     const double vol = I.V[c];
     const double rho = I.rho[c];
     const double f1 = I.f1[c];
@@ -253,7 +300,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
     const double relax = clampd(I.alphaT, 0.05, 1.0);
     // k equation
     {
-      SstRow r = sstRow<true>(I, c, i, j, 0.0, I.kIn);
+      SstRow r = sstRow<true>(t, gk, 0.0, I.kIn);
       const double srcE = I.prod[c];
       const double srcI = I.betaStar * rho * mx(I.w[c], kMinOmega);
       r.b += srcE * vol;
@@ -271,14 +318,14 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
       } else {
         mixedTurbRow(I, 0, rbIndex(i, j, nI, nJ), r);
       }
-      const double defect = sstDefect(I, r, I.k, c, i, j);
+      const double defect = sstDefect(t, r, I.k, c);
       if (I.gsOrder == 1 && I.precision == 1) I.fTurbRes[0][rbIndex(i, j, nI, nJ)] = static_cast<float>(defect);
       part[0] = fabs(defect);
       part[1] = fabs(r.d);
     }
     // omega equation (wall value is omega_inflow: KOmegaSST.cpp:507-509)
     {
-      SstRow r = sstRow<false>(I, c, i, j, I.wIn, I.wIn);
+      SstRow r = sstRow<false>(t, gw, I.wIn, I.wIn);
       const double gamma = f1 * I.gamma1 + (1.0 - f1) * I.gamma2;
       const double beta = f1 * I.beta1 + (1.0 - f1) * I.beta2;
       double srcE = gamma * rho * S * S;
@@ -299,7 +346,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_asm(const Inst* insts) {
       } else {
         mixedTurbRow(I, 1, rbIndex(i, j, nI, nJ), r);
       }
-      const double defect = sstDefect(I, r, I.w, c, i, j);
+      const double defect = sstDefect(t, r, I.w, c);
       if (I.gsOrder == 1 && I.precision == 1) I.fTurbRes[1][rbIndex(i, j, nI, nJ)] = static_cast<float>(defect);
       part[2] = fabs(defect);
       part[3] = fabs(r.d);
