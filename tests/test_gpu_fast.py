@@ -88,6 +88,35 @@ def test_every_launch_shape_matches_the_definition(G, gpu_available, restore_tun
         assert err < 1e-3
 
 
+@pytest.mark.parametrize("rows", [16, 32], ids=lambda r: f"ahead{r}")
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_tall_mesh_line_cg_matches_the_definition(G, gpu_available, restore_tuning, precision, rows):
+    """Columns of more than 256 rows: the start of the pass and launch B walk
+    each column with one lane (16 or 32 rows loaded ahead) instead of
+    splitting it over the waves of a block."""
+    G.set_launch_tuning(0, rows)
+    mesh = G.cgrid(0, 0, 12, options=G.cgrid_options("coarse", **dict(SMALL, normal_points=300)))
+    print(f"case: {mesh.ni}x{mesh.nj} SST, line CG ({'fp64' if precision == 0 else 'fp32'}), {rows} rows ahead, "
+          "cap 500", flush=True)
+    assert mesh.nj > 256
+    kind, vel, pres = G.face_conditions(mesh, 1.0, 4.0)
+    st = G.default_settings(**dict(FAST, precision=precision, pressure_iterations=500))
+    s = G.SimpleSolver(mesh, kind, vel, pres, st)
+    s.set_turbulence_model(G.MODEL_SST)
+    s.initialise(*G.uniform_field(mesh.cells, speed=1.0, alpha_deg=4.0, density=1.0, reynolds=1e5))
+    mon = s.iterate(2)
+    A, dP, pN, b = _system(mesh, s)
+    x, its = L.line_cg(A, dP, pN, b, mesh.ni, mesh.nj, st.pressure_tolerance, st.pressure_iterations)
+    err = np.max(np.abs(s.debug_array("correction") - x)) / np.max(np.abs(x))
+    print(f"CG iterations {mon.pressure_iterations} (NumPy {its}), relative difference {err:.3e}", flush=True)
+    if precision == 0:
+        assert mon.pressure_iterations == its and its < 500
+        assert err < 1e-10
+    else:
+        assert abs(mon.pressure_iterations - its) <= 2
+        assert err < 1e-3
+
+
 def test_launch_shape_changes_between_calls(G, gpu_available, restore_tuning):
     """A shape set between two calls takes effect at the next (the captured
     iteration graph is captured again): 10 + 10 iterations under two shapes
