@@ -55,6 +55,15 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 TRAFFIC_BYTES = {"k_gs_wave_mom": 142.32e6, "k_gs_wave_turb": 135.59e6, "k_cg_a + k_cg_b": 32.08e6,
                  "line_cg_iteration_mixed": 494.1e6}
 
+# HBM bytes per launch of the fast path's FP64 stencil kernels over the whole
+# 64-instance batch, counted the same way (profiles/mixed_batch_64_fast_kernels.md),
+# keyed by the stage that launches each once per outer iteration. These kernels
+# read about their compulsory bytes, so the counted bytes stand for the
+# algorithmic ones.
+BATCH_KERNEL_BYTES = {"gradients": ("k_grad", 1169.6e6), "momentum_assembly": ("k_mom", 1678.0e6),
+                      "rhie_chow": ("k_rhiechow", 1355.8e6), "pressure_assembly": ("k_pressure", 1104.7e6),
+                      "sst_blending": ("k_sst_blend", 1737.7e6), "sst_assembly": ("k_sst_asm", 1483.2e6)}
+
 WORKLOAD = dict(section=(2, 4, 12), surface_points=385, wake_points=128, normal_points=257,
                 first_layer_height=1e-6, upstream=15.0, vertical=15.0, downstream=20.0,
                 reynolds=1e6, alpha=4.0, speed=50.0, density=1.225, intensity=0.001, viscosity_ratio=10.0)
@@ -314,6 +323,22 @@ def batch_roofline(fast_st, cells, stage_ms, calls, cg_cells, max_rows):
     return roofline
 
 
+def batch_roofline_others(stage_ms, calls):
+    """Rooflines of the FP64 stencil kernels on the whole 64-instance batch:
+    counted bytes per launch over the launch's HIP-event time in this run."""
+    out = {}
+    for stage, (kernel, nbytes) in BATCH_KERNEL_BYTES.items():
+        n = calls.get(stage, 0)
+        if not n:
+            continue
+        ms = stage_ms[stage] / n
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        out[kernel] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": nbytes, "stage": stage, "launch_ms": ms,
+                       "basis": "counted HBM bytes per launch over the stage's HIP-event time"}
+    return out
+
+
 def run_roofline(args):
     """Only the timed region of the headline: K outer iterations of the fast path
     on the 64-instance batch after W untimed, with the roofline of its line-CG
@@ -334,6 +359,7 @@ def run_roofline(args):
                       "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "cells": cells,
                       "fast_path": settings_summary(fast_st), "launch_tuning": dict(LAUNCH_TUNING),
                       "roofline": batch_roofline(fast_st, cells, stage_ms, calls, cg_cells, batch_max_rows(W, rows)),
+                      "roofline_others": None if args.only else batch_roofline_others(stage_ms, calls),
                       "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()}}), flush=True)
 
 
@@ -403,6 +429,7 @@ def run_headline(args):
                         else None},
             "latency": {"p50_s": fs["time_to_convergence_p50_s"], "p99_s": fs["time_to_convergence_p99_s"]},
             "roofline": roofline,
+            "roofline_others": batch_roofline_others(stage_ms, calls),
             "stage_ms_per_step": {k: v / args.steps for k, v in stage_ms.items()}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_batch(rows, plain["results"])
